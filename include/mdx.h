@@ -1,0 +1,238 @@
+/*
+ * mdx -- C ABI of the MI355X-native 2D binary Lennard-Jones MD step.
+ *
+ * The reference (Xiaode2333/MD_Simulation_CUDA) exposes this path only as C++
+ * classes that launch CUDA kernels on thrust vectors; it has no FFI.  These
+ * entry points are the plain-C boundary those classes are rebuilt on: opaque
+ * handles, POD parameters, int status codes with a thread-local message from
+ * mdx_last_error().  The header-compatible C++ classes (md_env.hpp,
+ * md_energy_minimizer.hpp) call only what is declared here.
+ *
+ * Threading: calls on distinct handles may run concurrently from different
+ * host threads; calls on one handle must not overlap.  Device memory belongs to
+ * the handle.  Every call that returns data to the host synchronises the
+ * handle's stream; stepping calls may return before the GPU finishes.
+ */
+#ifndef MDX_H
+#define MDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MDX_OK              0
+#define MDX_ERR_INVALID     1   /* bad argument or configuration            */
+#define MDX_ERR_HIP         2   /* HIP runtime error                         */
+#define MDX_ERR_CAPACITY    3   /* size above a stated maximum              */
+#define MDX_ERR_BOX         4   /* invalid box (where the reference aborts) */
+#define MDX_ERR_NONFINITE   5   /* non-finite state detected               */
+#define MDX_ERR_STATE       6   /* call not valid in the current state     */
+#define MDX_ERR_NODEVICE    7   /* no usable GPU                            */
+
+/* ---- data types -------------------------------------------------------- */
+
+/* Byte-identical to the reference's Particle (include/md_particle.hpp:31-38):
+ * double2 pos @0, vel @16, acc @32 (acc = F/m), int type @48, 4 pad bytes,
+ * double theta @56; 64 bytes.  This is also the PFRAME v2 frame payload. */
+typedef struct mdx_particle64 {
+    double pos[2];
+    double vel[2];
+    double acc[2];
+    int32_t type;
+    int32_t pad;
+    double theta;
+} mdx_particle64;
+
+typedef enum mdx_ensemble {
+    MDX_NVE = 0,      /* step_single_NVE            src/md_env.cu:1576-1637 */
+    MDX_NVT = 1,      /* step_single_nose_hoover    src/md_env.cu:1639-1728 */
+    MDX_NPH = 2,      /* step_single_NPH            src/md_env.cu:1795-1918 */
+    MDX_PISTON = 3    /* step_single_NPH_piston     src/md_env.cu:1920-2023 */
+} mdx_ensemble;
+
+/* Arithmetic of the pair pass. */
+typedef enum mdx_precision {
+    MDX_PREC_COMP32 = 0,   /* FP32 pair, FP32 two-sum (compensated) accumulation (default) */
+    MDX_PREC_ACC64 = 1,    /* FP32 pair, FP64 accumulation (the reference's form)         */
+    MDX_PREC_FP64 = 2      /* FP64 pair, FP64 accumulation                                  */
+} mdx_precision;
+
+/* mdx_step flags */
+#define MDX_STEP_MIDDLE_WRAP 1u   /* NVT only: middle_reflect_LG after the drift */
+
+/* Physical and numerical parameters (the subset of MDConfig the step reads,
+ * include/md_config.hpp:14-75). */
+typedef struct mdx_params {
+    double Lx, Ly;                     /* box_w_global, box_h_global */
+    double sigma_aa, sigma_bb, sigma_ab;
+    double eps_aa, eps_bb, eps_ab;
+    double mass_a, mass_b;             /* type 0 uses mass_a, every other type mass_b */
+    double cutoff;                     /* dimensionless: rc_ij = cutoff * sigma_ij */
+    double dt;
+    double T_target;
+    double Q;                          /* Nose-Hoover mass */
+    double barostat_mass;              /* NPH area-barostat / piston mass W */
+    double k_piston;                   /* piston wall stiffness */
+    double barostat_area_rate_init;
+    double barostat_height_min_ratio;
+    double barostat_height_max_ratio;
+    double devide_p;                   /* slab fraction for middle_reflect_LG */
+    double cell_scale;                 /* cell side >= cell_scale * cutoff * sigma_max (>= 1; 0 -> 1) */
+    int32_t precision;                 /* mdx_precision */
+    int32_t device;                    /* HIP device ordinal (-1: current) */
+} mdx_params;
+
+/* Observables.  Filled by mdx_observables according to mask. */
+#define MDX_OBS_K        1u   /* cal_total_K                    */
+#define MDX_OBS_U        2u   /* cal_total_U (+wall energy in piston mode) */
+#define MDX_OBS_P        4u   /* cal_instant_pressure (updates last_P)     */
+#define MDX_OBS_VIRIAL   8u   /* 1/2 sum_ordered r.F at the current positions */
+
+typedef struct mdx_obs {
+    double K, U, P, W;
+    double xi;          /* Nose-Hoover variable                 */
+    double p_A;         /* NPH area momentum                    */
+    double area_rate;   /* get_nph_area_rate(): p_A / W         */
+    double v_p;         /* get_nph_piston_velocity()            */
+    double F_p;         /* get_nph_piston_force()               */
+    double last_P;      /* get_last_instant_pressure()          */
+    double Lx, Ly;      /* get_Lx(), get_Ly()                   */
+    double t;           /* simulated time                       */
+    int32_t piston_active;
+    int32_t pad;
+} mdx_obs;
+
+/* FIRE parameters (include/md_energy_minimizer.hpp:11-25) plus the documented
+ * overlap policy of this implementation. */
+typedef struct mdx_fire_params {
+    double dt_init, dt_max, alpha0, f_inc, f_dec;
+    int32_t N_min;
+    int32_t energy_interval;   /* potential sampled every this many steps (reference: 1000) */
+    double max_step;           /* cap on |dt v| per step, 0 = uncapped (reference behaviour) */
+    double fp64_switch;        /* switch the pair pass to FP64 once max|a| < this (0 = never) */
+} mdx_fire_params;
+
+typedef struct mdx_fire_result {
+    int32_t steps;
+    int32_t converged;
+    double max_force;
+    double potential_energy;
+    int32_t n_energy;          /* samples produced (may exceed the caller's cap) */
+    int32_t pad;
+} mdx_fire_result;
+
+typedef struct mdx_sim mdx_sim;
+
+/* ---- lifecycle --------------------------------------------------------- */
+
+/* Replaces MDSimulation(cfg, comm) (src/md_env.cu:21-79) minus init_particles:
+ * uploads the frame, builds the cell list and computes the initial
+ * accelerations (cal_forces, src/md_env.cu:1511-1542). */
+int mdx_create(const mdx_params *params, const mdx_particle64 *frame, int64_t n,
+               mdx_sim **out);
+int mdx_destroy(mdx_sim *sim);
+
+/* Replaces the resume path (src/md_env.cu:81-164) and
+ * EnergyMinimizer::initialize_from_frame: replaces positions, velocities and
+ * types (n must equal the handle's n), rewraps, recomputes accelerations. */
+int mdx_set_state(mdx_sim *sim, const mdx_particle64 *frame, int64_t n);
+
+/* ---- stepping ---------------------------------------------------------- */
+
+/* nsteps steps of one ensemble (mdx_ensemble); p_ext is the external
+ * pressure for MDX_NPH / MDX_PISTON. */
+int mdx_step(mdx_sim *sim, int32_t kind, int32_t nsteps, double p_ext, uint32_t flags);
+
+/* FIRE minimisation of the handle's current frame
+ * (EnergyMinimizer::minimize_frame, src/md_energy_minimizer.cu:122-272).
+ * energy_steps/energy_trace (may be NULL when cap == 0) receive up to cap
+ * (step, U) samples.  The minimised frame stays in the handle. */
+int mdx_fire(mdx_sim *sim, const mdx_fire_params *fp, double force_tol, int32_t max_steps,
+             mdx_fire_result *result, int32_t *energy_steps, double *energy_trace,
+             int32_t cap);
+
+/* ---- observables and frames -------------------------------------------- */
+
+/* Getters of src/md_env.cu:2141-2406. */
+int mdx_observables(mdx_sim *sim, uint32_t mask, mdx_obs *out);
+/* cal_partial_U_lambda(epsilon_lambda), src/md_env.cu:2362-2406. */
+int mdx_partial_U_lambda(mdx_sim *sim, double eps_lambda, double *out);
+
+/* Frame in original particle order (sample_collect + get_host_particles,
+ * src/md_env.cu:2502-2514).  cap >= n required. */
+int mdx_collect(mdx_sim *sim, mdx_particle64 *out, int64_t cap);
+
+/* Recompute accelerations at the current positions without moving anything
+ * (cal_forces / cal_forces_piston LJ part); periodic_y selects the y boundary.
+ * When acc_out != NULL it receives 2n doubles (ax, ay per particle). */
+int mdx_compute_forces(mdx_sim *sim, int32_t periodic_y, double *acc_out);
+
+int mdx_get_n(const mdx_sim *sim, int64_t *n);
+int mdx_set_precision(mdx_sim *sim, int32_t precision);
+int mdx_set_dt(mdx_sim *sim, double dt);
+int mdx_sync(mdx_sim *sim);
+
+/* Canonical cell list of the current positions (for parity checks):
+ * keys[n], cell_start[ncell+1], sorted[n] (original indices, ascending within
+ * a cell).  Any array may be NULL; ncx/ncy always written. */
+int mdx_cell_list(mdx_sim *sim, int32_t periodic_y, int32_t *ncx, int32_t *ncy,
+                  uint32_t *keys, uint32_t *cell_start, uint32_t *sorted,
+                  int64_t cap_n, int64_t cap_cells);
+
+/* ---- kernel timing (HIP events on the handle's stream) ------------------ */
+typedef struct mdx_timing {
+    double force_ms;        /* summed duration of pair-pass launches timed */
+    int64_t force_launches;
+    double step_ms;         /* summed duration of whole timed steps        */
+    int64_t steps;
+} mdx_timing;
+
+/* enable != 0 starts recording (and clears counters); 0 stops. */
+int mdx_timing_enable(mdx_sim *sim, int32_t enable);
+int mdx_timing_read(mdx_sim *sim, mdx_timing *out);
+
+
+/* ---- initial frames (host only, no GPU) --------------------------------- */
+
+/* Inputs of MDSimulation::init_particles (src/md_env.cu:1276-1445). */
+typedef struct mdx_init_params {
+    int64_t n_particles_global;
+    int64_t n_particles_type0;
+    double box_w, box_h;
+    double T_init;
+    double mass_a, mass_b;
+    double devide_p;
+} mdx_init_params;
+
+/* The reference constructor's initial frame: phase-separated square grids,
+ * std::mt19937(12345) jitter, Maxwell-Boltzmann velocities with the centre of
+ * mass removed and K rescaled to (2N-2) T_init / 2, random theta.  Built with
+ * libstdc++'s distributions, as the reference is. */
+int mdx_init_particles(const mdx_init_params *ip, mdx_particle64 *out);
+
+/* Seeded synthetic frames (std::mt19937_64, own uniform and Box-Muller):
+ *   lattice: square box of side sqrt(n/density); ceil(sqrt(n))^2 grid, first n
+ *            sites row-major, uniform jitter +-jitter; n_type0 particles of
+ *            type 0 chosen by Fisher-Yates; Maxwell-Boltzmann velocities at
+ *            T (zero momentum, K = (2n-2) T / 2; T <= 0 gives v = 0).
+ *   uniform: the same but positions uniform in the box.
+ * box_out receives (Lx, Ly). */
+int mdx_gen_lattice(int64_t n, int64_t n_type0, double density, double jitter, double T,
+                    double mass_a, double mass_b, uint64_t seed, mdx_particle64 *out,
+                    double *box_out);
+int mdx_gen_uniform(int64_t n, int64_t n_type0, double density, double T, double mass_a,
+                    double mass_b, uint64_t seed, mdx_particle64 *out, double *box_out);
+
+/* ---- misc ---------------------------------------------------------------- */
+const char *mdx_last_error(void);
+const char *mdx_version(void);
+int mdx_device_count(int32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDX_H */

@@ -1,0 +1,294 @@
+"""ctypes binding of the C ABI in include/mdx.h (libmdx.so).
+
+The library is loaded from md_simulation_cuda_amd/_build/.  There is no
+fallback: if the built HIP library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+BUILD_DIR = Path(__file__).resolve().parent / "_build"
+LIB_PATH = BUILD_DIR / "libmdx.so"
+
+MDX_OK = 0
+MDX_ERR_INVALID = 1
+MDX_ERR_HIP = 2
+MDX_ERR_CAPACITY = 3
+MDX_ERR_BOX = 4
+MDX_ERR_NONFINITE = 5
+MDX_ERR_STATE = 6
+MDX_ERR_NODEVICE = 7
+
+NVE, NVT, NPH, PISTON = 0, 1, 2, 3
+PREC_COMP32, PREC_ACC64, PREC_FP64 = 0, 1, 2
+STEP_MIDDLE_WRAP = 1
+OBS_K, OBS_U, OBS_P, OBS_VIRIAL = 1, 2, 4, 8
+
+PARTICLE_DTYPE = np.dtype(
+    {
+        "names": ["pos", "vel", "acc", "type", "pad", "theta"],
+        "formats": [("<f8", 2), ("<f8", 2), ("<f8", 2), "<i4", "<i4", "<f8"],
+        "offsets": [0, 16, 32, 48, 52, 56],
+        "itemsize": 64,
+    }
+)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("Lx", C.c_double), ("Ly", C.c_double),
+        ("sigma_aa", C.c_double), ("sigma_bb", C.c_double), ("sigma_ab", C.c_double),
+        ("eps_aa", C.c_double), ("eps_bb", C.c_double), ("eps_ab", C.c_double),
+        ("mass_a", C.c_double), ("mass_b", C.c_double),
+        ("cutoff", C.c_double), ("dt", C.c_double), ("T_target", C.c_double), ("Q", C.c_double),
+        ("barostat_mass", C.c_double), ("k_piston", C.c_double),
+        ("barostat_area_rate_init", C.c_double),
+        ("barostat_height_min_ratio", C.c_double), ("barostat_height_max_ratio", C.c_double),
+        ("devide_p", C.c_double), ("cell_scale", C.c_double),
+        ("precision", C.c_int32), ("device", C.c_int32),
+    ]
+
+
+class Obs(C.Structure):
+    _fields_ = [
+        ("K", C.c_double), ("U", C.c_double), ("P", C.c_double), ("W", C.c_double),
+        ("xi", C.c_double), ("p_A", C.c_double), ("area_rate", C.c_double),
+        ("v_p", C.c_double), ("F_p", C.c_double), ("last_P", C.c_double),
+        ("Lx", C.c_double), ("Ly", C.c_double), ("t", C.c_double),
+        ("piston_active", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+class FireParams(C.Structure):
+    _fields_ = [
+        ("dt_init", C.c_double), ("dt_max", C.c_double), ("alpha0", C.c_double),
+        ("f_inc", C.c_double), ("f_dec", C.c_double), ("N_min", C.c_int32),
+        ("energy_interval", C.c_int32), ("max_step", C.c_double), ("fp64_switch", C.c_double),
+    ]
+
+
+class FireResult(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int32), ("converged", C.c_int32), ("max_force", C.c_double),
+        ("potential_energy", C.c_double), ("n_energy", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+class Timing(C.Structure):
+    _fields_ = [("force_ms", C.c_double), ("force_launches", C.c_int64),
+                ("step_ms", C.c_double), ("steps", C.c_int64)]
+
+
+class InitParams(C.Structure):
+    _fields_ = [
+        ("n_particles_global", C.c_int64), ("n_particles_type0", C.c_int64),
+        ("box_w", C.c_double), ("box_h", C.c_double), ("T_init", C.c_double),
+        ("mass_a", C.c_double), ("mass_b", C.c_double), ("devide_p", C.c_double),
+    ]
+
+
+_P = C.c_void_p
+_SIGNATURES = {
+    "mdx_create": (C.c_int, [C.POINTER(Params), _P, C.c_int64, C.POINTER(_P)]),
+    "mdx_destroy": (C.c_int, [_P]),
+    "mdx_set_state": (C.c_int, [_P, _P, C.c_int64]),
+    "mdx_step": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, C.c_uint32]),
+    "mdx_fire": (C.c_int, [_P, C.POINTER(FireParams), C.c_double, C.c_int32, C.POINTER(FireResult),
+                           _P, _P, C.c_int32]),
+    "mdx_observables": (C.c_int, [_P, C.c_uint32, C.POINTER(Obs)]),
+    "mdx_partial_U_lambda": (C.c_int, [_P, C.c_double, C.POINTER(C.c_double)]),
+    "mdx_collect": (C.c_int, [_P, _P, C.c_int64]),
+    "mdx_compute_forces": (C.c_int, [_P, C.c_int32, _P]),
+    "mdx_get_n": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "mdx_set_precision": (C.c_int, [_P, C.c_int32]),
+    "mdx_set_dt": (C.c_int, [_P, C.c_double]),
+    "mdx_sync": (C.c_int, [_P]),
+    "mdx_cell_list": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P, _P, _P,
+                                C.c_int64, C.c_int64]),
+    "mdx_timing_enable": (C.c_int, [_P, C.c_int32]),
+    "mdx_timing_read": (C.c_int, [_P, C.POINTER(Timing)]),
+    "mdx_last_error": (C.c_char_p, []),
+    "mdx_version": (C.c_char_p, []),
+    "mdx_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
+    "mdx_init_particles": (C.c_int, [C.POINTER(InitParams), _P]),
+    "mdx_gen_lattice": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_uint64, _P, _P]),
+    "mdx_gen_uniform": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.c_uint64, _P, _P]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libmdx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"libmdx.so not built at {LIB_PATH}; run python -m md_simulation_cuda_amd.build")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class MdxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mdx error {code}: {msg}")
+        self.code = code
+
+
+def check(rc: int) -> None:
+    if rc != MDX_OK:
+        raise MdxError(rc, load().mdx_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def device_count() -> int:
+    c = C.c_int32(0)
+    load().mdx_device_count(C.byref(c))
+    return int(c.value)
+
+
+def empty_frame(n: int) -> np.ndarray:
+    return np.zeros(n, dtype=PARTICLE_DTYPE)
+
+
+def init_particles(n: int, n_type0: int, box_w: float, box_h: float, T_init: float,
+                   mass_a: float = 1.0, mass_b: float = 1.0, devide_p: float = 0.5) -> np.ndarray:
+    ip = InitParams(n, n_type0, box_w, box_h, T_init, mass_a, mass_b, devide_p)
+    out = empty_frame(n)
+    check(load().mdx_init_particles(C.byref(ip), _ptr(out)))
+    return out
+
+
+def gen_lattice(n: int, n_type0: int, density: float, jitter: float, T: float, seed: int,
+                mass_a: float = 1.0, mass_b: float = 1.0):
+    out = empty_frame(n)
+    box = np.zeros(2)
+    check(load().mdx_gen_lattice(n, n_type0, density, jitter, T, mass_a, mass_b, seed, _ptr(out), _ptr(box)))
+    return out, float(box[0]), float(box[1])
+
+
+def gen_uniform(n: int, n_type0: int, density: float, T: float, seed: int,
+                mass_a: float = 1.0, mass_b: float = 1.0):
+    out = empty_frame(n)
+    box = np.zeros(2)
+    check(load().mdx_gen_uniform(n, n_type0, density, T, mass_a, mass_b, seed, _ptr(out), _ptr(box)))
+    return out, float(box[0]), float(box[1])
+
+
+def make_params(Lx: float, Ly: float, **kw) -> Params:
+    """Defaults follow MDConfig (include/md_config.hpp:22-55)."""
+    d = dict(sigma_aa=1.0, sigma_bb=1.0, sigma_ab=1.0, eps_aa=1.0, eps_bb=1.0, eps_ab=0.25,
+             mass_a=1.0, mass_b=1.0, cutoff=2.5, dt=1e-3, T_target=1.0, Q=100.0,
+             barostat_mass=1.0e4, k_piston=100.0, barostat_area_rate_init=0.0,
+             barostat_height_min_ratio=0.6, barostat_height_max_ratio=1.4, devide_p=0.5,
+             cell_scale=1.0, precision=PREC_COMP32, device=-1)
+    d.update(kw)
+    return Params(Lx=Lx, Ly=Ly, **d)
+
+
+class Simulation:
+    """Thin owner of one mdx_sim handle."""
+
+    def __init__(self, params: Params, frame: np.ndarray):
+        self._lib = load()
+        frame = np.ascontiguousarray(frame, dtype=PARTICLE_DTYPE)
+        self.n = len(frame)
+        self.params = params
+        h = C.c_void_p()
+        check(self._lib.mdx_create(C.byref(params), _ptr(frame), self.n, C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.mdx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def step(self, kind: int, nsteps: int = 1, p_ext: float = 0.0, flags: int = 0) -> None:
+        check(self._lib.mdx_step(self._h, kind, nsteps, p_ext, flags))
+
+    def sync(self) -> None:
+        check(self._lib.mdx_sync(self._h))
+
+    def collect(self) -> np.ndarray:
+        out = empty_frame(self.n)
+        check(self._lib.mdx_collect(self._h, _ptr(out), self.n))
+        return out
+
+    def set_state(self, frame: np.ndarray) -> None:
+        frame = np.ascontiguousarray(frame, dtype=PARTICLE_DTYPE)
+        check(self._lib.mdx_set_state(self._h, _ptr(frame), len(frame)))
+
+    def observables(self, mask: int = OBS_K | OBS_U | OBS_P | OBS_VIRIAL) -> Obs:
+        o = Obs()
+        check(self._lib.mdx_observables(self._h, mask, C.byref(o)))
+        return o
+
+    def partial_U_lambda(self, eps: float) -> float:
+        v = C.c_double()
+        check(self._lib.mdx_partial_U_lambda(self._h, eps, C.byref(v)))
+        return float(v.value)
+
+    def compute_forces(self, periodic_y: bool = True) -> np.ndarray:
+        acc = np.zeros((self.n, 2))
+        check(self._lib.mdx_compute_forces(self._h, 1 if periodic_y else 0, _ptr(acc)))
+        return acc
+
+    def set_precision(self, prec: int) -> None:
+        check(self._lib.mdx_set_precision(self._h, prec))
+
+    def cell_list(self, periodic_y: bool = True):
+        ncx, ncy = C.c_int32(), C.c_int32()
+        check(self._lib.mdx_cell_list(self._h, 1 if periodic_y else 0, C.byref(ncx), C.byref(ncy),
+                                      None, None, None, 0, 0))
+        ncell = ncx.value * ncy.value
+        keys = np.zeros(self.n, np.uint32)
+        start = np.zeros(ncell + 1, np.uint32)
+        srt = np.zeros(self.n, np.uint32)
+        check(self._lib.mdx_cell_list(self._h, 1 if periodic_y else 0, C.byref(ncx), C.byref(ncy),
+                                      _ptr(keys), _ptr(start), _ptr(srt), self.n, ncell + 1))
+        return ncx.value, ncy.value, keys, start, srt
+
+    def fire(self, fp: FireParams, force_tol: float, max_steps: int, cap: int = 64):
+        res = FireResult()
+        es = np.zeros(cap, np.int32)
+        et = np.zeros(cap, np.float64)
+        check(self._lib.mdx_fire(self._h, C.byref(fp), force_tol, max_steps, C.byref(res),
+                                 _ptr(es), _ptr(et), cap))
+        k = min(res.n_energy, cap)
+        return res, es[:k].copy(), et[:k].copy()
+
+    def timing_enable(self, on: bool) -> None:
+        check(self._lib.mdx_timing_enable(self._h, 1 if on else 0))
+
+    def timing_read(self) -> Timing:
+        t = Timing()
+        check(self._lib.mdx_timing_read(self._h, C.byref(t)))
+        return t

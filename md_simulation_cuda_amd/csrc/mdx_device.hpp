@@ -1,0 +1,149 @@
+// Device-side building blocks of the gfx950 LJ step: pair parameters, the
+// canonical cell coordinate, compensated sums and deterministic grid
+// reductions.  Included by mdx_kernels.hip only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdx {
+
+constexpr int kBlock = 256;          // threads per workgroup for every kernel
+constexpr int kWaves = kBlock / 64;  // wave64
+constexpr int kScanChunk = 1024;     // cells per workgroup in the two-pass scan
+
+// Type class of a particle: 0 (type 0), 1 (type 1), 2 (any other type).  Pair
+// parameters follow src/md_cuda_common.cu:220-229: (0,0)->AA, (1,1)->BB,
+// everything else -> AB, i.e. index = (ci == cj) ? ci : 2 with AB stored at 2.
+__host__ __device__ inline int pair_index(int ci, int cj) { return ci == cj ? ci : 2; }
+
+// Per type-pair constants, index 0 = AA, 1 = BB, 2 = AB.
+struct PairTable {
+    float rc2f[3], sig2f[3], eps24f[3], eps4f[3];
+    double rc2[3], sig2[3], eps24[3], eps4[3];
+    double vc[3];          // eps * 4 (c^-12 - c^-6): energy shift
+    double inv_m[3];       // 1 / mass by class (class 0 -> mass_a, 1,2 -> mass_b)
+    double m[3];
+    float r2_guard;        // FP32 pairs closer than this are evaluated in FP64
+    float pad;
+};
+
+// Periodic wrap of include/md_cuda_common.hpp:20-29.
+__host__ __device__ inline double pbc_wrap(double x, double L) {
+    if (L <= 0.0) return x;
+    x = fmod(x, L);
+    if (x < 0.0) x += L;
+    return x;
+}
+
+// Canonical cell coordinate: floor(x * n / L) from the FP64 position,
+// clamped to [0, n-1].  The oracle evaluates the identical expression.
+__host__ __device__ inline int cell_coord(double x, double L, int n) {
+    const double q = (x * static_cast<double>(n)) / L;
+    if (!(q >= 0.0)) return 0;
+    if (q >= static_cast<double>(n)) return n - 1;
+    return static_cast<int>(q);
+}
+
+// Geometry of one cell grid.
+struct Grid {
+    double Lx, Ly;
+    double w, h;       // cell sides
+    float Lxf, Lyf, wf, hf;
+    int ncx, ncy;
+    int periodic_y;
+    int small_x;       // cell side < rc_max in x: per-pair minimum image
+    int small_y;
+    int pad;
+};
+
+// Error-free transformation a + b = s + e (Knuth TwoSum), used to carry the
+// rounding error of the FP32 force sums.
+struct Comp32 {
+    float s, c;
+    __device__ inline void add(float t) {
+        const float snew = s + t;
+        const float bp = snew - s;
+        const float err = (s - (snew - bp)) + (t - bp);
+        s = snew;
+        c += err;
+    }
+    __device__ inline double value() const { return static_cast<double>(s) + static_cast<double>(c); }
+};
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Deterministic grid-wide reduction of NV doubles: each workgroup reduces in
+// a fixed tree and writes one partial; the workgroup that arrives last (atomic
+// ticket) combines the partials in index order.  Component k is a max when
+// bit k of MAXMASK is set, a sum otherwise.  Returns true in every thread of
+// the finishing workgroup, which then holds the totals in out[].  The ticket
+// counter is reset to zero by that workgroup, so it is ready for the next
+// launch.  Requires blockDim.x == kBlock.
+template <int NV, unsigned MAXMASK>
+__device__ bool grid_reduce(double (&v)[NV], double *partials, unsigned *ticket, double (&out)[NV]) {
+    __shared__ double sh[NV][kWaves];
+    __shared__ int is_last;
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int nb = gridDim.x;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double r = ((MAXMASK >> k) & 1u) ? wave_max(v[k]) : wave_sum(v[k]);
+        if (lane == 0) sh[k][wid] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double b = sh[k][0];
+            for (int w = 1; w < kWaves; ++w) b = ((MAXMASK >> k) & 1u) ? fmax(b, sh[k][w]) : b + sh[k][w];
+            partials[k * nb + blockIdx.x] = b;
+        }
+        __threadfence();
+        const unsigned t = atomicAdd(ticket, 1u);
+        is_last = (t == static_cast<unsigned>(nb - 1)) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    const volatile double *vp = partials;
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const bool mx = ((MAXMASK >> k) & 1u) != 0;
+        double a = mx ? -1.0e308 : 0.0;
+        for (int i = threadIdx.x; i < nb; i += kBlock) {
+            const double p = vp[k * nb + i];
+            a = mx ? fmax(a, p) : a + p;
+        }
+        acc[k] = a;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double r = ((MAXMASK >> k) & 1u) ? wave_max(acc[k]) : wave_sum(acc[k]);
+        if (lane == 0) sh[k][wid] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double b = sh[k][0];
+        for (int w = 1; w < kWaves; ++w) b = ((MAXMASK >> k) & 1u) ? fmax(b, sh[k][w]) : b + sh[k][w];
+        out[k] = b;
+    }
+    if (threadIdx.x == 0) *ticket = 0u;
+    return true;
+}
+
+}  // namespace mdx

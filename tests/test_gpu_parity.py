@@ -1,0 +1,263 @@
+"""Parity of the HIP library (through the C ABI) with the FP64 oracle.
+
+Bars: the cell list is bit-exact; the FP64 pair form reproduces the oracle's
+forces and NVE trajectories bit for bit; the FP32 pair forms stay within the
+stated force tolerance; integrator scalars (xi, box, p_A, piston) follow the
+oracle to round-off.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from md_simulation_cuda_amd import abi
+from oracle import oracle as O
+from tests.helpers import lattice_frame, near_cutoff_mask, uniform_frame
+
+pytestmark = pytest.mark.gpu
+
+# FP32 pair arithmetic: max_i |dF_i| / rms|F| over particles without a pair in
+# the cutoff rounding band (|r^2 - rc^2| < 1e-6 rc^2), and rms_i |dF_i| / rms|F|.
+FP32_FORCE_MAX_REL = 1e-5
+FP32_FORCE_RMS_REL = 1e-6
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def both(frame, Lx, Ly, prec, **kw):
+    gp = abi.make_params(Lx, Ly, precision=prec, **kw)
+    op = O.make_params(Lx, Ly, **kw)
+    return abi.Simulation(gp, frame), O.OracleSim(op, frame)
+
+
+CASES = {
+    "c1_square_4096": lambda: lattice_frame(4096, 0.8, seed=1001, jitter=0.05, T=0.7),
+    "slab_2to1_3000": lambda: lattice_frame(3000, 0.75, seed=77, jitter=0.3, T=0.6, aspect=2.0),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_cell_list_bit_exact(case):
+    f, Lx, Ly = CASES[case]()
+    say(f"cell list {case} n={len(f)}")
+    g, o = both(f, Lx, Ly, abi.PREC_COMP32, eps_ab=0.5)
+    for py in (True, False):
+        a = g.cell_list(py)
+        b = o.cell_list(py)
+        assert a[0] == b[0] and a[1] == b[1]
+        for k in (2, 3, 4):
+            assert np.array_equal(a[k], b[k]), (case, py, k)
+    g.close()
+
+
+def test_cell_list_small_and_open_boxes():
+    for (Lx, Ly, py) in [(6.0, 20.0, True), (7.4, 7.4, True), (30.0, 14.0, False)]:
+        f = uniform_frame(int(0.7 * Lx * Ly), Lx, Ly, seed=5)
+        if not py:
+            f["pos"][:20, 1] += np.linspace(-1, 1, 20)
+        say(f"cell list box {Lx}x{Ly} py={py}")
+        g, o = both(f, Lx, Ly, abi.PREC_FP64)
+        a, b = g.cell_list(py), o.cell_list(py)
+        for k in range(5):
+            assert np.array_equal(np.asarray(a[k]), np.asarray(b[k]))
+        g.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fp64_forces_bit_exact(case):
+    f, Lx, Ly = CASES[case]()
+    say(f"fp64 forces {case}")
+    g, o = both(f, Lx, Ly, abi.PREC_FP64, eps_ab=0.5)
+    acc_g = g.compute_forces(True)
+    acc_o, _, _, _ = o.pair_pass(True)
+    assert np.array_equal(acc_g, acc_o)
+    g.close()
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_COMP32, abi.PREC_ACC64])
+@pytest.mark.parametrize("case", list(CASES))
+def test_fp32_forces_within_tolerance(case, prec):
+    f, Lx, Ly = CASES[case]()
+    say(f"fp32 forces {case} prec={prec}")
+    g, o = both(f, Lx, Ly, prec, eps_ab=0.5)
+    acc_g = g.compute_forces(True)
+    acc_o, _, _, _ = o.pair_pass(True)
+    rms = np.sqrt((acc_o ** 2).sum(1).mean())
+    err = np.sqrt(((acc_g - acc_o) ** 2).sum(1))
+    band = near_cutoff_mask(f, Lx, Ly, 2.5, (1.0, 1.0, 1.0))
+    max_rel = err[~band].max() / rms
+    rms_rel = np.sqrt((err ** 2).mean()) / rms
+    say(f"  max_rel(excl band {band.sum()})={max_rel:.3e} rms_rel={rms_rel:.3e} max_all={err.max() / rms:.3e}")
+    assert max_rel <= FP32_FORCE_MAX_REL
+    assert rms_rel <= FP32_FORCE_RMS_REL
+    g.close()
+
+
+@pytest.mark.parametrize("prec,rel", [(abi.PREC_FP64, 1e-12), (abi.PREC_ACC64, 1e-6), (abi.PREC_COMP32, 1e-6)])
+def test_observables(prec, rel):
+    f, Lx, Ly = CASES["slab_2to1_3000"]()
+    say(f"observables prec={prec}")
+    g, o = both(f, Lx, Ly, prec, eps_ab=0.5)
+    ob = g.observables()
+    assert ob.K == pytest.approx(o.K(), rel=1e-13)
+    assert ob.U == pytest.approx(o.U(), rel=rel)
+    assert ob.P == pytest.approx(o.pressure(), rel=rel)
+    assert g.partial_U_lambda(0.37) == pytest.approx(o.partial_U_lambda(0.37), rel=max(rel, 1e-12), abs=1e-9)
+    g.close()
+
+
+def _frames_equal(a, b):
+    return np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"])
+
+
+def test_fp64_nve_trajectory_bit_exact():
+    f, Lx, Ly = CASES["c1_square_4096"]()
+    say("fp64 NVE 200 steps")
+    g, o = both(f, Lx, Ly, abi.PREC_FP64, eps_ab=0.5, dt=0.005)
+    o.compute_forces()
+    g.step(abi.NVE, 200)
+    o.step_nve(200)
+    a, b = g.collect(), o.frame()
+    assert _frames_equal(a, b)
+    assert np.array_equal(a["acc"], b["acc"])
+    assert np.array_equal(a["type"], b["type"]) and np.array_equal(a["theta"], f["theta"])
+    g.close()
+
+
+def test_fp64_nvt_trajectory():
+    f, Lx, Ly = CASES["c1_square_4096"]()
+    say("fp64 NVT 200 steps")
+    g, o = both(f, Lx, Ly, abi.PREC_FP64, eps_ab=0.5, dt=0.005, T_target=0.9, Q=10.0)
+    o.compute_forces()
+    g.step(abi.NVT, 200)
+    o.step_nvt(200)
+    a, b = g.collect(), o.frame()
+    assert np.abs(a["pos"] - b["pos"]).max() < 1e-9
+    assert np.abs(a["vel"] - b["vel"]).max() < 1e-9
+    assert g.observables().xi == pytest.approx(o.obs().xi, rel=1e-9, abs=1e-12)
+    g.close()
+
+
+def test_fp64_nvt_middle_wrap():
+    f, Lx, Ly = lattice_frame(1200, 0.75, seed=5, T=0.8, aspect=3.0)
+    say("fp64 NVT middle wrap")
+    g, o = both(f, Lx, Ly, abi.PREC_FP64, eps_ab=0.5, dt=0.005, T_target=0.8, Q=10.0, devide_p=0.4)
+    o.compute_forces()
+    g.step(abi.NVT, 50, flags=abi.STEP_MIDDLE_WRAP)
+    o.step_nvt(50, middle_wrap=True)
+    a, b = g.collect(), o.frame()
+    assert np.abs(a["pos"] - b["pos"]).max() < 1e-9
+    g.close()
+
+
+def test_fp64_nvt_then_nph_like_reference_smoke():
+    # shape of the reference's tests/nph/nph_smoke.cpp: NVT, then NPH at the
+    # averaged instantaneous pressure
+    f, Lx, Ly = lattice_frame(512, 0.32, seed=2, T=0.9, aspect=4.0)
+    say("fp64 NVT->NPH")
+    kw = dict(eps_ab=0.25, dt=0.001, T_target=0.9, Q=100.0, barostat_mass=50.0)
+    g, o = both(f, Lx, Ly, abi.PREC_FP64, **kw)
+    o.compute_forces()
+    pg = po = 0.0
+    for _ in range(50):
+        g.step(abi.NVT, 1)
+        o.step_nvt(1)
+        pg += g.observables(abi.OBS_P).P
+        po += o.pressure()
+    assert pg == pytest.approx(po, rel=1e-9)
+    g.step(abi.NPH, 100, p_ext=pg / 50 * 1.5)
+    o.step_nph(100, po / 50 * 1.5)
+    og, oo = g.observables(0), o.obs()
+    assert og.Lx == pytest.approx(oo.Lx, rel=1e-11)
+    assert og.Ly == pytest.approx(oo.Ly, rel=1e-11)
+    assert og.p_A == pytest.approx(oo.p_A, rel=1e-8, abs=1e-12)
+    assert og.last_P == pytest.approx(oo.last_P, rel=1e-9)
+    a, b = g.collect(), o.frame()
+    assert np.abs(a["pos"] - b["pos"]).max() < 1e-8
+    g.close()
+
+
+def test_fp64_piston():
+    f, Lx, Ly = lattice_frame(512, 0.32, seed=3, T=0.9, aspect=4.0)
+    say("fp64 piston")
+    kw = dict(eps_ab=0.25, dt=0.001, T_target=0.9, barostat_mass=50.0, k_piston=100.0)
+    g, o = both(f, Lx, Ly, abi.PREC_FP64, **kw)
+    o.compute_forces()
+    g.step(abi.NVT, 20)
+    o.step_nvt(20)
+    g.step(abi.PISTON, 100, p_ext=0.4)
+    o.step_piston(100, 0.4)
+    og, oo = g.observables(0), o.obs()
+    assert og.piston_active == 1
+    assert og.Ly == pytest.approx(oo.Ly, rel=1e-11)
+    assert og.v_p == pytest.approx(oo.v_p, rel=1e-8, abs=1e-12)
+    assert og.F_p == pytest.approx(oo.F_p, rel=1e-8, abs=1e-12)
+    assert og.last_P == pytest.approx(oo.last_P, rel=1e-8, abs=1e-12)
+    ob = g.observables(abi.OBS_U | abi.OBS_P)
+    assert ob.U == pytest.approx(o.U(), rel=1e-11)
+    assert ob.P == pytest.approx(o.pressure(), rel=1e-11)
+    a, b = g.collect(), o.frame()
+    assert np.abs(a["pos"] - b["pos"]).max() < 1e-8
+    g.close()
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_ACC64])
+def test_fire_two_particle_reference_case(prec):
+    f = np.zeros(2, abi.PARTICLE_DTYPE)
+    f["pos"][0] = (1.0, 1.0)
+    f["pos"][1] = (3.0, 1.0)
+    say(f"fire 2-particle prec={prec}")
+    g, o = both(f, 10.0, 10.0, prec, eps_ab=1.0, dt=1e-3)
+    fp = abi.FireParams(1e-3, 5e-3, 0.1, 1.1, 0.5, 5, 1000, 0.0, 0.0)
+    res, es, et = g.fire(fp, 1e-3, 20000)
+    ro, eso, eto = o.fire(dt_init=1e-3, dt_max=5e-3, force_tol=1e-3, max_steps=20000)
+    assert res.converged == 1 and ro.converged == 1
+    if prec == abi.PREC_FP64:
+        assert res.steps == ro.steps
+        assert np.array_equal(es, eso)
+        assert np.allclose(et, eto, rtol=1e-12)
+    else:
+        assert abs(res.steps - ro.steps) <= 2
+    a, b = g.collect(), o.frame()
+    ra = abs(a["pos"][1, 0] - a["pos"][0, 0])
+    assert ra == pytest.approx(2 ** (1 / 6), abs=1e-4)
+    g.close()
+
+
+def test_fire_small_random_start_fp64():
+    Lx = Ly = 12.0
+    f = uniform_frame(100, Lx, Ly, seed=31)
+    say("fire random 100 fp64")
+    g, o = both(f, Lx, Ly, abi.PREC_FP64, eps_ab=0.5, dt=0.005)
+    fp = abi.FireParams(0.005, 0.05, 0.1, 1.1, 0.5, 5, 1000, 0.1, 0.0)
+    res, es, et = g.fire(fp, 1e-6, 30)
+    ro, eso, eto = o.fire(dt_init=0.005, dt_max=0.05, max_step=0.1, force_tol=1e-6, max_steps=30)
+    a, b = g.collect(), o.frame()
+    # 30 steps from an overlapping start: identical arithmetic, identical path
+    assert np.array_equal(a["pos"], b["pos"])
+    assert res.max_force == ro.max_force
+    g.close()
+
+
+def test_set_state_and_errors():
+    f, Lx, Ly = CASES["c1_square_4096"]()
+    say("set_state / errors")
+    g = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_FP64, eps_ab=0.5), f)
+    h = f.copy()
+    h["pos"][:, 0] = np.mod(h["pos"][:, 0] + 1.0, Lx)
+    g.set_state(h)
+    o = O.OracleSim(O.make_params(Lx, Ly, eps_ab=0.5), h)
+    acc_o, _, _, _ = o.pair_pass(True)
+    assert np.array_equal(g.collect()["acc"], acc_o)
+    with pytest.raises(abi.MdxError) as e:
+        g.set_state(h[:10])
+    assert e.value.code == abi.MDX_ERR_INVALID
+    with pytest.raises(abi.MdxError):
+        g.step(7, 1)
+    g.close()
+    with pytest.raises(abi.MdxError) as e:
+        abi.Simulation(abi.make_params(-1.0, 5.0), f)
+    assert e.value.code == abi.MDX_ERR_BOX
