@@ -1,0 +1,77 @@
+// MDSimulation: the environment API of the reference (include/md_env.hpp:51-263
+// there) for the 2D binary-LJ step, on one GPU through the mdx C ABI.
+//
+// The MPI communicator of the reference's constructors is accepted and
+// ignored (one GPU holds the whole periodic box).  Triangulation, interface,
+// profile and ABP members of the reference are not part of this library.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "md_common.hpp"
+#include "md_config.hpp"
+#include "md_particle.hpp"
+#include "mdx.h"
+
+class MDSimulation {
+public:
+    explicit MDSimulation(MDConfigManager config_manager);
+    MDSimulation(MDConfigManager config_manager, const std::string &filename, int step);
+    // Reference signatures with a communicator (src/md_env.cu:21, :81).
+    template <class Comm>
+    MDSimulation(MDConfigManager config_manager, Comm) : MDSimulation(std::move(config_manager)) {}
+    template <class Comm>
+    MDSimulation(MDConfigManager config_manager, Comm, const std::string &filename, int step)
+        : MDSimulation(std::move(config_manager), filename, step) {}
+    ~MDSimulation();
+    MDSimulation(const MDSimulation &) = delete;
+    MDSimulation &operator=(const MDSimulation &) = delete;
+
+    double t = 0.0;   // evolution time
+
+    void save_env(const std::string &filename, const int step);
+    void get_host_particles(std::vector<Particle> &out) const;
+
+    double cal_total_K();
+    double cal_total_U();
+    double cal_partial_U_lambda(double epsilon_lambda);
+    double get_Lx() const { return cfg_manager.config.box_w_global; }
+    double get_Ly() const { return cfg_manager.config.box_h_global; }
+
+    void step_single();   // README name of step_single_NVE
+    void step_single_NVE();
+    void step_single_nose_hoover(bool do_middle_wrap = false);
+    void step_single_NPH(double pressure_ext);
+    void step_single_NPH_piston(double pressure_ext);
+    double cal_instant_pressure();
+    double get_last_instant_pressure() const { return last_instant_pressure; }
+    double get_nph_area_rate() const;
+    double get_nph_piston_velocity() const { return nph_piston_velocity; }
+    double get_nph_piston_force() const { return nph_piston_force; }
+
+    void sample_collect();
+
+    // Extensions of this library.
+    void step_many(int kind, int nsteps, double pressure_ext = 0.0, bool do_middle_wrap = false);
+    void set_precision(int precision);   // mdx_precision
+    mdx_sim *handle() { return sim_; }
+    const MDConfigManager &config() const { return cfg_manager; }
+
+private:
+    void create_from_frame(const std::vector<Particle> &frame);
+    void sync_scalars();
+    void check(int rc, const char *where, bool throws = false) const;
+
+    MDConfigManager cfg_manager;
+    mdx_sim *sim_ = nullptr;
+    std::unique_ptr<FileWriter> particle_writer;
+    std::vector<Particle> h_particles;
+    double last_instant_pressure = 0.0;
+    double nph_area_momentum = 0.0;
+    double nph_piston_velocity = 0.0;
+    double nph_piston_force = 0.0;
+    int precision_ = MDX_PREC_COMP32;
+};

@@ -82,7 +82,7 @@ def build_cpp_tests() -> list[Path]:
     for src in sorted(tdir.glob("*.cpp")):
         exe = od / src.stem
         _run([CXX, *CXX_FLAGS, str(src), "-o", str(exe), f"-L{OUT}", "-lmd_env", "-lmdx", "-lz",
-              f"-Wl,-rpath,{OUT}"])
+              "-Wl,-rpath,$ORIGIN/.."])
         outs.append(exe)
     return outs
 

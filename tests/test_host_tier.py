@@ -1,0 +1,135 @@
+"""CPU tests of the host side: C ABI exports, the config JSON against the
+reference's golden file, PFRAME v2 bytes checked independently with Python's
+struct and zlib, and the initial-frame generators."""
+import json
+import re
+import struct
+import subprocess
+import zlib
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from md_simulation_cuda_amd import abi
+
+ROOT = Path(__file__).resolve().parent.parent
+CPP = ROOT / "md_simulation_cuda_amd" / "_build" / "cpp_tests"
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def declared_symbols():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"\b(mdx_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.load()
+    names = declared_symbols()
+    assert len(names) >= 20
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert abi.load().mdx_version().startswith(b"mdx")
+
+
+def test_particle_record_layout():
+    assert abi.PARTICLE_DTYPE.itemsize == 64
+    assert abi.PARTICLE_DTYPE.fields["type"][1] == 48
+    assert abi.PARTICLE_DTYPE.fields["theta"][1] == 56
+
+
+def test_errors_without_device_are_reported():
+    f = abi.empty_frame(4)
+    with pytest.raises(abi.MdxError):
+        abi.Simulation(abi.make_params(-1.0, 1.0), f)   # invalid box before any device use
+
+
+def run(exe, *args):
+    return subprocess.run([str(CPP / exe), *map(str, args)], capture_output=True, text=True, timeout=120)
+
+
+def test_config_round_trip_matches_reference_golden(tmp_path):
+    out = tmp_path / "cfg.json"
+    r = run("test_config", out)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ours = out.read_text()
+    golden = (GOLDEN / "test_config_output.json").read_text()
+    assert json.loads(ours) == json.loads(golden)
+    # dump(4) layout: identical text up to key order (the golden file lists
+    # k_piston out of sorted order)
+    strip = lambda s: sorted(l.rstrip(",") for l in s.strip().splitlines())
+    assert strip(ours) == strip(golden)
+    keys = list(json.loads(ours).keys())
+    assert keys == sorted(keys)
+    assert ours.endswith("}\n")
+
+
+def test_frame_file_bytes(tmp_path):
+    path = tmp_path / "frames.bin"
+    r = run("save_load_frame", path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    data = path.read_bytes()
+    magic, version, reserved = struct.unpack_from("<8sII", data, 0)
+    assert magic == b"PFRAME\0\0" and version == 2 and reserved == 0
+    off = 16
+    frames = []
+    while off < len(data):
+        idx, n, unc, comp, crc = struct.unpack_from("<QQQQI", data, off)
+        off += 36
+        raw = zlib.decompress(data[off:off + comp])
+        off += comp
+        assert len(raw) == unc == 64 * n
+        assert zlib.crc32(raw) == crc
+        rec = np.frombuffer(raw, dtype=abi.PARTICLE_DTYPE)
+        pad = np.frombuffer(raw, dtype=np.uint8).reshape(n, 64)[:, 52:56]
+        assert not pad.any()
+        frames.append((idx, rec))
+    assert [f[0] for f in frames] == [0, 1, 5]
+    f0 = frames[0][1]
+    assert np.array_equal(f0["pos"], [[0, 0], [1, 2], [2, 4]])
+    assert np.array_equal(f0["type"], [0, 1, 2])
+    assert np.allclose(f0["vel"], [[0.1, 0.2]] * 3)
+    assert np.array_equal(frames[1][1]["type"], [10, 11])
+
+
+def test_init_particles_invariants():
+    # reference constructor frame (src/md_env.cu:1276-1445): phase separated,
+    # zero momentum, K = (2N-2) T / 2, all sites inside the box
+    n, n0, Lx, Ly, T = 2000, 1000, 80.0, 40.0, 0.6
+    f = abi.init_particles(n, n0, Lx, Ly, T, mass_a=1.0, mass_b=2.0)
+    m = np.where(f["type"] == 0, 1.0, 2.0)
+    assert (f["type"] == 0).sum() == n0 and (f["type"] == 1).sum() == n - n0
+    p = (m[:, None] * f["vel"]).sum(0)
+    assert np.abs(p).max() < 1e-10
+    K = 0.5 * (m * (f["vel"] ** 2).sum(1)).sum()
+    assert K == pytest.approx((2 * n - 2) * 0.5 * T, rel=1e-12)
+    assert np.all(f["pos"][:, 0] >= 0) and np.all(f["pos"][:, 0] < Lx)
+    assert np.all(f["pos"][:, 1] >= -1e-5) and np.all(f["pos"][:, 1] < Ly)
+    xa = f["pos"][f["type"] == 0, 0]
+    assert xa.min() >= Lx / 4 - 1e-4 and xa.max() < 3 * Lx / 4   # A in the central half
+    assert np.all((f["theta"] >= 0) & (f["theta"] < 2 * np.pi))
+    # deterministic: the reference seeds mt19937 with 12345
+    g = abi.init_particles(n, n0, Lx, Ly, T, mass_a=1.0, mass_b=2.0)
+    assert f.tobytes() == g.tobytes()
+
+
+def test_synthetic_generators():
+    f, Lx, Ly = abi.gen_lattice(4096, 2048, 0.8, 0.05, 0.7, seed=1001)
+    assert Lx == pytest.approx(np.sqrt(4096 / 0.8)) and Lx == Ly
+    a = Lx / 64
+    site = np.stack([(np.arange(4096) % 64 + 0.5) * a, (np.arange(4096) // 64 + 0.5) * a], 1)
+    d = f["pos"] - site
+    d -= Lx * np.round(d / Lx)
+    assert np.abs(d).max() <= 0.05 + 1e-12
+    assert (f["type"] == 0).sum() == 2048
+    K = 0.5 * (f["vel"] ** 2).sum()
+    assert K == pytest.approx((2 * 4096 - 2) * 0.5 * 0.7, rel=1e-12)
+    assert np.abs(f["vel"].sum(0)).max() < 1e-9
+    u, L2, _ = abi.gen_uniform(1000, 500, 0.8, 0.0, seed=3003)
+    assert np.all((u["pos"] >= 0) & (u["pos"] < L2)) and not u["vel"].any()
+    f2, _, _ = abi.gen_lattice(4096, 2048, 0.8, 0.05, 0.7, seed=1001)
+    assert f.tobytes() == f2.tobytes()
