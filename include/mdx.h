@@ -50,7 +50,8 @@ typedef enum mdx_ensemble {
     MDX_NVE = 0,      /* step_single_NVE            src/md_env.cu:1576-1637 */
     MDX_NVT = 1,      /* step_single_nose_hoover    src/md_env.cu:1639-1728 */
     MDX_NPH = 2,      /* step_single_NPH            src/md_env.cu:1795-1918 */
-    MDX_PISTON = 3    /* step_single_NPH_piston     src/md_env.cu:1920-2023 */
+    MDX_PISTON = 3,   /* step_single_NPH_piston     src/md_env.cu:1920-2023 */
+    MDX_IDLE = -1     /* batch slot that does not advance              */
 } mdx_ensemble;
 
 /* Arithmetic of the pair pass. */
@@ -171,6 +172,8 @@ int mdx_collect(mdx_sim *sim, mdx_particle64 *out, int64_t cap);
 int mdx_compute_forces(mdx_sim *sim, int32_t periodic_y, double *acc_out);
 
 int mdx_get_n(const mdx_sim *sim, int64_t *n);
+/* Replay each step from a captured hipGraph (default on). */
+int mdx_set_graph(mdx_sim *sim, int32_t enable);
 int mdx_set_precision(mdx_sim *sim, int32_t precision);
 int mdx_set_dt(mdx_sim *sim, double dt);
 int mdx_sync(mdx_sim *sim);
@@ -194,6 +197,40 @@ typedef struct mdx_timing {
 int mdx_timing_enable(mdx_sim *sim, int32_t enable);
 int mdx_timing_read(mdx_sim *sim, mdx_timing *out);
 
+
+
+/* ---- batches of independent systems ---------------------------------------
+ * No counterpart in the reference, which runs each sweep point (temperature,
+ * epsilon_AB, lambda) as a separate job.  A batch holds nslots slots; slot r
+ * holds up to n_cap[r] particles and cell_cap[r] cells (mdx_cell_capacity
+ * gives the cells a box needs).  Every mdx_batch_step launch advances all
+ * slots whose kind is not MDX_IDLE; each slot's reductions depend only on its
+ * own particles, so a system gives the same result alone and in any mix.  All
+ * slots share one precision. */
+typedef struct mdx_batch mdx_batch;
+
+/* Cells of the canonical grid for the params' box scaled by growth (>= 1). */
+int mdx_cell_capacity(const mdx_params *params, double growth, int32_t *out);
+
+int mdx_batch_create(int32_t nslots, const int64_t *n_cap, const int32_t *cell_cap, int32_t precision,
+                     int32_t device, mdx_batch **out);
+int mdx_batch_destroy(mdx_batch *b);
+/* Admit a system into a slot: upload, cell list, initial accelerations. */
+int mdx_batch_load(mdx_batch *b, int32_t slot, const mdx_params *params, const mdx_particle64 *frame,
+                   int64_t n);
+int mdx_batch_unload(mdx_batch *b, int32_t slot);
+/* What the slot does on the next mdx_batch_step calls (mdx_ensemble). */
+int mdx_batch_set_kind(mdx_batch *b, int32_t slot, int32_t kind, double p_ext, uint32_t flags);
+int mdx_batch_step(mdx_batch *b, int32_t nsteps);
+int mdx_batch_observables(mdx_batch *b, int32_t slot, uint32_t mask, mdx_obs *out);
+int mdx_batch_collect(mdx_batch *b, int32_t slot, mdx_particle64 *out, int64_t cap);
+int mdx_batch_fire(mdx_batch *b, int32_t slot, const mdx_fire_params *fp, double force_tol,
+                   int32_t max_steps, mdx_fire_result *result, int32_t *energy_steps,
+                   double *energy_trace, int32_t cap);
+int mdx_batch_sync(mdx_batch *b);
+int mdx_batch_set_graph(mdx_batch *b, int32_t enable);
+int mdx_batch_timing_enable(mdx_batch *b, int32_t enable);
+int mdx_batch_timing_read(mdx_batch *b, mdx_timing *out);
 
 /* ---- initial frames (host only, no GPU) --------------------------------- */
 

@@ -22,7 +22,7 @@ MDX_ERR_NONFINITE = 5
 MDX_ERR_STATE = 6
 MDX_ERR_NODEVICE = 7
 
-NVE, NVT, NPH, PISTON = 0, 1, 2, 3
+NVE, NVT, NPH, PISTON, IDLE = 0, 1, 2, 3, -1
 PREC_COMP32, PREC_ACC64, PREC_FP64 = 0, 1, 2
 STEP_MIDDLE_WRAP = 1
 OBS_K, OBS_U, OBS_P, OBS_VIRIAL = 1, 2, 4, 8
@@ -105,6 +105,22 @@ _SIGNATURES = {
     "mdx_get_n": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "mdx_set_precision": (C.c_int, [_P, C.c_int32]),
     "mdx_set_dt": (C.c_int, [_P, C.c_double]),
+    "mdx_set_graph": (C.c_int, [_P, C.c_int32]),
+    "mdx_cell_capacity": (C.c_int, [C.POINTER(Params), C.c_double, C.POINTER(C.c_int32)]),
+    "mdx_batch_create": (C.c_int, [C.c_int32, _P, _P, C.c_int32, C.c_int32, C.POINTER(_P)]),
+    "mdx_batch_destroy": (C.c_int, [_P]),
+    "mdx_batch_load": (C.c_int, [_P, C.c_int32, C.POINTER(Params), _P, C.c_int64]),
+    "mdx_batch_unload": (C.c_int, [_P, C.c_int32]),
+    "mdx_batch_set_kind": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, C.c_uint32]),
+    "mdx_batch_step": (C.c_int, [_P, C.c_int32]),
+    "mdx_batch_observables": (C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(Obs)]),
+    "mdx_batch_collect": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
+    "mdx_batch_fire": (C.c_int, [_P, C.c_int32, C.POINTER(FireParams), C.c_double, C.c_int32,
+                                 C.POINTER(FireResult), _P, _P, C.c_int32]),
+    "mdx_batch_sync": (C.c_int, [_P]),
+    "mdx_batch_set_graph": (C.c_int, [_P, C.c_int32]),
+    "mdx_batch_timing_enable": (C.c_int, [_P, C.c_int32]),
+    "mdx_batch_timing_read": (C.c_int, [_P, C.POINTER(Timing)]),
     "mdx_sync": (C.c_int, [_P]),
     "mdx_cell_list": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P, _P, _P,
                                 C.c_int64, C.c_int64]),
@@ -285,10 +301,93 @@ class Simulation:
         k = min(res.n_energy, cap)
         return res, es[:k].copy(), et[:k].copy()
 
+    def set_graph(self, on: bool) -> None:
+        check(self._lib.mdx_set_graph(self._h, 1 if on else 0))
+
     def timing_enable(self, on: bool) -> None:
         check(self._lib.mdx_timing_enable(self._h, 1 if on else 0))
 
     def timing_read(self) -> Timing:
         t = Timing()
         check(self._lib.mdx_timing_read(self._h, C.byref(t)))
+        return t
+
+
+def cell_capacity(params: Params, growth: float = 1.0) -> int:
+    out = C.c_int32()
+    check(load().mdx_cell_capacity(C.byref(params), growth, C.byref(out)))
+    return int(out.value)
+
+
+class Batch:
+    """Owner of one mdx_batch: many independent systems advanced together."""
+
+    def __init__(self, n_cap, cell_cap, precision: int = PREC_COMP32, device: int = -1):
+        self._lib = load()
+        self.n_cap = np.ascontiguousarray(n_cap, dtype=np.int64)
+        self.cell_cap = np.ascontiguousarray(cell_cap, dtype=np.int32)
+        self.precision = precision
+        self.n = [0] * len(self.n_cap)
+        h = C.c_void_p()
+        check(self._lib.mdx_batch_create(len(self.n_cap), _ptr(self.n_cap), _ptr(self.cell_cap), precision,
+                                         device, C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.mdx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, slot: int, params: Params, frame: np.ndarray) -> None:
+        frame = np.ascontiguousarray(frame, dtype=PARTICLE_DTYPE)
+        check(self._lib.mdx_batch_load(self._h, slot, C.byref(params), _ptr(frame), len(frame)))
+        self.n[slot] = len(frame)
+
+    def unload(self, slot: int) -> None:
+        check(self._lib.mdx_batch_unload(self._h, slot))
+        self.n[slot] = 0
+
+    def set_kind(self, slot: int, kind: int, p_ext: float = 0.0, flags: int = 0) -> None:
+        check(self._lib.mdx_batch_set_kind(self._h, slot, kind, p_ext, flags))
+
+    def step(self, nsteps: int = 1) -> None:
+        check(self._lib.mdx_batch_step(self._h, nsteps))
+
+    def sync(self) -> None:
+        check(self._lib.mdx_batch_sync(self._h))
+
+    def observables(self, slot: int, mask: int = OBS_K | OBS_U | OBS_P | OBS_VIRIAL) -> Obs:
+        o = Obs()
+        check(self._lib.mdx_batch_observables(self._h, slot, mask, C.byref(o)))
+        return o
+
+    def collect(self, slot: int) -> np.ndarray:
+        out = empty_frame(self.n[slot])
+        check(self._lib.mdx_batch_collect(self._h, slot, _ptr(out), len(out)))
+        return out
+
+    def fire(self, slot: int, fp: FireParams, force_tol: float, max_steps: int, cap: int = 64):
+        res = FireResult()
+        es = np.zeros(cap, np.int32)
+        et = np.zeros(cap, np.float64)
+        check(self._lib.mdx_batch_fire(self._h, slot, C.byref(fp), force_tol, max_steps, C.byref(res),
+                                       _ptr(es), _ptr(et), cap))
+        k = min(res.n_energy, cap)
+        return res, es[:k].copy(), et[:k].copy()
+
+    def set_graph(self, on: bool) -> None:
+        check(self._lib.mdx_batch_set_graph(self._h, 1 if on else 0))
+
+    def timing_enable(self, on: bool) -> None:
+        check(self._lib.mdx_batch_timing_enable(self._h, 1 if on else 0))
+
+    def timing_read(self) -> Timing:
+        t = Timing()
+        check(self._lib.mdx_batch_timing_read(self._h, C.byref(t)))
         return t

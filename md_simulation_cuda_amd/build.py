@@ -54,7 +54,8 @@ def build_mdx() -> Path:
     OUT.mkdir(parents=True, exist_ok=True)
     target = OUT / "libmdx.so"
     tmp = OUT / "libmdx.so.tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", str(CSRC / "mdx_api.hip"), str(CSRC / "mdx_init.cpp"),
+    _run([HIPCC, *HIP_FLAGS, "-shared", str(CSRC / "mdx_api.hip"), str(CSRC / "mdx_batch.hip"),
+          str(CSRC / "mdx_init.cpp"),
           "-o", str(tmp)])
     os.replace(tmp, target)
     return target

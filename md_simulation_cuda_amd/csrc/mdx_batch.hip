@@ -1,0 +1,879 @@
+// Batch of independent LJ systems on one GPU: allocation, slot admission,
+// launch sequences per ensemble, observables, FIRE, hipGraph capture.
+//
+// Step sequencing restates src/md_env.cu:1576-2023 (NVE, Nose-Hoover, NPH,
+// piston) and src/md_energy_minimizer.cu:122-272 (FIRE) for one periodic
+// domain per slot: the MPI slab exchange of the reference (src/md_env.cu:
+// 757-1274) is replaced by a cell list rebuilt on the device every step.
+#include "mdx_batch.hpp"
+#include "mdx_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace mdx {
+
+namespace {
+thread_local std::string g_err;
+}
+
+int set_error(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+const char *last_error() { return g_err.c_str(); }
+
+#define MDX_HIP(...)                                                                                \
+    do {                                                                                            \
+        hipError_t e_ = (__VA_ARGS__);                                                              \
+        if (e_ != hipSuccess)                                                                       \
+            return set_error(MDX_ERR_HIP, std::string(#__VA_ARGS__) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define MDX_TRY(...)                   \
+    do {                               \
+        int rc_ = (__VA_ARGS__);       \
+        if (rc_ != MDX_OK) return rc_; \
+    } while (0)
+
+namespace {
+
+int ceil_div(int64_t a, int64_t b) { return static_cast<int>((a + b - 1) / b); }
+
+PairTable make_pair_table(const mdx_params &p, double &rc_max) {
+    PairTable pt{};
+    const double sig[3] = {p.sigma_aa, p.sigma_bb, p.sigma_ab};
+    const double eps[3] = {p.eps_aa, p.eps_bb, p.eps_ab};
+    const double inv_c = 1.0 / p.cutoff;
+    const double s2 = inv_c * inv_c, s6 = s2 * s2 * s2, s12 = s6 * s6;
+    const double vcc = 4.0 * (s12 - s6);   // src/md_cuda_common.cu:719-727
+    const float cutf = static_cast<float>(p.cutoff);
+    for (int k = 0; k < 3; ++k) {
+        const double rc = p.cutoff * sig[k];
+        pt.rc2[k] = rc * rc;
+        pt.sig2[k] = sig[k] * sig[k];
+        pt.eps24[k] = 24.0 * eps[k];
+        pt.eps4[k] = 4.0 * eps[k];
+        pt.vc[k] = eps[k] * vcc;
+        // FP32 constants formed as src/md_cuda_common.cu:220-241 forms them
+        const float sf = static_cast<float>(sig[k]), ef = static_cast<float>(eps[k]);
+        const float rcf = cutf * sf;
+        pt.rc2f[k] = rcf * rcf;
+        pt.sig2f[k] = sf * sf;
+        pt.eps24f[k] = 24.0f * ef;
+        pt.eps4f[k] = 4.0f * ef;
+    }
+    pt.m[0] = p.mass_a;
+    pt.m[1] = p.mass_b;
+    pt.m[2] = p.mass_b;
+    for (int k = 0; k < 3; ++k) pt.inv_m[k] = 1.0 / pt.m[k];
+    // FP32 (sigma/r)^12 / r^2 overflows below r ~ 2.3e-3 sigma: such
+    // overlapping pairs are evaluated in FP64.
+    const double smin = std::min(sig[0], std::min(sig[1], sig[2]));
+    pt.r2_guard = static_cast<float>(1.0e-4 * smin * smin);
+    rc_max = p.cutoff * std::max(sig[0], std::max(sig[1], sig[2]));
+    return pt;
+}
+
+int validate_params(const mdx_params &p) {
+    auto pos = [](double v) { return v > 0.0 && std::isfinite(v); };
+    if (!pos(p.Lx) || !pos(p.Ly)) return set_error(MDX_ERR_BOX, "box sides must be positive and finite");
+    if (!pos(p.sigma_aa) || !pos(p.sigma_bb) || !pos(p.sigma_ab))
+        return set_error(MDX_ERR_INVALID, "sigma must be > 0");
+    if (!pos(p.mass_a) || !pos(p.mass_b)) return set_error(MDX_ERR_INVALID, "masses must be > 0");
+    if (!pos(p.cutoff)) return set_error(MDX_ERR_INVALID, "cutoff must be > 0");
+    if (!std::isfinite(p.dt)) return set_error(MDX_ERR_INVALID, "dt must be finite");
+    return MDX_OK;
+}
+
+}  // namespace
+
+int mdx_cell_capacity(const mdx_params &p, double ratio) {
+    double rc_max = 0.0;
+    (void)make_pair_table(p, rc_max);
+    const double side = std::max(p.cell_scale, 1.0) * rc_max;
+    const double f = std::max(1.0, ratio);
+    const Grid g = make_grid(p.Lx * f, p.Ly * f, side, rc_max, 1);
+    const int64_t c = static_cast<int64_t>(g.ncx) * g.ncy;
+    return c > (int64_t{1} << 30) ? -1 : static_cast<int>(c);
+}
+
+Batch::~Batch() {
+    if (device >= 0) (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    invalidate_graphs();
+    void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
+                     d_sp, d_sc, d_blk2sys, d_cblk2sys, d_want};
+    for (void *p : ptrs_)
+        if (p) (void)hipFree(p);
+    for (auto *vec : {&force_ev, &step_ev})
+        for (auto &e : *vec) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Batch::invalidate_graphs() {
+    for (auto &g : graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int precision, int device_ordinal) {
+    if (nslots <= 0 || nslots > (1 << 20)) return set_error(MDX_ERR_INVALID, "slot count out of range");
+    if (precision < MDX_PREC_COMP32 || precision > MDX_PREC_FP64)
+        return set_error(MDX_ERR_INVALID, "unknown precision");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_error(MDX_ERR_NODEVICE, "no HIP device");
+    if (device_ordinal >= ndev) return set_error(MDX_ERR_INVALID, "device ordinal out of range");
+    if (device_ordinal >= 0) MDX_HIP(hipSetDevice(device_ordinal));
+    MDX_HIP(hipGetDevice(&device));
+    MDX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    prec = precision;
+    slots.assign(nslots, Slot{});
+    h_sp.assign(nslots, SysParams{});
+    h_want.assign(nslots, 0);
+    std::vector<int32_t> b2s, cb2s;
+    int64_t off = 0, coff = 0;
+    int boff = 0, cboff = 0;
+    for (int r = 0; r < nslots; ++r) {
+        if (n_cap[r] < 0 || ccap[r] < 0) return set_error(MDX_ERR_INVALID, "negative capacity");
+        Slot &s = slots[r];
+        s.n_cap = n_cap[r];
+        s.ccap = ccap[r];
+        SysParams &sp = h_sp[r];
+        sp.off = static_cast<int32_t>(off);
+        sp.n_cap = static_cast<int32_t>(n_cap[r]);
+        sp.coff = static_cast<int32_t>(coff);
+        sp.ccap = ccap[r];
+        sp.boff = boff;
+        sp.nblk = ceil_div(n_cap[r], kBlock);
+        sp.cboff = cboff;
+        sp.ncblk = ceil_div(ccap[r], kBlock);
+        sp.periodic_y = 1;
+        sp.wrap_keys = 1;
+        for (int b = 0; b < sp.nblk; ++b) b2s.push_back(r);
+        for (int b = 0; b < sp.ncblk; ++b) cb2s.push_back(r);
+        off += n_cap[r];
+        coff += ccap[r];
+        boff += sp.nblk;
+        cboff += sp.ncblk;
+        if (off > INT32_MAX - kBlock || coff > INT32_MAX / 2)
+            return set_error(MDX_ERR_CAPACITY, "batch above 2^31 particles or 2^30 cells");
+    }
+    N = off;
+    C = static_cast<int32_t>(coff);
+    TB = boff;
+    TCB = cboff;
+    const size_t n16 = static_cast<size_t>(std::max<int64_t>(N, 1)) * 16;
+    const size_t n4 = static_cast<size_t>(std::max<int64_t>(N, 1)) * 4;
+    const size_t n1 = static_cast<size_t>(std::max<int64_t>(N, 1));
+    MDX_HIP(hipMalloc(&x, n16));
+    MDX_HIP(hipMalloc(&v, n16));
+    MDX_HIP(hipMalloc(&a, n16));
+    MDX_HIP(hipMalloc(&img, n16));
+    MDX_HIP(hipMalloc(&cls, n1));
+    MDX_HIP(hipMalloc(&icls, n1));
+    MDX_HIP(hipMalloc(&key, n4));
+    MDX_HIP(hipMalloc(&sorted, n4));
+    MDX_HIP(hipMalloc(&scell, n4));
+    MDX_HIP(hipMalloc(&count, (static_cast<size_t>(C) + 1) * 4));
+    MDX_HIP(hipMalloc(&cs, (static_cast<size_t>(C) + 1) * 4));
+    MDX_HIP(hipMalloc(&bsum, static_cast<size_t>(ceil_div(C, kScanChunk) + 1) * 4));
+    MDX_HIP(hipMalloc(&partials, static_cast<size_t>(std::max(TB, 1)) * 3 * sizeof(double)));
+    MDX_HIP(hipMalloc(&d_sp, nslots * sizeof(SysParams)));
+    MDX_HIP(hipMalloc(&d_sc, nslots * sizeof(DevScalars)));
+    MDX_HIP(hipMalloc(&d_blk2sys, std::max<size_t>(b2s.size(), 1) * 4));
+    MDX_HIP(hipMalloc(&d_cblk2sys, std::max<size_t>(cb2s.size(), 1) * 4));
+    MDX_HIP(hipMalloc(&d_want, nslots * 4));
+    MDX_HIP(hipMemset(count, 0, (static_cast<size_t>(C) + 1) * 4));
+    MDX_HIP(hipMemset(d_sc, 0, nslots * sizeof(DevScalars)));
+    MDX_HIP(hipMemset(x, 0, n16));
+    MDX_HIP(hipMemset(v, 0, n16));
+    MDX_HIP(hipMemset(a, 0, n16));
+    if (!b2s.empty()) MDX_HIP(hipMemcpy(d_blk2sys, b2s.data(), b2s.size() * 4, hipMemcpyHostToDevice));
+    if (!cb2s.empty()) MDX_HIP(hipMemcpy(d_cblk2sys, cb2s.data(), cb2s.size() * 4, hipMemcpyHostToDevice));
+    MDX_HIP(hipMemcpy(d_sp, h_sp.data(), nslots * sizeof(SysParams), hipMemcpyHostToDevice));
+    ptrs.x = x;
+    ptrs.v = v;
+    ptrs.a = a;
+    ptrs.cls = cls;
+    ptrs.key = key;
+    ptrs.count = count;
+    ptrs.cs = cs;
+    ptrs.sorted = sorted;
+    ptrs.scell = scell;
+    ptrs.img = img;
+    ptrs.icls = icls;
+    ptrs.partials = partials;
+    ptrs.sp = d_sp;
+    ptrs.sc = d_sc;
+    ptrs.blk2sys = d_blk2sys;
+    ptrs.cblk2sys = d_cblk2sys;
+    ptrs.TB = TB;
+    ptrs.S = nslots;
+    ptrs.C = C;
+    return MDX_OK;
+}
+
+int Batch::upload_sp(int r) {
+    MDX_HIP(hipMemcpyAsync(d_sp + r, &h_sp[r], sizeof(SysParams), hipMemcpyHostToDevice, stream));
+    return MDX_OK;
+}
+
+int Batch::upload_all_sp() {
+    MDX_HIP(hipMemcpyAsync(d_sp, h_sp.data(), h_sp.size() * sizeof(SysParams), hipMemcpyHostToDevice, stream));
+    return MDX_OK;
+}
+
+int Batch::read_scalars(int r, DevScalars &h) {
+    MDX_HIP(hipMemcpyAsync(&h, d_sc + r, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+    MDX_HIP(hipStreamSynchronize(stream));
+    return MDX_OK;
+}
+
+int Batch::write_scalar_block(int r, const DevScalars &h) {
+    MDX_HIP(hipMemcpyAsync(d_sc + r, &h, sizeof(DevScalars), hipMemcpyHostToDevice, stream));
+    MDX_HIP(hipStreamSynchronize(stream));
+    return MDX_OK;
+}
+
+int Batch::sync() {
+    MDX_HIP(hipStreamSynchronize(stream));
+    return MDX_OK;
+}
+
+// Launch one step of every slot (their kind/pmode in d_sp).
+int Batch::launch_step_kernels(bool obs) {
+    const int S = static_cast<int>(slots.size());
+    k_box<<<ceil_div(S, 64), 64, 0, stream>>>(ptrs);
+    if (TB > 0) {
+        if (prec == MDX_PREC_COMP32)
+            k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs);
+        else
+            k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs);
+    }
+    const int nchunk = ceil_div(C, kScanChunk);
+    if (nchunk > 0) {
+        k_scan_a<<<nchunk, kBlock, 0, stream>>>(count, C, bsum);
+        k_scan_b<<<nchunk, kBlock, 0, stream>>>(count, C, bsum, cs);
+    }
+    if (TB > 0) k_scatter<<<TB, kBlock, 0, stream>>>(ptrs);
+    if (TCB > 0) {
+        if (prec == MDX_PREC_FP64)
+            k_cell_image<double2><<<TCB, kBlock, 0, stream>>>(ptrs);
+        else
+            k_cell_image<float2><<<TCB, kBlock, 0, stream>>>(ptrs);
+    }
+    if (TB > 0) {
+        const bool timed = timing;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timed && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
+            (void)hipEventRecord(e0, stream);
+        switch (prec) {
+        case MDX_PREC_COMP32:
+            if (obs) k_pair<float2, ACC_COMP32, true><<<TB, kBlock, 0, stream>>>(ptrs);
+            else k_pair<float2, ACC_COMP32, false><<<TB, kBlock, 0, stream>>>(ptrs);
+            break;
+        case MDX_PREC_ACC64:
+            if (obs) k_pair<float2, ACC_FP64, true><<<TB, kBlock, 0, stream>>>(ptrs);
+            else k_pair<float2, ACC_FP64, false><<<TB, kBlock, 0, stream>>>(ptrs);
+            break;
+        default:
+            if (obs) k_pair<double2, ACC_FP64, true><<<TB, kBlock, 0, stream>>>(ptrs);
+            else k_pair<double2, ACC_FP64, false><<<TB, kBlock, 0, stream>>>(ptrs);
+            break;
+        }
+        if (e1) {
+            (void)hipEventRecord(e1, stream);
+            force_ev.emplace_back(e0, e1);
+        }
+    }
+    MDX_HIP(hipGetLastError());
+    return MDX_OK;
+}
+
+// count passes of the step sequence; replayed from a captured hipGraph unless
+// kernel timing is on.
+int Batch::run_passes(int count_, bool obs) {
+    if (count_ <= 0) return MDX_OK;
+    hipEvent_t s0 = nullptr, s1 = nullptr;
+    if (timing && hipEventCreate(&s0) == hipSuccess && hipEventCreate(&s1) == hipSuccess)
+        (void)hipEventRecord(s0, stream);
+    if (use_graph && !timing) {
+        hipGraphExec_t &ge = graph[obs ? 1 : 0];
+        if (!ge) {
+            hipGraph_t g = nullptr;
+            MDX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            const int rc = launch_step_kernels(obs);
+            hipError_t e = hipStreamEndCapture(stream, &g);
+            if (rc != MDX_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            MDX_HIP(e);
+            e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            MDX_HIP(e);
+        }
+        for (int k = 0; k < count_; ++k) MDX_HIP(hipGraphLaunch(ge, stream));
+    } else {
+        for (int k = 0; k < count_; ++k) MDX_TRY(launch_step_kernels(obs));
+    }
+    if (s1) {
+        (void)hipEventRecord(s1, stream);
+        step_ev.emplace_back(s0, s1);
+    }
+    return MDX_OK;
+}
+
+namespace {
+void set_idle(SysParams &sp, const Slot &s) {
+    sp.kind = SK_NONE;
+    sp.pmode = PM_NONE;
+    sp.periodic_y = s.cells_py;
+    sp.wrap_keys = 1;
+    sp.middle_wrap = 0;
+}
+}  // namespace
+
+int Batch::load(int r, const mdx_params &p, const mdx_particle64 *frame, int64_t n) {
+    if (r < 0 || r >= static_cast<int>(slots.size())) return set_error(MDX_ERR_INVALID, "slot out of range");
+    if (!frame || n <= 0) return set_error(MDX_ERR_INVALID, "empty frame");
+    Slot &s = slots[r];
+    if (n > s.n_cap) return set_error(MDX_ERR_CAPACITY, "frame larger than the slot capacity");
+    MDX_TRY(validate_params(p));
+    MDX_HIP(hipSetDevice(device));
+    SysParams &sp = h_sp[r];
+    double rc_max = 0.0;
+    sp.pt = make_pair_table(p, rc_max);
+    const double side = std::max(p.cell_scale, 1.0) * rc_max;
+    const Grid g = make_grid(p.Lx, p.Ly, side, rc_max, 1);
+    if (static_cast<int64_t>(g.ncx) * g.ncy > s.ccap)
+        return set_error(MDX_ERR_CAPACITY, "box needs more cells than the slot holds");
+    s.p = p;
+    s.rc_max = rc_max;
+    s.n = n;
+    s.loaded = true;
+    s.t = 0.0;
+    s.Lx0 = p.Lx;
+    s.Ly0 = p.Ly;
+    s.cells_valid = false;
+    s.cells_py = 1;
+    s.K_valid = s.W_valid = false;
+    s.piston_active = s.nph_init = s.vp_init = false;
+    s.kind = SK_NONE;
+    s.flags = 0;
+    sp.n = static_cast<int32_t>(n);
+    sp.cell_side = side;
+    sp.rc_max = rc_max;
+    sp.dt = p.dt;
+    sp.T0 = p.T_target;
+    sp.Q = p.Q;
+    sp.g_dof = 2.0 * static_cast<double>(n) - 2.0;   // src/md_env.cu:1667
+    sp.W_baro = std::max(p.barostat_mass, 1e-12);
+    sp.k_piston = p.k_piston;
+    set_idle(sp, s);
+    // particles
+    std::vector<double2> hx(n), hv(n);
+    std::vector<uint8_t> hc(n);
+    s.type.resize(n);
+    s.theta.resize(n);
+    for (int64_t i = 0; i < n; ++i) {
+        hx[i] = make_double2(frame[i].pos[0], frame[i].pos[1]);
+        hv[i] = make_double2(frame[i].vel[0], frame[i].vel[1]);
+        const int t = frame[i].type;
+        hc[i] = static_cast<uint8_t>(t == 0 ? 0 : (t == 1 ? 1 : 2));
+        s.type[i] = t;
+        s.theta[i] = frame[i].theta;
+    }
+    MDX_HIP(hipMemcpyAsync(x + sp.off, hx.data(), n * 16, hipMemcpyHostToDevice, stream));
+    MDX_HIP(hipMemcpyAsync(v + sp.off, hv.data(), n * 16, hipMemcpyHostToDevice, stream));
+    MDX_HIP(hipMemcpyAsync(cls + sp.off, hc.data(), n, hipMemcpyHostToDevice, stream));
+    DevScalars h{};
+    h.g = g;
+    h.gamma = p.Lx / p.Ly;
+    h.Ly_floor = std::max(1e-12, p.barostat_height_min_ratio * p.Ly);
+    h.Ly_ceil = std::max(h.Ly_floor, p.barostat_height_max_ratio * p.Ly);
+    MDX_HIP(hipMemcpyAsync(d_sc + r, &h, sizeof(h), hipMemcpyHostToDevice, stream));
+    MDX_TRY(upload_sp(r));
+    MDX_HIP(hipStreamSynchronize(stream));
+    return compute_forces({r}, 1, false);   // src/md_env.cu:66
+}
+
+int Batch::unload(int r) {
+    if (r < 0 || r >= static_cast<int>(slots.size())) return set_error(MDX_ERR_INVALID, "slot out of range");
+    Slot &s = slots[r];
+    s.loaded = false;
+    s.n = 0;
+    h_sp[r].n = 0;
+    set_idle(h_sp[r], s);
+    return upload_sp(r);
+}
+
+int Batch::set_state(int r, const mdx_particle64 *frame, int64_t n) {
+    if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
+        return set_error(MDX_ERR_STATE, "slot not loaded");
+    Slot &s = slots[r];
+    if (n != s.n)
+        return set_error(MDX_ERR_INVALID, "loaded particle count " + std::to_string(n) + " doesn't match n " +
+                                              std::to_string(s.n));
+    const int64_t off = h_sp[r].off;
+    std::vector<double2> hx(n), hv(n);
+    std::vector<uint8_t> hc(n);
+    for (int64_t i = 0; i < n; ++i) {
+        hx[i] = make_double2(frame[i].pos[0], frame[i].pos[1]);
+        hv[i] = make_double2(frame[i].vel[0], frame[i].vel[1]);
+        const int t = frame[i].type;
+        hc[i] = static_cast<uint8_t>(t == 0 ? 0 : (t == 1 ? 1 : 2));
+        s.type[i] = t;
+        s.theta[i] = frame[i].theta;
+    }
+    MDX_HIP(hipMemcpyAsync(x + off, hx.data(), n * 16, hipMemcpyHostToDevice, stream));
+    MDX_HIP(hipMemcpyAsync(v + off, hv.data(), n * 16, hipMemcpyHostToDevice, stream));
+    MDX_HIP(hipMemcpyAsync(cls + off, hc.data(), n, hipMemcpyHostToDevice, stream));
+    MDX_HIP(hipStreamSynchronize(stream));
+    s.piston_active = false;
+    s.K_valid = s.W_valid = false;
+    return compute_forces({r}, 1, false);
+}
+
+int Batch::set_kind(int r, int kind, double p_ext, uint32_t flags) {
+    if (r < 0 || r >= static_cast<int>(slots.size())) return set_error(MDX_ERR_INVALID, "slot out of range");
+    if (kind != SK_NONE && !slots[r].loaded) return set_error(MDX_ERR_STATE, "slot not loaded");
+    slots[r].kind = kind;
+    slots[r].p_ext = p_ext;
+    slots[r].flags = flags;
+    return MDX_OK;
+}
+
+int Batch::kinetic(const std::vector<int> &which) {
+    if (which.empty()) return MDX_OK;
+    std::fill(h_want.begin(), h_want.end(), 0);
+    for (int r : which) h_want[r] = 1;
+    MDX_HIP(hipMemcpyAsync(d_want, h_want.data(), h_want.size() * 4, hipMemcpyHostToDevice, stream));
+    if (TB > 0) k_kinetic<<<TB, kBlock, 0, stream>>>(ptrs, d_want);
+    MDX_HIP(hipGetLastError());
+    for (int r : which) slots[r].K_valid = true;
+    return MDX_OK;
+}
+
+int Batch::walls(const std::vector<int> &which) {
+    if (which.empty()) return MDX_OK;
+    std::fill(h_want.begin(), h_want.end(), 0);
+    for (int r : which) h_want[r] = 1;
+    MDX_HIP(hipMemcpyAsync(d_want, h_want.data(), h_want.size() * 4, hipMemcpyHostToDevice, stream));
+    if (TB > 0) k_walls<<<TB, kBlock, 0, stream>>>(ptrs, d_want);
+    MDX_HIP(hipGetLastError());
+    return MDX_OK;
+}
+
+// One pass at the current positions: slots in `which` rebuild their cell list
+// with the given y boundary and run pair mode pmode; the others idle.
+static int one_pass(Batch &b, const std::vector<int> &which, int periodic_y, int pmode) {
+    for (size_t r = 0; r < b.slots.size(); ++r) set_idle(b.h_sp[r], b.slots[r]);
+    for (int r : which) {
+        SysParams &sp = b.h_sp[r];
+        sp.kind = SK_KEYS;
+        sp.periodic_y = periodic_y;
+        sp.pmode = pmode;
+        sp.wrap_keys = 1;
+    }
+    MDX_TRY(b.upload_all_sp());
+    const bool obs = pmode == PM_OBS;
+    MDX_TRY(b.run_passes(1, obs));
+    for (int r : which) {
+        b.slots[r].cells_valid = true;
+        b.slots[r].cells_py = periodic_y;
+    }
+    return MDX_OK;
+}
+
+int Batch::rebuild_cells(const std::vector<int> &which, int periodic_y) {
+    return one_pass(*this, which, periodic_y, PM_NONE);
+}
+
+int Batch::compute_forces(const std::vector<int> &which, int periodic_y, bool with_walls) {
+    MDX_TRY(one_pass(*this, which, periodic_y, with_walls ? PM_WALLS : PM_PLAIN));
+    for (int r : which) slots[r].W_valid = false;
+    return MDX_OK;
+}
+
+int Batch::obs_pass(int r, int periodic_y) { return one_pass(*this, {r}, periodic_y, PM_OBS); }
+
+// Preconditions of the reference's steps that need a pass of their own:
+// K for Nose-Hoover, P_old for NPH, cal_forces_piston on the first piston step.
+int Batch::pre_step() {
+    std::vector<int> need_k, need_w, need_piston;
+    for (size_t r = 0; r < slots.size(); ++r) {
+        Slot &s = slots[r];
+        if (!s.loaded || s.kind == SK_NONE) continue;
+        switch (s.kind) {
+        case SK_NVE:
+            s.piston_active = false;
+            break;
+        case SK_NVT:
+            s.piston_active = false;
+            if (!s.K_valid) need_k.push_back(static_cast<int>(r));
+            break;
+        case SK_NPH:
+            s.piston_active = false;
+            if (!s.nph_init) {   // init_nph_state_if_needed, src/md_env.cu:1757-1778
+                DevScalars h;
+                MDX_TRY(read_scalars(static_cast<int>(r), h));
+                h.gamma = h.g.Lx / h.g.Ly;
+                h.p_A = std::max(s.p.barostat_mass, 1e-12) * s.p.barostat_area_rate_init;
+                MDX_TRY(write_scalar_block(static_cast<int>(r), h));
+                s.nph_init = true;
+            }
+            if (!s.K_valid) need_k.push_back(static_cast<int>(r));
+            if (!s.W_valid || !s.cells_valid || s.cells_py != 1) need_w.push_back(static_cast<int>(r));
+            break;
+        case SK_PISTON:
+            if (!s.vp_init) {   // init_nph_piston_state_if_needed, src/md_env.cu:1780-1793
+                DevScalars h;
+                MDX_TRY(read_scalars(static_cast<int>(r), h));
+                h.v_p = 0.0;
+                MDX_TRY(write_scalar_block(static_cast<int>(r), h));
+                s.vp_init = true;
+            }
+            if (!s.piston_active) {
+                s.piston_active = true;
+                need_piston.push_back(static_cast<int>(r));
+            }
+            break;
+        default:
+            break;
+        }
+    }
+    MDX_TRY(kinetic(need_k));
+    if (!need_w.empty()) {
+        MDX_TRY(one_pass(*this, need_w, 1, PM_OBS));
+        for (int r : need_w) slots[r].W_valid = true;
+    }
+    if (!need_piston.empty()) MDX_TRY(compute_forces(need_piston, 0, true));
+    return MDX_OK;
+}
+
+int Batch::step(int nsteps) {
+    if (nsteps < 0) return set_error(MDX_ERR_INVALID, "nsteps < 0");
+    if (nsteps == 0) return MDX_OK;
+    MDX_HIP(hipSetDevice(device));
+    MDX_TRY(pre_step());
+    bool obs = false, any = false, nph = false;
+    for (size_t r = 0; r < slots.size(); ++r) {
+        Slot &s = slots[r];
+        SysParams &sp = h_sp[r];
+        if (!s.loaded || s.kind == SK_NONE) {
+            set_idle(sp, s);
+            continue;
+        }
+        any = true;
+        sp.kind = s.kind;
+        sp.p_ext = s.p_ext;
+        sp.wrap_keys = 0;
+        sp.middle_wrap = 0;
+        sp.periodic_y = s.kind == SK_PISTON ? 0 : 1;
+        switch (s.kind) {
+        case SK_NVE: sp.pmode = PM_NVE; break;
+        case SK_NVT: sp.pmode = PM_NVT; break;
+        case SK_NPH: sp.pmode = PM_NPH; obs = true; nph = true; break;
+        case SK_PISTON: sp.pmode = PM_PISTON; break;
+        default: return set_error(MDX_ERR_INVALID, "unknown ensemble");
+        }
+        if (s.kind == SK_NVT && (s.flags & MDX_STEP_MIDDLE_WRAP)) {
+            DevScalars h;
+            MDX_TRY(read_scalars(static_cast<int>(r), h));
+            const double Lx = h.g.Lx;
+            if (Lx > 0.0 && h.g.Ly > 0.0 && s.p.devide_p > 0.0) {   // src/md_env.cu:1457-1473
+                const double smax = std::max(s.p.sigma_aa, std::max(s.p.sigma_bb, s.p.sigma_ab));
+                double slab = s.p.devide_p * Lx + 2.0 * smax;
+                if (slab > Lx) slab = Lx;
+                sp.middle_wrap = 1;
+                sp.slab_width = slab;
+                sp.slab_xmin = 0.5 * Lx - 0.5 * slab;
+                sp.slab_period = 2.0 * slab;
+            }
+        }
+    }
+    if (!any) return MDX_OK;
+    MDX_TRY(upload_all_sp());
+    MDX_TRY(run_passes(nsteps, obs));
+    if (timing) step_n += nsteps;
+    for (size_t r = 0; r < slots.size(); ++r) {
+        Slot &s = slots[r];
+        if (!s.loaded || s.kind == SK_NONE) continue;
+        s.t += s.p.dt * nsteps;
+        s.cells_valid = true;
+        s.cells_py = s.kind == SK_PISTON ? 0 : 1;
+        s.K_valid = s.kind == SK_NVT || s.kind == SK_NPH;
+        s.W_valid = s.kind == SK_NPH;
+    }
+    if (nph) {   // the reference aborts on an invalid proposed area
+        for (size_t r = 0; r < slots.size(); ++r) {
+            if (!slots[r].loaded || slots[r].kind != SK_NPH) continue;
+            DevScalars h;
+            MDX_TRY(read_scalars(static_cast<int>(r), h));
+            if (h.box_error) return set_error(MDX_ERR_BOX, "step_single_NPH: invalid proposed area");
+        }
+    }
+    return MDX_OK;
+}
+
+int Batch::observables(int r, uint32_t mask, mdx_obs *out) {
+    if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
+        return set_error(MDX_ERR_STATE, "slot not loaded");
+    MDX_HIP(hipSetDevice(device));
+    Slot &s = slots[r];
+    if ((mask & (MDX_OBS_K | MDX_OBS_P)) && !s.K_valid) MDX_TRY(kinetic({r}));
+    if (s.piston_active) {
+        if (mask & (MDX_OBS_U | MDX_OBS_VIRIAL)) MDX_TRY(obs_pass(r, 0));
+        if (mask & (MDX_OBS_U | MDX_OBS_P)) MDX_TRY(walls({r}));
+    } else if (mask & (MDX_OBS_U | MDX_OBS_P | MDX_OBS_VIRIAL)) {
+        MDX_TRY(obs_pass(r, 1));
+    }
+    DevScalars h;
+    MDX_TRY(read_scalars(r, h));
+    std::memset(out, 0, sizeof(*out));
+    out->K = h.K;
+    const double Lx = h.g.Lx, Ly = h.g.Ly;
+    if (mask & MDX_OBS_VIRIAL) out->W = h.W;
+    if (s.piston_active) {
+        if (mask & MDX_OBS_U) out->U = h.U + h.wallE;
+        if (mask & MDX_OBS_P) {   // src/md_env.cu:2255-2261
+            h.last_P = (Lx > 0.0) ? h.F_p / Lx : 0.0;
+            MDX_TRY(write_scalar_block(r, h));
+            out->P = h.last_P;
+        }
+    } else {
+        if (mask & MDX_OBS_U) out->U = h.U;
+        if (mask & MDX_OBS_P) {   // src/md_env.cu:2205-2252
+            const double area = Lx * Ly;
+            h.last_P = (area > 0.0) ? (2.0 * h.K + h.W) / (2.0 * area) : 0.0;
+            MDX_TRY(write_scalar_block(r, h));
+            out->P = h.last_P;
+        }
+    }
+    out->xi = h.xi;
+    out->p_A = h.p_A;
+    out->area_rate = h.p_A / std::max(s.p.barostat_mass, 1e-12);
+    out->v_p = h.v_p;
+    out->F_p = h.F_p;
+    out->last_P = h.last_P;
+    out->Lx = Lx;
+    out->Ly = Ly;
+    out->t = s.t;
+    out->piston_active = s.piston_active ? 1 : 0;
+    return MDX_OK;
+}
+
+int Batch::partial_U_lambda(int r, double eps, double *out) {
+    if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
+        return set_error(MDX_ERR_STATE, "slot not loaded");
+    MDX_HIP(hipSetDevice(device));
+    MDX_TRY(obs_pass(r, slots[r].piston_active ? 0 : 1));
+    DevScalars h;
+    MDX_TRY(read_scalars(r, h));
+    *out = -eps * h.lam;
+    return MDX_OK;
+}
+
+int Batch::collect(int r, mdx_particle64 *out, int64_t cap) {
+    if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
+        return set_error(MDX_ERR_STATE, "slot not loaded");
+    const Slot &s = slots[r];
+    if (cap < s.n) return set_error(MDX_ERR_CAPACITY, "output capacity below n");
+    MDX_HIP(hipSetDevice(device));
+    const int64_t n = s.n, off = h_sp[r].off;
+    std::vector<double2> hx(n), hv(n);
+    std::vector<unsigned char> ha(n * 16);
+    const bool f32 = prec == MDX_PREC_COMP32;
+    MDX_HIP(hipMemcpyAsync(hx.data(), x + off, n * 16, hipMemcpyDeviceToHost, stream));
+    MDX_HIP(hipMemcpyAsync(hv.data(), v + off, n * 16, hipMemcpyDeviceToHost, stream));
+    if (f32)
+        MDX_HIP(hipMemcpyAsync(ha.data(), static_cast<float2 *>(a) + off, n * 8, hipMemcpyDeviceToHost, stream));
+    else
+        MDX_HIP(hipMemcpyAsync(ha.data(), static_cast<double2 *>(a) + off, n * 16, hipMemcpyDeviceToHost, stream));
+    MDX_HIP(hipStreamSynchronize(stream));
+    for (int64_t i = 0; i < n; ++i) {
+        mdx_particle64 &q = out[i];
+        std::memset(&q, 0, sizeof(q));
+        q.pos[0] = hx[i].x;
+        q.pos[1] = hx[i].y;
+        q.vel[0] = hv[i].x;
+        q.vel[1] = hv[i].y;
+        if (f32) {
+            float fa[2];
+            std::memcpy(fa, &ha[i * 8], 8);
+            q.acc[0] = fa[0];
+            q.acc[1] = fa[1];
+        } else {
+            std::memcpy(q.acc, &ha[i * 16], 16);
+        }
+        q.type = s.type[i];
+        q.theta = s.theta[i];
+    }
+    return MDX_OK;
+}
+
+int Batch::cell_list(int r, int periodic_y, int32_t *ncx, int32_t *ncy, uint32_t *keys, uint32_t *cell_start,
+                     uint32_t *sorted_ids, int64_t cap_n, int64_t cap_cells) {
+    if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
+        return set_error(MDX_ERR_STATE, "slot not loaded");
+    MDX_HIP(hipSetDevice(device));
+    MDX_TRY(rebuild_cells({r}, periodic_y));
+    DevScalars h;
+    MDX_TRY(read_scalars(r, h));
+    const Slot &s = slots[r];
+    const SysParams &sp = h_sp[r];
+    const int64_t ncell = static_cast<int64_t>(h.g.ncx) * h.g.ncy;
+    if (ncx) *ncx = h.g.ncx;
+    if (ncy) *ncy = h.g.ncy;
+    if ((keys || sorted_ids) && cap_n < s.n) return set_error(MDX_ERR_CAPACITY, "cap_n below n");
+    if (cell_start && cap_cells < ncell + 1) return set_error(MDX_ERR_CAPACITY, "cap_cells below ncell+1");
+    std::vector<uint32_t> st(ncell + 1);
+    MDX_HIP(hipMemcpyAsync(st.data(), cs + sp.coff, (ncell + 1) * 4, hipMemcpyDeviceToHost, stream));
+    MDX_HIP(hipStreamSynchronize(stream));
+    const uint32_t base = st[0];
+    if (keys) {
+        MDX_HIP(hipMemcpyAsync(keys, key + sp.off, s.n * 4, hipMemcpyDeviceToHost, stream));
+        MDX_HIP(hipStreamSynchronize(stream));
+        for (int64_t i = 0; i < s.n; ++i) keys[i] -= static_cast<uint32_t>(sp.coff);
+    }
+    if (cell_start)
+        for (int64_t c = 0; c <= ncell; ++c) cell_start[c] = st[c] - base;
+    if (sorted_ids) {
+        MDX_HIP(hipMemcpyAsync(sorted_ids, sorted + base, s.n * 4, hipMemcpyDeviceToHost, stream));
+        MDX_HIP(hipStreamSynchronize(stream));
+        for (int64_t i = 0; i < s.n; ++i) sorted_ids[i] -= static_cast<uint32_t>(sp.off);
+    }
+    return MDX_OK;
+}
+
+int Batch::set_precision(int precision) {
+    if (precision < MDX_PREC_COMP32 || precision > MDX_PREC_FP64)
+        return set_error(MDX_ERR_INVALID, "unknown precision");
+    if (precision == prec) return MDX_OK;
+    MDX_HIP(hipSetDevice(device));
+    MDX_HIP(hipStreamSynchronize(stream));
+    prec = precision;
+    invalidate_graphs();
+    // accelerations and the cell image change representation: recompute
+    std::vector<int> per, pis;
+    for (size_t r = 0; r < slots.size(); ++r) {
+        if (!slots[r].loaded) continue;
+        (slots[r].piston_active ? pis : per).push_back(static_cast<int>(r));
+    }
+    if (!per.empty()) MDX_TRY(compute_forces(per, 1, false));
+    if (!pis.empty()) MDX_TRY(compute_forces(pis, 0, true));
+    MDX_HIP(hipStreamSynchronize(stream));
+    return MDX_OK;
+}
+
+// FIRE on slot r (EnergyMinimizer::minimize_frame, src/md_energy_minimizer.cu:
+// 122-272).  Standard step j runs the drift of iteration j-1 and the force
+// pass of iteration j, so max_steps iterations take max_steps + 1 steps.
+int Batch::fire(int r, const mdx_fire_params &fp, double ftol, int max_steps, mdx_fire_result *res,
+                int32_t *es, double *et, int cap) {
+    if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
+        return set_error(MDX_ERR_STATE, "slot not loaded");
+    if (max_steps <= 0) return set_error(MDX_ERR_INVALID, "EnergyMinimizer: max_steps must be positive.");
+    MDX_HIP(hipSetDevice(device));
+    Slot &s = slots[r];
+    s.piston_active = false;
+    // Overlapping starts produce accelerations beyond FP32 range: FIRE keeps
+    // FP64 accelerations (FP32 pairs with FP64 accumulation, or FP64 pairs).
+    if (prec == MDX_PREC_COMP32) MDX_TRY(set_precision(MDX_PREC_ACC64));
+    std::fill(h_want.begin(), h_want.end(), 0);
+    h_want[r] = 1;
+    MDX_HIP(hipMemcpyAsync(d_want, h_want.data(), h_want.size() * 4, hipMemcpyHostToDevice, stream));
+    if (TB > 0) k_fire_init<<<TB, kBlock, 0, stream>>>(ptrs, d_want);   // :343-380
+    MDX_HIP(hipGetLastError());
+    DevScalars h;
+    MDX_TRY(read_scalars(r, h));
+    h.f_dt = fp.dt_init;
+    h.f_alpha = fp.alpha0;
+    h.f_maxF = h.f_power = 0.0;
+    h.f_streak = h.f_done = h.f_zero_v = h.f_step = h.f_conv_step = h.f_drifted = 0;
+    h.f_do_drift = h.f_do_pair = 0;
+    MDX_TRY(write_scalar_block(r, h));
+    SysParams &sp = h_sp[r];
+    sp.ftol = ftol;
+    sp.f_inc = fp.f_inc;
+    sp.f_dec = fp.f_dec;
+    sp.dt_max = fp.dt_max;
+    sp.alpha0 = fp.alpha0;
+    sp.N_min = fp.N_min;
+    sp.max_step = fp.max_step;
+    sp.f_max_steps = max_steps;
+    int ns = 0, last_pushed = -1;
+    auto push = [&](int step, double U) {
+        if (ns < cap) {
+            es[ns] = step;
+            et[ns] = U;
+        }
+        ++ns;
+        last_pushed = step;
+    };
+    auto potential = [&](double &U) -> int {
+        MDX_TRY(obs_pass(r, 1));
+        DevScalars q;
+        MDX_TRY(read_scalars(r, q));
+        U = q.U;
+        return MDX_OK;
+    };
+    double U = 0.0;
+    MDX_TRY(potential(U));   // :145-147
+    push(0, U);
+    const int interval = fp.energy_interval > 0 ? fp.energy_interval : 1000;
+    const int total = max_steps + 1;
+    int j = 0;
+    bool converged = false;
+    while (j < total) {
+        int end = std::min(total, ((j - 1 + interval) / interval) * interval + 1);
+        if (end <= j) end = std::min(total, j + interval);
+        end = std::min(end, j + 256);
+        for (size_t q = 0; q < slots.size(); ++q) set_idle(h_sp[q], slots[q]);
+        sp.kind = SK_FIRE;
+        sp.pmode = PM_FIRE;
+        sp.periodic_y = 1;
+        sp.wrap_keys = 0;
+        MDX_TRY(upload_all_sp());
+        MDX_TRY(run_passes(end - j, false));
+        j = end;
+        MDX_TRY(read_scalars(r, h));
+        const bool sample = (j - 1) >= interval && (j - 1) % interval == 0 && h.f_drifted >= j - 1;
+        if (sample || h.f_done) MDX_TRY(potential(U));
+        if (sample) push(j - 1, U);   // :209-215
+        if (h.f_done) {
+            converged = true;
+            push(h.f_conv_step + 1, U);   // :170-178
+            break;
+        }
+        if (fp.fp64_switch > 0.0 && prec != MDX_PREC_FP64 && h.f_maxF < fp.fp64_switch) {
+            MDX_TRY(set_precision(MDX_PREC_FP64));
+        }
+    }
+    if (!converged && last_pushed != max_steps) {   // :227-233
+        MDX_TRY(potential(U));
+        push(max_steps, U);
+    }
+    res->steps = converged ? h.f_conv_step : max_steps;
+    res->converged = converged ? 1 : 0;
+    res->max_force = h.f_maxF;
+    MDX_TRY(potential(U));   // :270
+    res->potential_energy = U;
+    res->n_energy = ns;
+    res->pad = 0;
+    set_idle(sp, s);
+    s.kind = SK_NONE;
+    s.K_valid = s.W_valid = false;
+    return upload_sp(r);
+}
+
+}  // namespace mdx

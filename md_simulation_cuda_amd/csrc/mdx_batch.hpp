@@ -1,0 +1,112 @@
+// Host-side batch of independent systems ("slots") sharing one set of device
+// arrays, one HIP stream and one launch per kernel per step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "mdx.h"
+#include "mdx_sim.hpp"
+
+namespace mdx {
+
+int set_error(int code, const std::string &msg);
+
+struct Slot {
+    // capacity (fixed when the batch is built)
+    int64_t n_cap = 0;
+    int32_t ccap = 0;
+    // occupancy
+    bool loaded = false;
+    int64_t n = 0;
+    mdx_params p{};
+    double rc_max = 0.0;
+    std::vector<int32_t> type;
+    std::vector<double> theta;
+    double t = 0.0;
+    double Lx0 = 0.0, Ly0 = 0.0;
+    // validity of device-side quantities for the current positions
+    bool cells_valid = false;
+    int cells_py = 1;
+    bool K_valid = false, W_valid = false;
+    bool piston_active = false, nph_init = false, vp_init = false;
+    // what the next step does
+    int kind = SK_NONE;
+    uint32_t flags = 0;
+    double p_ext = 0.0;
+};
+
+struct Batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int prec = MDX_PREC_COMP32;
+    std::vector<Slot> slots;
+    int64_t N = 0;     // total particle capacity
+    int32_t C = 0;     // total cell capacity
+    int32_t TB = 0;    // per-particle workgroups
+    int32_t TCB = 0;   // per-cell workgroups
+    std::vector<SysParams> h_sp;
+    std::vector<int32_t> h_want;
+    // device memory
+    double2 *x = nullptr, *v = nullptr;
+    void *a = nullptr, *img = nullptr;
+    uint8_t *cls = nullptr, *icls = nullptr;
+    uint32_t *key = nullptr, *sorted = nullptr, *scell = nullptr, *count = nullptr, *cs = nullptr,
+             *bsum = nullptr;
+    double *partials = nullptr;
+    SysParams *d_sp = nullptr;
+    DevScalars *d_sc = nullptr;
+    int32_t *d_blk2sys = nullptr, *d_cblk2sys = nullptr, *d_want = nullptr;
+    BatchPtrs ptrs{};
+    // captured step graphs, by [obs]
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    bool use_graph = true;
+    // timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> force_ev, step_ev;
+    double force_ms = 0.0, step_ms = 0.0;
+    int64_t force_n = 0, step_n = 0;
+
+    ~Batch();
+    int init(int nslots, const int64_t *n_cap, const int32_t *ccap, int precision, int device_ordinal);
+
+    // slot lifecycle
+    int load(int r, const mdx_params &p, const mdx_particle64 *frame, int64_t n);
+    int unload(int r);
+    int set_state(int r, const mdx_particle64 *frame, int64_t n);
+
+    // stepping: every loaded slot with kind != SK_NONE advances nsteps
+    int step(int nsteps);
+    int set_kind(int r, int kind, double p_ext, uint32_t flags);
+
+    // passes at the current positions for the slots in `which`
+    int compute_forces(const std::vector<int> &which, int periodic_y, bool walls);
+    int observables(int r, uint32_t mask, mdx_obs *out);
+    int partial_U_lambda(int r, double eps, double *out);
+    int collect(int r, mdx_particle64 *out, int64_t cap);
+    int cell_list(int r, int periodic_y, int32_t *ncx, int32_t *ncy, uint32_t *keys, uint32_t *cell_start,
+                  uint32_t *sorted_ids, int64_t cap_n, int64_t cap_cells);
+    int fire(int r, const mdx_fire_params &fp, double ftol, int max_steps, mdx_fire_result *res,
+             int32_t *es, double *et, int cap);
+    int set_precision(int precision);
+    int read_scalars(int r, DevScalars &h);
+    int sync();
+
+    // internals
+    int upload_sp(int r);
+    int upload_all_sp();
+    int launch_step_kernels(bool obs);
+    int run_passes(int count, bool obs);
+    int pre_step();
+    int rebuild_cells(const std::vector<int> &which, int periodic_y);
+    int kinetic(const std::vector<int> &which);
+    int walls(const std::vector<int> &which);
+    int obs_pass(int r, int periodic_y);
+    void invalidate_graphs();
+    int write_scalar_block(int r, const DevScalars &h);
+};
+
+}  // namespace mdx

@@ -28,12 +28,41 @@ struct PairTable {
     float pad;
 };
 
-// Periodic wrap of include/md_cuda_common.hpp:20-29.
+// Periodic wrap of include/md_cuda_common.hpp:20-29: fmod(x, L), plus L when
+// negative.  The common cases return exactly what fmod returns (x in [0, L):
+// x; x in [L, 2L): x - L, exact by Sterbenz; x in (-L, 0): x, then x + L)
+// without calling the slow general fmod.
 __host__ __device__ inline double pbc_wrap(double x, double L) {
     if (L <= 0.0) return x;
+    if (x >= 0.0) {
+        if (x < L) return x;
+        if (x < 2.0 * L) return x - L;
+    } else if (x > -L) {
+        return x + L;
+    }
     x = fmod(x, L);
     if (x < 0.0) x += L;
     return x;
+}
+
+// Workgroup remap for the 8 XCDs of MI355X: hardware places workgroup i on
+// XCD i % 8, so consecutive ids land on different L2s.  The returned logical
+// id gives each XCD one contiguous range of logical workgroups (spatially
+// adjacent cells), keeping neighbour-cell reads in the same L2.  A bijection
+// on [0, nb).
+__device__ inline int xcd_block(int i, int nb) {
+    constexpr int kXcd = 8;
+    const int q = nb / kXcd, rem = nb % kXcd;
+    const int x = i % kXcd, k = i / kXcd;
+    return x * q + (x < rem ? x : rem) + k;
+}
+
+// 1/x in FP32: hardware reciprocal refined by one Newton step (error about
+// 1 ulp, against ~10 instructions for the IEEE division).
+__device__ inline float fast_rcp(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, r, 1.0f);
+    return __builtin_fmaf(r, e, r);
 }
 
 // Canonical cell coordinate: floor(x * n / L) from the FP64 position,
@@ -83,32 +112,35 @@ __device__ inline double wave_max(double v) {
     return v;
 }
 
-// Deterministic grid-wide reduction of NV doubles: each workgroup reduces in
-// a fixed tree and writes one partial; the workgroup that arrives last (atomic
-// ticket) combines the partials in index order.  Component k is a max when
-// bit k of MAXMASK is set, a sum otherwise.  Returns true in every thread of
-// the finishing workgroup, which then holds the totals in out[].  The ticket
-// counter is reset to zero by that workgroup, so it is ready for the next
-// launch.  Requires blockDim.x == kBlock.
-template <int NV, unsigned MAXMASK>
-__device__ bool grid_reduce(double (&v)[NV], double *partials, unsigned *ticket, double (&out)[NV]) {
+// Deterministic reduction of NV doubles over the nb workgroups of one slot:
+// each workgroup reduces in a fixed tree and writes one partial (at
+// partials[k * stride + gblk]); the workgroup that arrives last (atomic ticket)
+// combines the slot's partials, partials[k * stride + base + 0 .. nb-1], in
+// index order.  Component k is a max when bit k of maxmask is set, a sum
+// otherwise.  Returns true in every thread of the finishing workgroup, which
+// then holds the totals in out[].  That workgroup resets the ticket to zero.
+// Requires blockDim.x == kBlock; every workgroup of the slot must call it.
+template <int NV>
+__device__ bool grid_reduce(double (&v)[NV], unsigned maxmask, double *partials, int stride, int gblk,
+                            int base, int nb, unsigned *ticket, double (&out)[NV]) {
     __shared__ double sh[NV][kWaves];
     __shared__ int is_last;
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-    const int nb = gridDim.x;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        const double r = ((MAXMASK >> k) & 1u) ? wave_max(v[k]) : wave_sum(v[k]);
+        const bool mx = (maxmask >> k) & 1u;
+        const double r = mx ? wave_max(v[k]) : wave_sum(v[k]);
         if (lane == 0) sh[k][wid] = r;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
+            const bool mx = (maxmask >> k) & 1u;
             double b = sh[k][0];
-            for (int w = 1; w < kWaves; ++w) b = ((MAXMASK >> k) & 1u) ? fmax(b, sh[k][w]) : b + sh[k][w];
-            partials[k * nb + blockIdx.x] = b;
+            for (int w = 1; w < kWaves; ++w) b = mx ? fmax(b, sh[k][w]) : b + sh[k][w];
+            partials[k * stride + gblk] = b;
         }
         __threadfence();
         const unsigned t = atomicAdd(ticket, 1u);
@@ -121,25 +153,27 @@ __device__ bool grid_reduce(double (&v)[NV], double *partials, unsigned *ticket,
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        const bool mx = ((MAXMASK >> k) & 1u) != 0;
+        const bool mx = (maxmask >> k) & 1u;
         double a = mx ? -1.0e308 : 0.0;
         for (int i = threadIdx.x; i < nb; i += kBlock) {
-            const double p = vp[k * nb + i];
-            a = mx ? fmax(a, p) : a + p;
+            const double q = vp[k * stride + base + i];
+            a = mx ? fmax(a, q) : a + q;
         }
         acc[k] = a;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        const double r = ((MAXMASK >> k) & 1u) ? wave_max(acc[k]) : wave_sum(acc[k]);
+        const bool mx = (maxmask >> k) & 1u;
+        const double r = mx ? wave_max(acc[k]) : wave_sum(acc[k]);
         if (lane == 0) sh[k][wid] = r;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
+        const bool mx = (maxmask >> k) & 1u;
         double b = sh[k][0];
-        for (int w = 1; w < kWaves; ++w) b = ((MAXMASK >> k) & 1u) ? fmax(b, sh[k][w]) : b + sh[k][w];
+        for (int w = 1; w < kWaves; ++w) b = mx ? fmax(b, sh[k][w]) : b + sh[k][w];
         out[k] = b;
     }
     if (threadIdx.x == 0) *ticket = 0u;

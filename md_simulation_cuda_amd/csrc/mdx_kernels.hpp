@@ -1,85 +1,174 @@
-// gfx950 kernels of the 2D binary Lennard-Jones step.
+// gfx950 kernels of the 2D binary Lennard-Jones step, batched over slots.
 //
-// Data layout (structure of arrays, original particle order unless noted):
-//   x, v      double2[n]   FP64 master positions / velocities
-//   a         float2[n] (compensated form) or double2[n] (FP64-accumulating forms)
-//   cls       uint8[n]     type class: 0, 1, or 2 for any other type
-//   key       uint32[n]    canonical cell index of each particle
-//   count     uint32[ncell] per-cell occupancy (self-clearing: the scatter
-//                          counts it back down to zero)
-//   cs        uint32[ncell+1] cell_start (exclusive prefix of count)
-//   sorted    uint32[n]    particle ids in cell order, ascending id per cell
-//   img       float2[n] or double2[n], in cell order: cell-centre-relative FP32
-//                          position, or the wrapped FP64 position
-//   icls, scell           class and cell of each cell-ordered slot
+// Data layout (structure of arrays over the concatenated particles of all
+// slots; slot r owns [off_r, off_r + n_cap_r)):
+//   x, v      double2       FP64 master positions / velocities
+//   a         float2 (compensated FP32 form) or double2 (FP64-accumulating forms)
+//   cls       uint8         type class: 0, 1, or 2 for any other type
+//   key       uint32        global cell index of each particle
+//   count     uint32[C]     per-cell occupancy (self-clearing: the scatter
+//                           counts it back down to zero)
+//   cs        uint32[C+1]   cell start: exclusive prefix of count over all cells
+//   sorted    uint32        particle ids in cell order, ascending id per cell
+//   img       float2/double2, in cell order: cell-centre-relative FP32
+//                           position, or the wrapped FP64 position
+//   icls, scell             class and slot-local cell of each sorted slot
 //
-// One step = drift kernel (kick/drift/wrap/cell key/count, plus the ensemble's
-// reduction) -> two-pass scan -> scatter -> per-cell sort + image -> pair
-// kernel (forces, second kick, the ensemble's reduction).  Global FP64 scalars
-// (Nose-Hoover xi, NPH box and area momentum, piston, FIRE schedule) live in a
-// device struct and are finalised by the last workgroup of a reduction.
+// One step = k_box (one thread per slot: box, barostat, piston, grid, FIRE
+// bookkeeping) -> k_drift (kick/drift/wrap, cell key, count, the ensemble's
+// reduction) -> k_scan_a/k_scan_b -> k_scatter -> k_cell_image (per-cell id
+// sort + image) -> k_pair (forces, second kick, the ensemble's reduction).
+// Global FP64 scalars are finalised by the last workgroup of a slot's
+// reduction.
 #pragma once
-#include "mdx_device.hpp"
-#include "mdx_sim.hpp"
-
 #include <math.h>
 
 #include <type_traits>
 
+#include "mdx_device.hpp"
+#include "mdx_sim.hpp"
+
 namespace mdx {
 
-// --------------------------------------------------------------------------
-// drift kernels
-// --------------------------------------------------------------------------
-
-enum DriftKind { DK_KEYS = 0, DK_NVE = 1, DK_NVT = 2, DK_NPH = 3, DK_PISTON = 4, DK_FIRE = 5 };
-
 template <typename AT>
-__device__ inline double2 load_acc(const AT *a, int i);
-template <>
-__device__ inline double2 load_acc<float2>(const float2 *a, int i) {
-    const float2 q = a[i];
-    return make_double2(static_cast<double>(q.x), static_cast<double>(q.y));
-}
-template <>
-__device__ inline double2 load_acc<double2>(const double2 *a, int i) {
-    return a[i];
+__device__ inline double2 load_acc(const void *a, int i) {
+    if constexpr (std::is_same_v<AT, float2>) {
+        const float2 q = static_cast<const float2 *>(a)[i];
+        return make_double2(static_cast<double>(q.x), static_cast<double>(q.y));
+    } else {
+        return static_cast<const double2 *>(a)[i];
+    }
 }
 
-// Kick / drift / wrap of one particle by ensemble, then its cell key.
-// Restates src/md_cuda_common.cu:381-496 (VV, NH, NPH scale+drift, piston)
-// and src/md_energy_minimizer.cu:32-71 (FIRE zero/mix/drift).
-template <int KIND, typename AT>
-__global__ void __launch_bounds__(kBlock)
-k_drift(int n, double2 *__restrict__ x, double2 *__restrict__ v, const AT *__restrict__ a,
-        const uint8_t *__restrict__ cls, uint32_t *__restrict__ key, uint32_t *__restrict__ count,
-        Grid g, DriftArgs da, PairTable pt, DevScalars *sc, double *partials) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if constexpr (KIND == DK_FIRE) {
-        if (sc->f_done) return;
+// --------------------------------------------------------------------------
+// per-slot scalar kernel
+// --------------------------------------------------------------------------
+
+// One thread per slot, first kernel of a step.  NPH: area barostat of
+// src/md_env.cu:1824-1877.  Piston: src/md_env.cu:1950-1980.  FIRE: decides
+// whether the pending drift and the next force pass run.  Every slot: the
+// cell grid of its (possibly new) box.
+__global__ void k_box(BatchPtrs P) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= P.S) return;
+    const SysParams &sp = P.sp[r];
+    DevScalars &sc = P.sc[r];
+    if (sp.n <= 0) return;
+    double Lx = sc.g.Lx, Ly = sc.g.Ly;
+    if (sp.kind == SK_NPH) {
+        const double area_old = Lx * Ly;
+        sc.box_error = 0;
+        if (!(area_old > 0.0)) {
+            sc.box_error = 1;
+            sc.nph_lin = 1.0;
+            sc.nph_vsc = 1.0;
+        } else {
+            const double P_old = (2.0 * sc.K + sc.W) / (2.0 * area_old);
+            sc.P_old = P_old;
+            const double pA = sc.p_A + 0.5 * sp.dt * (P_old - sp.p_ext);
+            double area_new = area_old + sp.dt * (pA / sp.W_baro);
+            sc.p_A = pA;
+            if (!(area_new > 0.0) || !isfinite(area_new)) {
+                sc.box_error = 2;   // the reference aborts here
+                sc.nph_lin = 1.0;
+                sc.nph_vsc = 1.0;
+            } else {
+                double gam = sc.gamma;
+                if (!(gam > 0.0) || !isfinite(gam)) gam = Lx / Ly;
+                double Ly_new = sqrt(area_new / gam);
+                double Lx_new = gam * Ly_new;
+                int clamped = 0;
+                if (Ly_new < sc.Ly_floor) {
+                    Ly_new = sc.Ly_floor;
+                    Lx_new = gam * Ly_new;
+                    area_new = Lx_new * Ly_new;
+                    clamped = 1;
+                } else if (Ly_new > sc.Ly_ceil) {
+                    Ly_new = sc.Ly_ceil;
+                    Lx_new = gam * Ly_new;
+                    area_new = Lx_new * Ly_new;
+                    clamped = 1;
+                }
+                const double lin = sqrt(area_new / area_old);
+                sc.nph_lin = lin;
+                sc.nph_vsc = 1.0 / lin;
+                sc.gamma = gam;
+                sc.clamped = clamped;
+                Lx = Lx_new;
+                Ly = Ly_new;
+            }
+        }
+    } else if (sp.kind == SK_PISTON) {
+        const double acc_old = (sc.F_p - sp.p_ext * Lx) / sp.W_baro;
+        double vp = sc.v_p + 0.5 * sp.dt * acc_old;
+        double Ly_new = Ly + sp.dt * vp;
+        int clamped = 0;
+        if (Ly_new < sc.Ly_floor) {
+            Ly_new = sc.Ly_floor;
+            clamped = 1;
+            if (vp < 0.0) vp = 0.0;
+        } else if (Ly_new > sc.Ly_ceil) {
+            Ly_new = sc.Ly_ceil;
+            clamped = 1;
+            if (vp > 0.0) vp = 0.0;
+        }
+        sc.v_p = vp;
+        sc.clamped = clamped;
+        Ly = Ly_new;
+    } else if (sp.kind == SK_FIRE) {
+        const int live = !sc.f_done;
+        sc.f_do_drift = (live && sc.f_drifted < sc.f_step) ? 1 : 0;
+        if (sc.f_do_drift) sc.f_drifted = sc.f_step;
+        sc.f_do_pair = (live && sc.f_step < sp.f_max_steps) ? 1 : 0;
     }
+    Grid g = make_grid(Lx, Ly, sp.cell_side, sp.rc_max, sp.periodic_y);
+    double side = sp.cell_side;
+    while (g.ncx * g.ncy > sp.ccap) {   // never for boxes within the allocation bound
+        side *= 1.25;
+        g = make_grid(Lx, Ly, side, sp.rc_max, sp.periodic_y);
+    }
+    sc.g = g;
+}
+
+// --------------------------------------------------------------------------
+// drift kernel
+// --------------------------------------------------------------------------
+
+// Kick / drift / wrap of one particle by its slot's ensemble, then its cell
+// key.  Restates src/md_cuda_common.cu:381-496 (VV, NH, NPH scale+drift,
+// piston) and src/md_energy_minimizer.cu:32-71 (FIRE zero/mix/drift).  Slots
+// that do not move this step (idle, converged FIRE) still recompute keys so
+// the cell list is rebuilt unchanged.
+template <typename AT>
+__global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P) {
+    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int r = P.blk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    DevScalars &sc = P.sc[r];
+    const int kind = sp.kind;
+    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
+    const bool reduce = (kind == SK_NVT || kind == SK_NPH);
+    const Grid g = sc.g;
     double red = 0.0;
     double xi1 = 0.0;
-    if constexpr (KIND == DK_NVT) {
-        xi1 = sc->xi + 0.5 * da.dt * (2.0 * sc->K - da.g_dof * da.T0) / da.Q;
-    }
-    if (i < n) {
-        double2 p = x[i];
-        const double dt = da.dt;
-        if constexpr (KIND == DK_KEYS) {
-            p.x = pbc_wrap(p.x, g.Lx);
-            if (g.periodic_y) p.y = pbc_wrap(p.y, g.Ly);
-        } else {
-            double2 q = v[i];
-            const double2 acc = load_acc<AT>(a, i);
-            if constexpr (KIND == DK_NVE || KIND == DK_PISTON) {
+    if (kind == SK_NVT) xi1 = sc.xi + 0.5 * sp.dt * (2.0 * sc.K - sp.g_dof * sp.T0) / sp.Q;
+    const bool moves = kind == SK_NVE || kind == SK_NVT || kind == SK_NPH || kind == SK_PISTON ||
+                       (kind == SK_FIRE && sc.f_do_drift);
+    if (li < sp.n) {
+        const int i = sp.off + li;
+        double2 p = P.x[i];
+        const double dt = sp.dt;
+        if (moves) {
+            double2 q = P.v[i];
+            const double2 acc = load_acc<AT>(P.a, i);
+            if (kind == SK_NVE || kind == SK_PISTON) {
                 q.x += 0.5 * dt * acc.x;
                 q.y += 0.5 * dt * acc.y;
                 p.x += dt * q.x;
                 p.y += dt * q.y;
                 p.x = pbc_wrap(p.x, g.Lx);
-                if (KIND == DK_NVE) p.y = pbc_wrap(p.y, g.Ly);
-            } else if constexpr (KIND == DK_NVT) {
+                if (kind == SK_NVE) p.y = pbc_wrap(p.y, g.Ly);
+            } else if (kind == SK_NVT) {
                 double vx = q.x, vy = q.y;
                 vx += 0.5 * dt * (acc.x - xi1 * vx);
                 vy += 0.5 * dt * (acc.y - xi1 * vy);
@@ -89,14 +178,14 @@ k_drift(int n, double2 *__restrict__ x, double2 *__restrict__ v, const AT *__res
                 p.y += dt * vy;
                 p.x = pbc_wrap(p.x, g.Lx);
                 p.y = pbc_wrap(p.y, g.Ly);
-                if (da.middle_wrap) {   // src/md_cuda_common.cu:26-65
-                    double ym = fmod(p.x - da.slab_xmin, da.slab_period);
-                    if (ym < 0.0) ym += da.slab_period;
-                    if (ym > da.slab_width) ym = da.slab_period - ym;
-                    p.x = da.slab_xmin + ym;
+                if (sp.middle_wrap) {   // src/md_cuda_common.cu:26-65
+                    double ym = fmod(p.x - sp.slab_xmin, sp.slab_period);
+                    if (ym < 0.0) ym += sp.slab_period;
+                    if (ym > sp.slab_width) ym = sp.slab_period - ym;
+                    p.x = sp.slab_xmin + ym;
                 }
-            } else if constexpr (KIND == DK_NPH) {
-                const double lin = sc->nph_lin, vsc = sc->nph_vsc;
+            } else if (kind == SK_NPH) {
+                const double lin = sc.nph_lin, vsc = sc.nph_vsc;
                 q.x += 0.5 * dt * acc.x;
                 q.y += 0.5 * dt * acc.y;
                 q.x *= vsc;
@@ -105,13 +194,13 @@ k_drift(int n, double2 *__restrict__ x, double2 *__restrict__ v, const AT *__res
                 p.y = lin * p.y + dt * q.y;
                 p.x = pbc_wrap(p.x, g.Lx);
                 p.y = pbc_wrap(p.y, g.Ly);
-            } else if constexpr (KIND == DK_FIRE) {
-                if (sc->f_zero_v) {
+            } else {   // SK_FIRE
+                if (sc.f_zero_v) {
                     q.x = 0.0;
                     q.y = 0.0;
                 }
-                const double alpha = sc->f_alpha;
-                const double fdt = sc->f_dt;
+                const double alpha = sc.f_alpha;
+                const double fdt = sc.f_dt;
                 const double vmag = sqrt(q.x * q.x + q.y * q.y);
                 const double fmag = sqrt(acc.x * acc.x + acc.y * acc.y);
                 if (vmag > 0.0 && fmag > 0.0) {
@@ -122,10 +211,10 @@ k_drift(int n, double2 *__restrict__ x, double2 *__restrict__ v, const AT *__res
                     q.y = nvy;
                 }
                 double ddx = fdt * q.x, ddy = fdt * q.y;
-                if (da.max_step > 0.0) {
+                if (sp.max_step > 0.0) {
                     const double d = sqrt(ddx * ddx + ddy * ddy);
-                    if (d > da.max_step) {
-                        const double f = da.max_step / d;
+                    if (d > sp.max_step) {
+                        const double f = sp.max_step / d;
                         ddx *= f;
                         ddy *= f;
                     }
@@ -133,26 +222,26 @@ k_drift(int n, double2 *__restrict__ x, double2 *__restrict__ v, const AT *__res
                 p.x = pbc_wrap(p.x + ddx, g.Lx);
                 p.y = pbc_wrap(p.y + ddy, g.Ly);
             }
-            v[i] = q;
-            x[i] = p;
-            if constexpr (KIND == DK_NVT || KIND == DK_NPH) {
-                const double m = pt.m[cls[i]];
-                red = 0.5 * m * (q.x * q.x + q.y * q.y);
-            }
+            P.v[i] = q;
+            P.x[i] = p;
+            if (reduce) red = 0.5 * sp.pt.m[P.cls[i]] * (q.x * q.x + q.y * q.y);
+        } else if (sp.wrap_keys) {
+            p.x = pbc_wrap(p.x, g.Lx);
+            if (g.periodic_y) p.y = pbc_wrap(p.y, g.Ly);
         }
         const int cx = cell_coord(p.x, g.Lx, g.ncx);
         const int cy = cell_coord(p.y, g.Ly, g.ncy);
-        const uint32_t c = static_cast<uint32_t>(cy) * static_cast<uint32_t>(g.ncx) + cx;
-        key[i] = c;
-        atomicAdd(&count[c], 1u);
+        const uint32_t c = static_cast<uint32_t>(sp.coff) + static_cast<uint32_t>(cy) * g.ncx + cx;
+        P.key[i] = c;
+        atomicAdd(&P.count[c], 1u);
     }
-    if constexpr (KIND == DK_NVT || KIND == DK_NPH) {
+    if (reduce) {
         double vals[1] = {red};
         double out[1];
-        if (grid_reduce<1, 0u>(vals, partials, &sc->ticket[0], out) && threadIdx.x == 0) {
-            sc->K_half = out[0];
-            if constexpr (KIND == DK_NVT)
-                sc->xi_mid = xi1 + 0.5 * da.dt * (2.0 * out[0] - da.g_dof * da.T0) / da.Q;
+        if (grid_reduce<1>(vals, 0u, P.partials, P.TB, gb, sp.boff, sp.nblk, &sc.ticket[0], out) &&
+            threadIdx.x == 0) {
+            sc.K_half = out[0];
+            if (kind == SK_NVT) sc.xi_mid = xi1 + 0.5 * sp.dt * (2.0 * out[0] - sp.g_dof * sp.T0) / sp.Q;
         }
     }
 }
@@ -161,9 +250,8 @@ k_drift(int n, double2 *__restrict__ x, double2 *__restrict__ v, const AT *__res
 // cell list: scan, scatter, per-cell sort + image
 // --------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(kBlock)
-k_scan_a(const uint32_t *__restrict__ count, int ncell, uint32_t *__restrict__ bsum, const int *skip) {
-    if (skip && *skip) return;
+__global__ void __launch_bounds__(kBlock) k_scan_a(const uint32_t *__restrict__ count, int ncell,
+                                                   uint32_t *__restrict__ bsum) {
     __shared__ uint32_t sh[kWaves];
     const int base = blockIdx.x * kScanChunk + threadIdx.x * 4;
     uint32_t s = 0;
@@ -180,14 +268,11 @@ k_scan_a(const uint32_t *__restrict__ count, int ncell, uint32_t *__restrict__ b
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_scan_b(const uint32_t *__restrict__ count, int ncell, const uint32_t *__restrict__ bsum,
-         uint32_t *__restrict__ cs, const int *skip) {
-    if (skip && *skip) return;
+__global__ void __launch_bounds__(kBlock) k_scan_b(const uint32_t *__restrict__ count, int ncell,
+                                                   const uint32_t *__restrict__ bsum, uint32_t *__restrict__ cs) {
     __shared__ uint32_t shw[kWaves];
     __shared__ uint32_t shp[kWaves];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    // prefix of the preceding chunks
     uint32_t pre = 0;
     for (int b = threadIdx.x; b < static_cast<int>(blockIdx.x); b += kBlock) pre += bsum[b];
     for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o, 64);
@@ -200,7 +285,6 @@ k_scan_b(const uint32_t *__restrict__ count, int ncell, const uint32_t *__restri
         c[j] = (base + j < ncell) ? count[base + j] : 0u;
         tsum += c[j];
     }
-    // inclusive wave scan of thread sums
     uint32_t inc = tsum;
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t y = __shfl_up(inc, o, 64);
@@ -221,29 +305,30 @@ k_scan_b(const uint32_t *__restrict__ count, int ncell, const uint32_t *__restri
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) cs[ncell] = run;
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_scatter(int n, const uint32_t *__restrict__ key, uint32_t *__restrict__ count,
-          const uint32_t *__restrict__ cs, uint32_t *__restrict__ sorted, const int *skip) {
-    if (skip && *skip) return;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t c = key[i];
-    const uint32_t slot = atomicSub(&count[c], 1u) - 1u;
-    sorted[cs[c] + slot] = static_cast<uint32_t>(i);
+__global__ void __launch_bounds__(kBlock) k_scatter(BatchPtrs P) {
+    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const SysParams &sp = P.sp[P.blk2sys[gb]];
+    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
+    if (li >= sp.n) return;
+    const int i = sp.off + li;
+    const uint32_t c = P.key[i];
+    const uint32_t slot = atomicSub(&P.count[c], 1u) - 1u;
+    P.sorted[P.cs[c] + slot] = static_cast<uint32_t>(i);
 }
 
 // One thread per cell: restore ascending-id order inside the cell (the
 // scatter's atomics leave it arbitrary), then write the cell-ordered image.
 template <typename IT>
-__global__ void __launch_bounds__(kBlock)
-k_cell_image(int ncell, const uint32_t *__restrict__ cs, uint32_t *__restrict__ sorted,
-             const double2 *__restrict__ x, const uint8_t *__restrict__ cls, Grid g, int wrap,
-             IT *__restrict__ img, uint8_t *__restrict__ icls, uint32_t *__restrict__ scell,
-             const int *skip) {
-    if (skip && *skip) return;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= ncell) return;
-    const uint32_t s0 = cs[c], s1 = cs[c + 1];
+__global__ void __launch_bounds__(kBlock) k_cell_image(BatchPtrs P) {
+    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int r = P.cblk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    const Grid g = P.sc[r].g;
+    const int lc = (gb - sp.cboff) * kBlock + threadIdx.x;
+    if (sp.n <= 0 || lc >= g.ncx * g.ncy) return;
+    const int c = sp.coff + lc;
+    const uint32_t s0 = P.cs[c], s1 = P.cs[c + 1];
+    uint32_t *sorted = P.sorted;
     for (uint32_t k = s0 + 1; k < s1; ++k) {
         const uint32_t id = sorted[k];
         uint32_t j = k;
@@ -255,23 +340,24 @@ k_cell_image(int ncell, const uint32_t *__restrict__ cs, uint32_t *__restrict__ 
         }
         sorted[j] = id;
     }
-    const int cx = c % g.ncx, cy = c / g.ncx;
+    const int cx = lc % g.ncx, cy = lc / g.ncx;
     const double ox = (static_cast<double>(cx) + 0.5) * g.w;
     const double oy = (static_cast<double>(cy) + 0.5) * g.h;
+    IT *img = static_cast<IT *>(P.img);
     for (uint32_t k = s0; k < s1; ++k) {
         const uint32_t id = sorted[k];
-        double2 p = x[id];
-        if (wrap) {
+        double2 p = P.x[id];
+        if (sp.wrap_keys) {
             p.x = pbc_wrap(p.x, g.Lx);
             if (g.periodic_y) p.y = pbc_wrap(p.y, g.Ly);
         }
-        if constexpr (sizeof(IT) == sizeof(float2)) {
+        if constexpr (std::is_same_v<IT, float2>) {
             img[k] = make_float2(static_cast<float>(p.x - ox), static_cast<float>(p.y - oy));
         } else {
             img[k] = p;
         }
-        icls[k] = cls[id];
-        scell[k] = static_cast<uint32_t>(c);
+        P.icls[k] = P.cls[id];
+        P.scell[k] = static_cast<uint32_t>(lc);
     }
 }
 
@@ -279,17 +365,8 @@ k_cell_image(int ncell, const uint32_t *__restrict__ cs, uint32_t *__restrict__ 
 // pair kernel
 // --------------------------------------------------------------------------
 
-enum ForceMode { FM_PLAIN = 0, FM_OBS = 1, FM_NVE = 2, FM_NVT = 3, FM_NPH = 4, FM_PISTON = 5, FM_FIRE = 6 };
 enum Acc { ACC_COMP32 = 0, ACC_FP64 = 1 };
 
-template <typename IT>
-struct ImgScalar;
-template <>
-struct ImgScalar<float2> { using T = float; };
-template <>
-struct ImgScalar<double2> { using T = double; };
-
-// Pair accumulators of one particle.
 template <int ACC>
 struct ForceSum;
 template <>
@@ -318,8 +395,8 @@ struct ObsSum {
 // the oracle's (and the reference's, src/md_cuda_common.cu:237-254) so the
 // FP64 form reproduces the oracle bit for bit.
 template <bool OBS>
-__device__ inline bool pair_fp64(double dx, double dy, int pidx, const PairTable &pt, double &fx,
-                                 double &fy, ObsSum &ob) {
+__device__ inline bool pair_fp64(double dx, double dy, int pidx, const PairTable &pt, double &fx, double &fy,
+                                 ObsSum &ob, bool want_obs) {
     const double r2 = dx * dx + dy * dy;
     if (!(r2 > 0.0 && r2 < pt.rc2[pidx])) return false;
     const double r2i = 1.0 / r2;
@@ -330,151 +407,166 @@ __device__ inline bool pair_fp64(double dx, double dy, int pidx, const PairTable
     fx = tmp * dx;
     fy = tmp * dy;
     if constexpr (OBS) {
-        ob.u += 0.5 * (pt.eps4[pidx] * (sr12 - sr6) - pt.vc[pidx]);
-        ob.w += 0.5 * (dx * fx + dy * fy);
-        ob.lam += 0.5 * (fx * dx - fy * dy);
+        if (want_obs) {
+            ob.u += 0.5 * (pt.eps4[pidx] * (sr12 - sr6) - pt.vc[pidx]);
+            ob.w += 0.5 * (dx * fx + dy * fy);
+            ob.lam += 0.5 * (fx * dx - fy * dy);
+        }
     }
     return true;
 }
 
-// Selects entry 0 (same class), 1 (other class) of a per-particle pair set.
-struct PairSel {
-    float rc2[2], sig2[2], eps24[2], eps4[2];
-};
-
-
-// Last-workgroup finalisation of the pair kernel's reduction out[0..2].
-template <int MODE, bool OBS, bool WALLS>
-__device__ inline void finalize_pair(const double (&out)[3], const PairArgs &pa, const Grid &g,
-                                     DevScalars *sc) {
-    if constexpr (MODE == FM_OBS) {
-        sc->U = out[0];
-        sc->W = out[1];
-        sc->lam = out[2];
-    } else if constexpr (MODE == FM_PLAIN) {
-        if constexpr (OBS) {
-            sc->U = out[0];
-            sc->W = out[1];
-            sc->lam = out[2];
-        }
-        if constexpr (WALLS) {   // cal_forces_piston, src/md_env.cu:1570-1573
-            sc->F_p = out[1];
-            sc->wallE = out[2];
-        }
-    } else if constexpr (MODE == FM_NVT) {
-        sc->K = out[0];
-        sc->xi = sc->xi_mid;
-    } else if constexpr (MODE == FM_NPH) {   // src/md_env.cu:1896-1917
+// Last-workgroup finalisation of a slot's pair-kernel reduction out[0..2].
+__device__ inline void finalize_pair(int pmode, const double (&out)[3], const SysParams &sp, const Grid &g,
+                                     DevScalars &sc) {
+    switch (pmode) {
+    case PM_OBS:
+        sc.U = out[0];
+        sc.W = out[1];
+        sc.lam = out[2];
+        break;
+    case PM_WALLS:   // cal_forces_piston, src/md_env.cu:1570-1573
+        sc.F_p = out[1];
+        sc.wallE = out[2];
+        sc.last_P = (g.Lx > 0.0) ? sc.F_p / g.Lx : 0.0;
+        break;
+    case PM_NVT:
+        sc.K = out[0];
+        sc.xi = sc.xi_mid;
+        break;
+    case PM_NPH: {   // src/md_env.cu:1896-1917
         const double W = out[1];
         const double area = g.Lx * g.Ly;
-        const double P_new = (area > 0.0) ? (2.0 * sc->K_half + W) / (2.0 * area) : 0.0;
-        double pA = sc->p_A + 0.5 * pa.dt * (P_new - pa.p_ext);
-        if (sc->clamped) {
-            if (g.Ly <= sc->Ly_floor && pA < 0.0) pA = 0.0;
-            if (g.Ly >= sc->Ly_ceil && pA > 0.0) pA = 0.0;
+        const double P_new = (area > 0.0) ? (2.0 * sc.K_half + W) / (2.0 * area) : 0.0;
+        double pA = sc.p_A + 0.5 * sp.dt * (P_new - sp.p_ext);
+        if (sc.clamped) {
+            if (g.Ly <= sc.Ly_floor && pA < 0.0) pA = 0.0;
+            if (g.Ly >= sc.Ly_ceil && pA > 0.0) pA = 0.0;
         }
-        sc->p_A = pA;
-        sc->last_P = P_new;
-        sc->K = out[0];
-        sc->W = W;
-    } else if constexpr (MODE == FM_PISTON) {   // src/md_env.cu:2004-2022
-        sc->F_p = out[1];
-        sc->wallE = out[2];
-        const double acc_new = (sc->F_p - pa.p_ext * g.Lx) / pa.W_baro;
-        double vp = sc->v_p + 0.5 * pa.dt * acc_new;
-        if (sc->clamped) {
-            if (g.Ly <= sc->Ly_floor && vp < 0.0) vp = 0.0;
-            if (g.Ly >= sc->Ly_ceil && vp > 0.0) vp = 0.0;
-        }
-        sc->v_p = vp;
-        sc->last_P = sc->F_p / g.Lx;
-    } else if constexpr (MODE == FM_FIRE) {   // src/md_energy_minimizer.cu:170-203
-        const double maxF = out[0], power = out[1];
-        sc->f_maxF = maxF;
-        sc->f_power = power;
-        const int step = sc->f_step;
-        if (maxF < pa.ftol) {
-            sc->f_done = 1;
-            sc->f_conv_step = step;
-        } else if (power > 0.0) {
-            sc->f_streak += 1;
-            if (sc->f_streak > pa.N_min) {
-                const double inc = sc->f_dt * pa.f_inc;
-                sc->f_dt = (pa.dt_max < inc) ? pa.dt_max : inc;
-                const double al = sc->f_alpha * 0.99;
-                sc->f_alpha = (0.0 < al) ? al : 0.0;
-            }
-            sc->f_zero_v = 0;
-        } else {
-            sc->f_dt *= pa.f_dec;
-            sc->f_streak = 0;
-            sc->f_alpha = pa.alpha0;
-            sc->f_zero_v = 1;
-        }
-        sc->f_step = step + 1;
+        sc.p_A = pA;
+        sc.last_P = P_new;
+        sc.K = out[0];
+        sc.W = W;
+        break;
     }
-    (void)pa;
-    (void)g;
+    case PM_PISTON: {   // src/md_env.cu:2004-2022
+        sc.F_p = out[1];
+        sc.wallE = out[2];
+        const double acc_new = (sc.F_p - sp.p_ext * g.Lx) / sp.W_baro;
+        double vp = sc.v_p + 0.5 * sp.dt * acc_new;
+        if (sc.clamped) {
+            if (g.Ly <= sc.Ly_floor && vp < 0.0) vp = 0.0;
+            if (g.Ly >= sc.Ly_ceil && vp > 0.0) vp = 0.0;
+        }
+        sc.v_p = vp;
+        sc.last_P = sc.F_p / g.Lx;
+        break;
+    }
+    case PM_FIRE: {   // src/md_energy_minimizer.cu:170-203
+        const double maxF = out[0], power = out[1];
+        sc.f_maxF = maxF;
+        sc.f_power = power;
+        const int step = sc.f_step;
+        if (maxF < sp.ftol) {
+            sc.f_done = 1;
+            sc.f_conv_step = step;
+        } else if (power > 0.0) {
+            sc.f_streak += 1;
+            if (sc.f_streak > sp.N_min) {
+                const double inc = sc.f_dt * sp.f_inc;
+                sc.f_dt = (sp.dt_max < inc) ? sp.dt_max : inc;
+                const double al = sc.f_alpha * 0.99;
+                sc.f_alpha = (0.0 < al) ? al : 0.0;
+            }
+            sc.f_zero_v = 0;
+        } else {
+            sc.f_dt *= sp.f_dec;
+            sc.f_streak = 0;
+            sc.f_alpha = sp.alpha0;
+            sc.f_zero_v = 1;
+        }
+        sc.f_step = step + 1;
+        break;
+    }
+    default:
+        break;
+    }
 }
 
-template <typename IT, int ACC, int MODE, bool OBS, bool WALLS>
-__global__ void __launch_bounds__(kBlock)
-k_pair(int n, const uint32_t *__restrict__ cs, const IT *__restrict__ img,
-       const uint8_t *__restrict__ icls, const uint32_t *__restrict__ scell,
-       const uint32_t *__restrict__ sorted, Grid g, PairTable pt, PairArgs pa,
-       typename std::conditional<ACC == ACC_COMP32, float2, double2>::type *__restrict__ a,
-       double2 *__restrict__ v, const double2 *__restrict__ x, DevScalars *sc, double *partials) {
-    using T = typename ImgScalar<IT>::T;
-    using AT = typename std::conditional<ACC == ACC_COMP32, float2, double2>::type;
-    constexpr bool kFp64 = sizeof(T) == sizeof(double);
-    constexpr bool kReduce = OBS || WALLS || MODE == FM_NVT || MODE == FM_NPH || MODE == FM_FIRE;
-    if constexpr (MODE == FM_FIRE) {
-        if (sc->f_done) return;
-    }
-    const int s = blockIdx.x * kBlock + threadIdx.x;
+// Forces of every particle of every active slot (thread per cell-ordered
+// slot, 3x3 neighbour cells in the oracle's order), then the slot's mode:
+// second kick, FIRE kick, walls, and its reduction.  OBS = true compiles the
+// energy/virial/lambda sums (used when some slot needs them).
+template <typename IT, int ACC, bool OBS>
+__global__ void __launch_bounds__(kBlock) k_pair(BatchPtrs P) {
+    using T = std::conditional_t<std::is_same_v<IT, float2>, float, double>;
+    using AT = std::conditional_t<ACC == ACC_COMP32, float2, double2>;
+    constexpr bool kFp64 = std::is_same_v<T, double>;
+    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int r = P.blk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    DevScalars &sc = P.sc[r];
+    int pmode = sp.pmode;
+    if (pmode == PM_FIRE && !sc.f_do_pair) pmode = PM_NONE;
+    if (pmode == PM_NONE || sp.n <= 0) return;
+    const bool want_obs = OBS && (pmode == PM_OBS || pmode == PM_NPH);
+    const Grid g = sc.g;
+    const PairTable &pt = sp.pt;
+    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
     double r0 = 0.0, r1 = 0.0, r2v = 0.0;
-    double xi_mid = 0.0;
-    if constexpr (MODE == FM_NVT) xi_mid = sc->xi_mid;
-    double fire_dt = 0.0;
-    if constexpr (MODE == FM_FIRE) fire_dt = sc->f_dt;
-    if (s < n) {
-        const uint32_t c = scell[s];
-        const int cx = static_cast<int>(c % static_cast<uint32_t>(g.ncx));
-        const int cy = static_cast<int>(c / static_cast<uint32_t>(g.ncx));
+    if (li < sp.n) {
+        const uint32_t s = P.cs[sp.coff] + static_cast<uint32_t>(li);
+        const IT *img = static_cast<const IT *>(P.img);
+        const uint32_t lc = P.scell[s];
+        const int cx = static_cast<int>(lc % static_cast<uint32_t>(g.ncx));
+        const int cy = static_cast<int>(lc / static_cast<uint32_t>(g.ncx));
         const IT pi = img[s];
-        const int ci = icls[s];
+        const int ci = P.icls[s];
         ForceSum<kFp64 ? ACC_FP64 : ACC> F;
         ObsSum ob;
-        PairSel ps;
+        float rc2s = 0.f, rc2o = 0.f, sig2s = 0.f, sig2o = 0.f, e24s = 0.f, e24o = 0.f, e4s = 0.f, e4o = 0.f;
         if constexpr (!kFp64) {
-            const int same = ci, other = 2;
-            ps.rc2[0] = pt.rc2f[same];   ps.rc2[1] = pt.rc2f[other];
-            ps.sig2[0] = pt.sig2f[same]; ps.sig2[1] = pt.sig2f[other];
-            ps.eps24[0] = pt.eps24f[same]; ps.eps24[1] = pt.eps24f[other];
-            ps.eps4[0] = pt.eps4f[same]; ps.eps4[1] = pt.eps4f[other];
+            rc2s = pt.rc2f[ci];
+            rc2o = pt.rc2f[2];
+            sig2s = pt.sig2f[ci];
+            sig2o = pt.sig2f[2];
+            e24s = pt.eps24f[ci];
+            e24o = pt.eps24f[2];
+            e4s = pt.eps4f[ci];
+            e4o = pt.eps4f[2];
         }
         for (int oy = -1; oy <= 1; ++oy) {
             int yy = cy + oy;
             T sy = 0;
             if (g.periodic_y) {
-                if (yy < 0) { yy += g.ncy; sy = -static_cast<T>(g.Ly); }
-                else if (yy >= g.ncy) { yy -= g.ncy; sy = static_cast<T>(g.Ly); }
+                if (yy < 0) {
+                    yy += g.ncy;
+                    sy = -static_cast<T>(g.Ly);
+                } else if (yy >= g.ncy) {
+                    yy -= g.ncy;
+                    sy = static_cast<T>(g.Ly);
+                }
             } else if (yy < 0 || yy >= g.ncy) {
                 continue;
             }
             const T shy = kFp64 ? sy : static_cast<T>(oy) * static_cast<T>(g.hf);
-            const int rowbase = yy * g.ncx;
+            const int rowbase = sp.coff + yy * g.ncx;
             for (int ox = -1; ox <= 1; ++ox) {
                 int xx = cx + ox;
                 T sx = 0;
-                if (xx < 0) { xx += g.ncx; sx = -static_cast<T>(g.Lx); }
-                else if (xx >= g.ncx) { xx -= g.ncx; sx = static_cast<T>(g.Lx); }
+                if (xx < 0) {
+                    xx += g.ncx;
+                    sx = -static_cast<T>(g.Lx);
+                } else if (xx >= g.ncx) {
+                    xx -= g.ncx;
+                    sx = static_cast<T>(g.Lx);
+                }
                 const T shx = kFp64 ? sx : static_cast<T>(ox) * static_cast<T>(g.wf);
-                const uint32_t k0 = cs[rowbase + xx], k1 = cs[rowbase + xx + 1];
+                const uint32_t k0 = P.cs[rowbase + xx], k1 = P.cs[rowbase + xx + 1];
                 for (uint32_t k = k0; k < k1; ++k) {
-                    if (k == static_cast<uint32_t>(s)) continue;
+                    if (k == s) continue;
                     const IT pj = img[k];
-                    const int cj = icls[k];
+                    const int cj = P.icls[k];
                     if constexpr (kFp64) {
                         double dx = (pi.x - pj.x) - shx;
                         double dy = (pi.y - pj.y) - shy;
@@ -487,38 +579,40 @@ k_pair(int n, const uint32_t *__restrict__ cs, const IT *__restrict__ img,
                             dy = d - g.Ly * round(d / g.Ly);
                         }
                         double fx, fy;
-                        if (pair_fp64<OBS>(dx, dy, pair_index(ci, cj), pt, fx, fy, ob)) F.addd(fx, fy);
+                        if (pair_fp64<OBS>(dx, dy, pair_index(ci, cj), pt, fx, fy, ob, want_obs)) F.addd(fx, fy);
                     } else {
                         float dx = (pi.x - pj.x) - shx;
                         float dy = (pi.y - pj.y) - shy;
                         if (g.small_x) dx -= g.Lxf * rintf(dx / g.Lxf);
                         if (g.periodic_y && g.small_y) dy -= g.Lyf * rintf(dy / g.Lyf);
-                        const float rr = dx * dx + dy * dy;
-                        const int o = (cj == ci) ? 0 : 1;
-                        const float rc2 = o ? ps.rc2[1] : ps.rc2[0];
+                        const float rr = __builtin_fmaf(dx, dx, dy * dy);
+                        const bool same = cj == ci;
+                        const float rc2 = same ? rc2s : rc2o;
                         if (rr < rc2 && rr > 0.0f) {
                             if (rr < pt.r2_guard) {
                                 double fx, fy;
-                                pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy),
-                                               pair_index(ci, cj), pt, fx, fy, ob);
+                                pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), pair_index(ci, cj),
+                                               pt, fx, fy, ob, want_obs);
                                 F.addd(fx, fy);
                             } else {
-                                const float sig2 = o ? ps.sig2[1] : ps.sig2[0];
-                                const float e24 = o ? ps.eps24[1] : ps.eps24[0];
-                                const float r2i = 1.0f / rr;
+                                const float sig2 = same ? sig2s : sig2o;
+                                const float e24 = same ? e24s : e24o;
+                                const float r2i = fast_rcp(rr);
                                 const float sr2 = sig2 * r2i;
                                 const float sr6 = sr2 * sr2 * sr2;
                                 const float sr12 = sr6 * sr6;
-                                const float tmp = e24 * (2.0f * sr12 - sr6) * r2i;
+                                const float tmp = e24 * __builtin_fmaf(2.0f, sr12, -sr6) * r2i;
                                 const float fx = tmp * dx, fy = tmp * dy;
                                 F.add(fx, fy);
                                 if constexpr (OBS) {
-                                    const float e4 = o ? ps.eps4[1] : ps.eps4[0];
-                                    const float u = e4 * (sr12 - sr6);
-                                    const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
-                                    ob.u += 0.5 * (static_cast<double>(u) - pt.vc[pair_index(ci, cj)]);
-                                    ob.w += 0.5 * (ddx * dfx + ddy * dfy);
-                                    ob.lam += 0.5 * (dfx * ddx - dfy * ddy);
+                                    if (want_obs) {
+                                        const float e4 = same ? e4s : e4o;
+                                        const float u = e4 * (sr12 - sr6);
+                                        const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
+                                        ob.u += 0.5 * (static_cast<double>(u) - pt.vc[pair_index(ci, cj)]);
+                                        ob.w += 0.5 * (ddx * dfx + ddy * dfy);
+                                        ob.lam += 0.5 * (dfx * ddx - dfy * ddy);
+                                    }
                                 }
                             }
                         }
@@ -526,7 +620,7 @@ k_pair(int n, const uint32_t *__restrict__ cs, const IT *__restrict__ img,
                 }
             }
         }
-        const uint32_t id = sorted[s];
+        const uint32_t id = P.sorted[s];
         double axd, ayd;
         if constexpr (kFp64 || ACC == ACC_FP64) {
             axd = F.X() / pt.m[ci];
@@ -535,204 +629,155 @@ k_pair(int n, const uint32_t *__restrict__ cs, const IT *__restrict__ img,
             axd = F.X() * pt.inv_m[ci];
             ayd = F.Y() * pt.inv_m[ci];
         }
-        if constexpr (WALLS) {   // src/md_cuda_common.cu:498-552
-            const double y = x[id].y;
+        if (pmode == PM_WALLS || pmode == PM_PISTON) {   // src/md_cuda_common.cu:498-552
+            const double y = P.x[id].y;
+            const double k = sp.k_piston;
             double fw = 0.0, up = 0.0, e = 0.0;
-            if (pa.k_piston > 0.0) {
+            if (k > 0.0) {
                 if (y > g.Ly) {
                     const double ov = y - g.Ly;
-                    fw -= pa.k_piston * ov;
-                    up += pa.k_piston * ov;
-                    e += 0.5 * pa.k_piston * ov * ov;
+                    fw -= k * ov;
+                    up += k * ov;
+                    e += 0.5 * k * ov * ov;
                 }
                 if (y < 0.0) {
                     const double ov = -y;
-                    fw += pa.k_piston * ov;
-                    e += 0.5 * pa.k_piston * ov * ov;
+                    fw += k * ov;
+                    e += 0.5 * k * ov * ov;
                 }
             }
             if (fw != 0.0) ayd += fw / pt.m[ci];
             r1 = up;
             r2v = e;
         }
-        if constexpr (MODE != FM_OBS) {
+        if (pmode != PM_OBS) {
             AT out;
             out.x = static_cast<decltype(out.x)>(axd);
             out.y = static_cast<decltype(out.y)>(ayd);
-            a[id] = out;
+            static_cast<AT *>(P.a)[id] = out;
             // the kicks use the stored (possibly FP32-rounded) acceleration
             axd = static_cast<double>(out.x);
             ayd = static_cast<double>(out.y);
         }
-        if constexpr (MODE == FM_NVE || MODE == FM_NPH || MODE == FM_PISTON) {
-            double2 q = v[id];
-            q.x += 0.5 * pa.dt * axd;
-            q.y += 0.5 * pa.dt * ayd;
-            v[id] = q;
-            if constexpr (MODE == FM_NPH) r0 = 0.5 * pt.m[ci] * (q.x * q.x + q.y * q.y);
-        } else if constexpr (MODE == FM_NVT) {
-            double2 q = v[id];
+        if (pmode == PM_NVE || pmode == PM_NPH || pmode == PM_PISTON) {
+            double2 q = P.v[id];
+            q.x += 0.5 * sp.dt * axd;
+            q.y += 0.5 * sp.dt * ayd;
+            P.v[id] = q;
+            if (pmode == PM_NPH) r0 = 0.5 * pt.m[ci] * (q.x * q.x + q.y * q.y);
+        } else if (pmode == PM_NVT) {
+            const double xi_mid = sc.xi_mid;
+            double2 q = P.v[id];
             double vx = q.x, vy = q.y;
-            vx += 0.5 * pa.dt * (axd - xi_mid * vx);
-            vy += 0.5 * pa.dt * (ayd - xi_mid * vy);
+            vx += 0.5 * sp.dt * (axd - xi_mid * vx);
+            vy += 0.5 * sp.dt * (ayd - xi_mid * vy);
             q.x = vx;
             q.y = vy;
-            v[id] = q;
+            P.v[id] = q;
             r0 = 0.5 * pt.m[ci] * (vx * vx + vy * vy);
-        } else if constexpr (MODE == FM_FIRE) {
-            double2 q = v[id];
-            q.x += fire_dt * axd;
-            q.y += fire_dt * ayd;
-            v[id] = q;
+        } else if (pmode == PM_FIRE) {
+            const double fdt = sc.f_dt;
+            double2 q = P.v[id];
+            q.x += fdt * axd;
+            q.y += fdt * ayd;
+            P.v[id] = q;
             r0 = sqrt(axd * axd + ayd * ayd);
             r1 = q.x * axd + q.y * ayd;
         }
-        if constexpr (MODE == FM_NPH) {
+        if (pmode == PM_NPH) {
             r1 = ob.w;
-        } else if constexpr (OBS) {
+        } else if (pmode == PM_OBS) {
             r0 = ob.u;
             r1 = ob.w;
             r2v = ob.lam;
         }
-        (void)x;
     }
-    if constexpr (kReduce) {
+    const bool reduce = pmode == PM_OBS || pmode == PM_WALLS || pmode == PM_NVT || pmode == PM_NPH ||
+                        pmode == PM_PISTON || pmode == PM_FIRE;
+    if (reduce) {
         double vals[3] = {r0, r1, r2v};
         double out[3];
-        constexpr unsigned mm = (MODE == FM_FIRE) ? 1u : 0u;
-        if (grid_reduce<3, mm>(vals, partials, &sc->ticket[1], out) && threadIdx.x == 0)
-            finalize_pair<MODE, OBS, WALLS>(out, pa, g, sc);
+        const unsigned mm = (pmode == PM_FIRE) ? 1u : 0u;
+        if (grid_reduce<3>(vals, mm, P.partials, P.TB, gb, sp.boff, sp.nblk, &sc.ticket[1], out) &&
+            threadIdx.x == 0)
+            finalize_pair(pmode, out, sp, g, sc);
     }
 }
 
 // --------------------------------------------------------------------------
-// small kernels: kinetic energy, walls, box updates, FIRE set-up
+// small kernels: kinetic energy, walls, FIRE set-up
 // --------------------------------------------------------------------------
 
-// K = sum 1/2 m v^2 (src/md_cuda_common.cu:554-589).
-__global__ void __launch_bounds__(kBlock)
-k_kinetic(int n, const double2 *__restrict__ v, const uint8_t *__restrict__ cls, PairTable pt,
-          DevScalars *sc, double *partials) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+// K = sum 1/2 m v^2 per slot (src/md_cuda_common.cu:554-589); slots with
+// want[r] == 0 are skipped.
+__global__ void __launch_bounds__(kBlock) k_kinetic(BatchPtrs P, const int32_t *want) {
+    const int gb = blockIdx.x;
+    const int r = P.blk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    if (!want[r] || sp.n <= 0) return;
+    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
     double k = 0.0;
-    if (i < n) {
-        const double2 q = v[i];
-        k = 0.5 * pt.m[cls[i]] * (q.x * q.x + q.y * q.y);
+    if (li < sp.n) {
+        const int i = sp.off + li;
+        const double2 q = P.v[i];
+        k = 0.5 * sp.pt.m[P.cls[i]] * (q.x * q.x + q.y * q.y);
     }
     double vals[1] = {k};
     double out[1];
-    if (grid_reduce<1, 0u>(vals, partials, &sc->ticket[2], out) && threadIdx.x == 0) sc->K = out[0];
+    if (grid_reduce<1>(vals, 0u, P.partials, P.TB, gb, sp.boff, sp.nblk, &P.sc[r].ticket[2], out) &&
+        threadIdx.x == 0)
+        P.sc[r].K = out[0];
 }
 
 // Piston wall observables without touching accelerations
 // (src/md_env.cu:2148-2203 with update_particle_acc = false).
-__global__ void __launch_bounds__(kBlock)
-k_walls(int n, const double2 *__restrict__ x, double Ly, double k_piston, DevScalars *sc,
-        double *partials) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_walls(BatchPtrs P, const int32_t *want) {
+    const int gb = blockIdx.x;
+    const int r = P.blk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    if (!want[r] || sp.n <= 0) return;
+    const double Ly = P.sc[r].g.Ly, k = sp.k_piston;
+    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
     double up = 0.0, e = 0.0;
-    if (i < n && k_piston > 0.0) {
-        const double y = x[i].y;
+    if (li < sp.n && k > 0.0) {
+        const double y = P.x[sp.off + li].y;
         if (y > Ly) {
             const double ov = y - Ly;
-            up += k_piston * ov;
-            e += 0.5 * k_piston * ov * ov;
+            up += k * ov;
+            e += 0.5 * k * ov * ov;
         }
         if (y < 0.0) {
             const double ov = -y;
-            e += 0.5 * k_piston * ov * ov;
+            e += 0.5 * k * ov * ov;
         }
     }
     double vals[2] = {up, e};
     double out[2];
-    if (grid_reduce<2, 0u>(vals, partials, &sc->ticket[3], out) && threadIdx.x == 0) {
-        sc->F_p = out[0];
-        sc->wallE = out[1];
+    if (grid_reduce<2>(vals, 0u, P.partials, P.TB, gb, sp.boff, sp.nblk, &P.sc[r].ticket[3], out) &&
+        threadIdx.x == 0) {
+        P.sc[r].F_p = out[0];
+        P.sc[r].wallE = out[1];
     }
-}
-
-// NPH box update of src/md_env.cu:1824-1877, one thread.
-__global__ void k_box_nph(double dt, double p_ext, double W_baro, DevScalars *sc) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const double Lx_old = sc->Lx, Ly_old = sc->Ly;
-    const double area_old = Lx_old * Ly_old;
-    sc->box_error = 0;
-    if (!(area_old > 0.0)) {
-        sc->box_error = 1;
-        return;
-    }
-    const double P_old = (2.0 * sc->K + sc->W) / (2.0 * area_old);
-    sc->P_old = P_old;
-    double pA = sc->p_A + 0.5 * dt * (P_old - p_ext);
-    double area_new = area_old + dt * (pA / W_baro);
-    if (!(area_new > 0.0) || !isfinite(area_new)) {
-        sc->box_error = 2;
-        sc->p_A = pA;
-        return;
-    }
-    double gam = sc->gamma;
-    if (!(gam > 0.0) || !isfinite(gam)) gam = Lx_old / Ly_old;
-    double Ly_new = sqrt(area_new / gam);
-    double Lx_new = gam * Ly_new;
-    int clamped = 0;
-    if (Ly_new < sc->Ly_floor) {
-        Ly_new = sc->Ly_floor;
-        Lx_new = gam * Ly_new;
-        area_new = Lx_new * Ly_new;
-        clamped = 1;
-    } else if (Ly_new > sc->Ly_ceil) {
-        Ly_new = sc->Ly_ceil;
-        Lx_new = gam * Ly_new;
-        area_new = Lx_new * Ly_new;
-        clamped = 1;
-    }
-    const double lin = sqrt(area_new / area_old);
-    sc->nph_lin = lin;
-    sc->nph_vsc = 1.0 / lin;
-    sc->gamma = gam;
-    sc->p_A = pA;
-    sc->clamped = clamped;
-    sc->Lx = Lx_new;
-    sc->Ly = Ly_new;
-}
-
-// Piston update of src/md_env.cu:1950-1980, one thread.
-__global__ void k_box_piston(double dt, double p_ext, double W_baro, DevScalars *sc) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const double Lx = sc->Lx, Ly_old = sc->Ly;
-    const double acc_old = (sc->F_p - p_ext * Lx) / W_baro;
-    double vp = sc->v_p + 0.5 * dt * acc_old;
-    double Ly_new = Ly_old + dt * vp;
-    int clamped = 0;
-    if (Ly_new < sc->Ly_floor) {
-        Ly_new = sc->Ly_floor;
-        clamped = 1;
-        if (vp < 0.0) vp = 0.0;
-    } else if (Ly_new > sc->Ly_ceil) {
-        Ly_new = sc->Ly_ceil;
-        clamped = 1;
-        if (vp > 0.0) vp = 0.0;
-    }
-    sc->v_p = vp;
-    sc->clamped = clamped;
-    sc->Ly = Ly_new;
-    sc->box_error = (Ly_new > 0.0 && isfinite(Ly_new)) ? 0 : 3;
 }
 
 // EnergyMinimizer::initialize_from_frame (src/md_energy_minimizer.cu:343-380):
-// wrap positions, zero velocities and accelerations (a buffer viewed as 16 B
-// per particle whatever its precision).
-__global__ void __launch_bounds__(kBlock)
-k_fire_init(int n, double2 *__restrict__ x, double2 *__restrict__ v, double2 *__restrict__ a16,
-            double Lx, double Ly) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    double2 p = x[i];
-    p.x = pbc_wrap(p.x, Lx);
-    p.y = pbc_wrap(p.y, Ly);
-    x[i] = p;
-    v[i] = make_double2(0.0, 0.0);
-    a16[i] = make_double2(0.0, 0.0);
+// wrap positions, zero velocities and accelerations (the acceleration buffer
+// is viewed as 16 bytes per particle whatever its precision).
+__global__ void __launch_bounds__(kBlock) k_fire_init(BatchPtrs P, const int32_t *want) {
+    const int gb = blockIdx.x;
+    const int r = P.blk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    if (!want[r]) return;
+    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
+    if (li >= sp.n) return;
+    const int i = sp.off + li;
+    const Grid g = P.sc[r].g;
+    double2 p = P.x[i];
+    p.x = pbc_wrap(p.x, g.Lx);
+    p.y = pbc_wrap(p.y, g.Ly);
+    P.x[i] = p;
+    P.v[i] = make_double2(0.0, 0.0);
+    static_cast<double2 *>(P.a)[i] = make_double2(0.0, 0.0);
 }
 
 }  // namespace mdx

@@ -1,4 +1,12 @@
-// Device-resident FP64 scalars of one system and the kernel argument packs.
+// Per-system parameters (host-written) and device-resident scalars
+// (device-written) of a batch of independent systems.
+//
+// A batch owns S slots.  Slot r owns particles [off_r, off_r + n_cap_r) of the
+// concatenated particle arrays, cells [coff_r, coff_r + ccap_r) of the
+// concatenated cell arrays, per-particle workgroups [boff_r, boff_r + nblk_r)
+// and per-cell workgroups [cboff_r, cboff_r + ncblk_r).  Every kernel covers
+// all slots in one launch; a workgroup belongs to exactly one slot, so each
+// slot's reductions depend only on its own particles.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,51 +18,116 @@
 
 namespace mdx {
 
-// Global scalars of one system.  Written by single-thread box kernels or by
-// the last workgroup of a grid reduction; read by later kernels.
+enum StepKind : int32_t {
+    SK_NONE = 0,     // slot idle this launch
+    SK_NVE = 1,
+    SK_NVT = 2,
+    SK_NPH = 3,
+    SK_PISTON = 4,
+    SK_FIRE = 5,
+    SK_KEYS = 6,     // rebuild the cell list at the current positions only
+};
+
+// Pair-kernel modes (per slot, runtime).
+enum PairMode : int32_t {
+    PM_NONE = 0,
+    PM_PLAIN = 1,    // accelerations only (cal_forces)
+    PM_WALLS = 2,    // accelerations + piston walls (cal_forces_piston)
+    PM_OBS = 3,      // U, W, lambda sums; no writes to particles
+    PM_NVE = 4,
+    PM_NVT = 5,
+    PM_NPH = 6,
+    PM_PISTON = 7,
+    PM_FIRE = 8,
+};
+
+struct SysParams {
+    int32_t n;           // particles in the slot (<= n_cap)
+    int32_t off;         // first particle
+    int32_t n_cap;
+    int32_t coff;        // first cell
+    int32_t ccap;        // cell capacity
+    int32_t boff, nblk;  // per-particle workgroups
+    int32_t cboff, ncblk;
+    int32_t kind;        // StepKind of the drift kernel for this slot
+    int32_t pmode;       // PairMode of the pair kernel for this slot
+    int32_t periodic_y;  // grid of the step: 1 except piston
+    int32_t middle_wrap;
+    int32_t wrap_keys;   // SK_KEYS: wrap positions for keys/image
+    double cell_side;    // cell side target = cell_scale * rc_max
+    double rc_max;
+    double dt, T0, Q, g_dof;
+    double slab_xmin, slab_period, slab_width;   // middle_reflect_LG
+    double p_ext, W_baro, k_piston;
+    // FIRE
+    double ftol, f_inc, f_dec, dt_max, alpha0, max_step;
+    int32_t N_min, f_max_steps;
+    PairTable pt;
+};
+
+// Device-written scalars of one slot.
 struct DevScalars {
-    double xi;        // Nose-Hoover variable after the last completed half update
-    double xi_mid;    // xi after the first and second half updates of the step in flight
-    double K;         // kinetic energy of the current velocities
-    double K_half;    // kinetic energy after the first kick (NVT: v^{n+1/2}; NPH: scaled)
-    double W;         // 1/2 sum_ordered r.F at the current positions
-    double U;
-    double lam;       // 1/2 sum_ordered (Fx dx - Fy dy) (unscaled)
-    double wallE;
-    double Lx, Ly;    // current box
-    double p_A;       // NPH area momentum
-    double gamma;     // NPH fixed aspect ratio
-    double nph_lin, nph_vsc;
-    double P_old;
-    double v_p;       // piston velocity
-    double F_p;       // piston force (upper wall)
-    double last_P;
+    Grid g;              // current cell grid (box lives in g.Lx, g.Ly)
+    double xi;           // Nose-Hoover variable after the last completed half update
+    double xi_mid;       // xi of the step in flight (after both half updates)
+    double K;            // kinetic energy of the current velocities
+    double K_half;       // kinetic energy after the first kick
+    double W, U, lam, wallE;
+    double p_A, gamma, nph_lin, nph_vsc, P_old;
+    double v_p, F_p, last_P;
     double Ly_floor, Ly_ceil;
-    int32_t clamped;
-    int32_t box_error;
+    int32_t clamped, box_error;
     // FIRE schedule (src/md_energy_minimizer.cu:170-203)
     double f_dt, f_alpha, f_maxF, f_power;
-    int32_t f_streak, f_done, f_zero_v, f_step, f_conv_step, f_pad;
+    int32_t f_streak, f_done, f_zero_v, f_step;
+    int32_t f_conv_step, f_drifted, f_do_drift, f_do_pair;
     unsigned ticket[4];
 };
 
-struct DriftArgs {
-    double dt, T0, Q, g_dof;
-    int middle_wrap;
-    int pad;
-    double slab_xmin, slab_period, slab_width;
-    double max_step;
+// Pointers of a batch, passed by value to every kernel.
+struct BatchPtrs {
+    double2 *x, *v;
+    void *a;
+    uint8_t *cls;
+    uint32_t *key, *count, *cs, *sorted, *scell;
+    void *img;
+    uint8_t *icls;
+    double *partials;       // [3][TB]
+    const SysParams *sp;
+    DevScalars *sc;
+    const int32_t *blk2sys;   // per-particle workgroup -> slot
+    const int32_t *cblk2sys;  // per-cell workgroup -> slot
+    int32_t TB;               // total per-particle workgroups
+    int32_t S;                // slots
+    int32_t C;                // total cells
+    int32_t pad;
 };
 
-struct PairArgs {
-    double dt;
-    double p_ext;
-    double W_baro;       // max(barostat_mass, 1e-12)
-    double k_piston;
-    // FIRE
-    double ftol, f_inc, f_dec, dt_max, alpha0;
-    int N_min;
-    int pad;
-};
+// Canonical grid of a box (the oracle evaluates the same expressions).
+__host__ __device__ inline int grid_dim(double L, double side, bool periodic) {
+    double q = floor(L / side);
+    const int lo = periodic ? 3 : 1;
+    if (!(q >= lo)) return lo;
+    if (q > 1.0e8) q = 1.0e8;
+    return static_cast<int>(q);
+}
+
+__host__ __device__ inline Grid make_grid(double Lx, double Ly, double side, double rc_max, int periodic_y) {
+    Grid g{};
+    g.Lx = Lx;
+    g.Ly = Ly;
+    g.ncx = grid_dim(Lx, side, true);
+    g.ncy = grid_dim(Ly, side, periodic_y != 0);
+    g.w = Lx / g.ncx;
+    g.h = Ly / g.ncy;
+    g.Lxf = static_cast<float>(Lx);
+    g.Lyf = static_cast<float>(Ly);
+    g.wf = static_cast<float>(g.w);
+    g.hf = static_cast<float>(g.h);
+    g.periodic_y = periodic_y;
+    g.small_x = g.w < rc_max ? 1 : 0;
+    g.small_y = g.h < rc_max ? 1 : 0;
+    return g;
+}
 
 }  // namespace mdx

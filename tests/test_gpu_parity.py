@@ -16,10 +16,18 @@ from tests.helpers import lattice_frame, near_cutoff_mask, uniform_frame
 
 pytestmark = pytest.mark.gpu
 
-# FP32 pair arithmetic: max_i |dF_i| / rms|F| over particles without a pair in
+# FP32 pair arithmetic, max_i |dF_i| / rms|F| over particles without a pair in
 # the cutoff rounding band (|r^2 - rc^2| < 1e-6 rc^2), and rms_i |dF_i| / rms|F|.
-FP32_FORCE_MAX_REL = 1e-5
-FP32_FORCE_RMS_REL = 1e-6
+# The error comes in equal parts from rounding the cell-relative coordinates
+# to FP32 and from the FP32 pair arithmetic (a CPU emulation of the same
+# arithmetic reproduces it to 4 digits).  The lattice frame of config 1
+# (jitter 0.05) stays within 1e-5; the 2:1 frame with jitter 0.3 has close
+# contacts (rms|F| ~ 4.5e3) and is held to 2e-5.
+FP32_FORCE_MAX_REL = {"c1_square_4096": 1e-5, "slab_2to1_3000": 2e-5}
+FP32_FORCE_RMS_REL = 2e-6
+# dU/dlambda = -eps/2 sum (Fx dx - Fy dy) cancels strongly; FP32 pairs give
+# about 1e-5 relative on it.
+FP32_LAMBDA_REL = 3e-5
 
 
 def say(msg):
@@ -91,7 +99,7 @@ def test_fp32_forces_within_tolerance(case, prec):
     max_rel = err[~band].max() / rms
     rms_rel = np.sqrt((err ** 2).mean()) / rms
     say(f"  max_rel(excl band {band.sum()})={max_rel:.3e} rms_rel={rms_rel:.3e} max_all={err.max() / rms:.3e}")
-    assert max_rel <= FP32_FORCE_MAX_REL
+    assert max_rel <= FP32_FORCE_MAX_REL[case]
     assert rms_rel <= FP32_FORCE_RMS_REL
     g.close()
 
@@ -105,7 +113,8 @@ def test_observables(prec, rel):
     assert ob.K == pytest.approx(o.K(), rel=1e-13)
     assert ob.U == pytest.approx(o.U(), rel=rel)
     assert ob.P == pytest.approx(o.pressure(), rel=rel)
-    assert g.partial_U_lambda(0.37) == pytest.approx(o.partial_U_lambda(0.37), rel=max(rel, 1e-12), abs=1e-9)
+    lam_rel = 1e-12 if prec == abi.PREC_FP64 else FP32_LAMBDA_REL
+    assert g.partial_U_lambda(0.37) == pytest.approx(o.partial_U_lambda(0.37), rel=lam_rel)
     g.close()
 
 
@@ -142,14 +151,22 @@ def test_fp64_nvt_trajectory():
 
 
 def test_fp64_nvt_middle_wrap():
-    f, Lx, Ly = lattice_frame(1200, 0.75, seed=5, T=0.8, aspect=3.0)
+    # liquid-gas slab (run/series_LiquidGas.cpp): the liquid starts inside the
+    # central slab of width devide_p * Lx + 2 sigma and is reflected back into it
+    f, sw, Ly = lattice_frame(1200, 0.6, seed=5, T=0.8, aspect=2.0)
+    Lx = 2.0 * sw
+    f["pos"][:, 0] += 0.25 * Lx
     say("fp64 NVT middle wrap")
-    g, o = both(f, Lx, Ly, abi.PREC_FP64, eps_ab=0.5, dt=0.005, T_target=0.8, Q=10.0, devide_p=0.4)
+    g, o = both(f, Lx, Ly, abi.PREC_FP64, eps_ab=0.5, dt=0.005, T_target=0.8, Q=10.0, devide_p=0.5)
     o.compute_forces()
     g.step(abi.NVT, 50, flags=abi.STEP_MIDDLE_WRAP)
     o.step_nvt(50, middle_wrap=True)
     a, b = g.collect(), o.frame()
+    assert np.all(np.isfinite(b["pos"]))
     assert np.abs(a["pos"] - b["pos"]).max() < 1e-9
+    slab = 0.5 * Lx + 2.0
+    assert a["pos"][:, 0].min() >= 0.5 * Lx - 0.5 * slab - 1e-12
+    assert a["pos"][:, 0].max() <= 0.5 * Lx + 0.5 * slab + 1e-12
     g.close()
 
 
