@@ -105,7 +105,8 @@ Batch::~Batch() {
     if (stream) (void)hipStreamSynchronize(stream);
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
-                     d_sp, d_sc, d_blk2sys, d_cblk2sys, d_want};
+                     d_sp, d_sc, d_blk2sys, d_cblk2sys, d_want, sorted_old, cs_old, movers, mover_off,
+                     n_movers};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
     for (auto *vec : {&force_ev, &step_ev})
@@ -183,6 +184,13 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     MDX_HIP(hipMalloc(&sorted, n4));
     MDX_HIP(hipMalloc(&scell, n4));
     MDX_HIP(hipMalloc(&count, (static_cast<size_t>(C) + 1) * 4));
+    MDX_HIP(hipMalloc(&sorted_old, n4));
+    MDX_HIP(hipMalloc(&cs_old, (static_cast<size_t>(C) + 1) * 4));
+    MDX_HIP(hipMalloc(&movers, n4));
+    MDX_HIP(hipMalloc(&mover_off, n4));
+    MDX_HIP(hipMalloc(&n_movers, 4));
+    MDX_HIP(hipMemset(n_movers, 0, 4));
+    MDX_HIP(hipMemset(key, 0, n4));
     MDX_HIP(hipMalloc(&cs, (static_cast<size_t>(C) + 1) * 4));
     MDX_HIP(hipMalloc(&bsum, static_cast<size_t>(ceil_div(C, kScanChunk) + 1) * 4));
     MDX_HIP(hipMalloc(&partials, static_cast<size_t>(std::max(TB, 1)) * 3 * sizeof(double)));
@@ -211,6 +219,11 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     ptrs.img = img;
     ptrs.icls = icls;
     ptrs.partials = partials;
+    ptrs.sorted_old = sorted_old;
+    ptrs.cs_old = cs_old;
+    ptrs.movers = movers;
+    ptrs.mover_off = mover_off;
+    ptrs.n_movers = n_movers;
     ptrs.sp = d_sp;
     ptrs.sc = d_sc;
     ptrs.blk2sys = d_blk2sys;
@@ -257,13 +270,16 @@ int Batch::launch_step_kernels(bool obs) {
             k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs);
         else
             k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs);
+        k_finalize<<<S, kBlock, 0, stream>>>(ptrs, FIN_DRIFT, d_want);
     }
+    if (TCB > 0) k_stay<<<TCB, kBlock, 0, stream>>>(ptrs);
+    if (TB > 0) k_arrive<<<std::min(TB, 1024), kBlock, 0, stream>>>(ptrs);
     const int nchunk = ceil_div(C, kScanChunk);
     if (nchunk > 0) {
         k_scan_a<<<nchunk, kBlock, 0, stream>>>(count, C, bsum);
         k_scan_b<<<nchunk, kBlock, 0, stream>>>(count, C, bsum, cs);
     }
-    if (TB > 0) k_scatter<<<TB, kBlock, 0, stream>>>(ptrs);
+    if (TCB > 0) k_fill<<<TCB, kBlock, 0, stream>>>(ptrs);
     if (TCB > 0) {
         if (prec == MDX_PREC_FP64)
             k_cell_image<double2><<<TCB, kBlock, 0, stream>>>(ptrs);
@@ -293,6 +309,7 @@ int Batch::launch_step_kernels(bool obs) {
             (void)hipEventRecord(e1, stream);
             force_ev.emplace_back(e0, e1);
         }
+        k_finalize<<<S, kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
     }
     MDX_HIP(hipGetLastError());
     return MDX_OK;
@@ -457,7 +474,10 @@ int Batch::kinetic(const std::vector<int> &which) {
     std::fill(h_want.begin(), h_want.end(), 0);
     for (int r : which) h_want[r] = 1;
     MDX_HIP(hipMemcpyAsync(d_want, h_want.data(), h_want.size() * 4, hipMemcpyHostToDevice, stream));
-    if (TB > 0) k_kinetic<<<TB, kBlock, 0, stream>>>(ptrs, d_want);
+    if (TB > 0) {
+        k_kinetic<<<TB, kBlock, 0, stream>>>(ptrs, d_want);
+        k_finalize<<<static_cast<int>(slots.size()), kBlock, 0, stream>>>(ptrs, FIN_KINETIC, d_want);
+    }
     MDX_HIP(hipGetLastError());
     for (int r : which) slots[r].K_valid = true;
     return MDX_OK;
@@ -468,7 +488,10 @@ int Batch::walls(const std::vector<int> &which) {
     std::fill(h_want.begin(), h_want.end(), 0);
     for (int r : which) h_want[r] = 1;
     MDX_HIP(hipMemcpyAsync(d_want, h_want.data(), h_want.size() * 4, hipMemcpyHostToDevice, stream));
-    if (TB > 0) k_walls<<<TB, kBlock, 0, stream>>>(ptrs, d_want);
+    if (TB > 0) {
+        k_walls<<<TB, kBlock, 0, stream>>>(ptrs, d_want);
+        k_finalize<<<static_cast<int>(slots.size()), kBlock, 0, stream>>>(ptrs, FIN_WALLS, d_want);
+    }
     MDX_HIP(hipGetLastError());
     return MDX_OK;
 }

@@ -56,6 +56,8 @@ struct Batch {
     uint8_t *cls = nullptr, *icls = nullptr;
     uint32_t *key = nullptr, *sorted = nullptr, *scell = nullptr, *count = nullptr, *cs = nullptr,
              *bsum = nullptr;
+    uint32_t *sorted_old = nullptr, *cs_old = nullptr, *movers = nullptr, *mover_off = nullptr,
+             *n_movers = nullptr;
     double *partials = nullptr;
     SysParams *d_sp = nullptr;
     DevScalars *d_sc = nullptr;

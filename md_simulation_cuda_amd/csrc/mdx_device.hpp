@@ -112,19 +112,15 @@ __device__ inline double wave_max(double v) {
     return v;
 }
 
-// Deterministic reduction of NV doubles over the nb workgroups of one slot:
-// each workgroup reduces in a fixed tree and writes one partial (at
-// partials[k * stride + gblk]); the workgroup that arrives last (atomic ticket)
-// combines the slot's partials, partials[k * stride + base + 0 .. nb-1], in
-// index order.  Component k is a max when bit k of maxmask is set, a sum
-// otherwise.  Returns true in every thread of the finishing workgroup, which
-// then holds the totals in out[].  That workgroup resets the ticket to zero.
-// Requires blockDim.x == kBlock; every workgroup of the slot must call it.
+// Workgroup part of a deterministic two-kernel reduction: the workgroup
+// reduces NV doubles in a fixed tree and thread 0 stores one partial per
+// component at partials[k * stride + gblk].  Component k is a max when bit k of
+// maxmask is set, a sum otherwise.  A later kernel (slot_reduce) combines a
+// slot's partials in index order; the kernel boundary orders the stores, so no
+// device-scope fence or atomic is needed.  Requires blockDim.x == kBlock.
 template <int NV>
-__device__ bool grid_reduce(double (&v)[NV], unsigned maxmask, double *partials, int stride, int gblk,
-                            int base, int nb, unsigned *ticket, double (&out)[NV]) {
+__device__ void block_partials(double (&v)[NV], unsigned maxmask, double *partials, int stride, int gblk) {
     __shared__ double sh[NV][kWaves];
-    __shared__ int is_last;
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
 #pragma unroll
@@ -142,30 +138,26 @@ __device__ bool grid_reduce(double (&v)[NV], unsigned maxmask, double *partials,
             for (int w = 1; w < kWaves; ++w) b = mx ? fmax(b, sh[k][w]) : b + sh[k][w];
             partials[k * stride + gblk] = b;
         }
-        __threadfence();
-        const unsigned t = atomicAdd(ticket, 1u);
-        is_last = (t == static_cast<unsigned>(nb - 1)) ? 1 : 0;
     }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    const volatile double *vp = partials;
-    double acc[NV];
+}
+
+// Second part: one workgroup combines nb partials (partials[k * stride + base
+// + i], i < nb) in a fixed order.  Every thread returns the totals.
+template <int NV>
+__device__ void slot_reduce(unsigned maxmask, const double *partials, int stride, int base, int nb,
+                            double (&out)[NV]) {
+    __shared__ double sh[NV][kWaves];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         const bool mx = (maxmask >> k) & 1u;
         double a = mx ? -1.0e308 : 0.0;
         for (int i = threadIdx.x; i < nb; i += kBlock) {
-            const double q = vp[k * stride + base + i];
+            const double q = partials[k * stride + base + i];
             a = mx ? fmax(a, q) : a + q;
         }
-        acc[k] = a;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const bool mx = (maxmask >> k) & 1u;
-        const double r = mx ? wave_max(acc[k]) : wave_sum(acc[k]);
+        const double r = mx ? wave_max(a) : wave_sum(a);
         if (lane == 0) sh[k][wid] = r;
     }
     __syncthreads();
@@ -176,8 +168,6 @@ __device__ bool grid_reduce(double (&v)[NV], unsigned maxmask, double *partials,
         for (int w = 1; w < kWaves; ++w) b = mx ? fmax(b, sh[k][w]) : b + sh[k][w];
         out[k] = b;
     }
-    if (threadIdx.x == 0) *ticket = 0u;
-    return true;
 }
 
 }  // namespace mdx

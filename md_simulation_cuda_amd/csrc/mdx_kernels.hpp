@@ -6,8 +6,7 @@
 //   a         float2 (compensated FP32 form) or double2 (FP64-accumulating forms)
 //   cls       uint8         type class: 0, 1, or 2 for any other type
 //   key       uint32        global cell index of each particle
-//   count     uint32[C]     per-cell occupancy (self-clearing: the scatter
-//                           counts it back down to zero)
+//   count     uint32[C]     per-cell occupancy of the list being built
 //   cs        uint32[C+1]   cell start: exclusive prefix of count over all cells
 //   sorted    uint32        particle ids in cell order, ascending id per cell
 //   img       float2/double2, in cell order: cell-centre-relative FP32
@@ -16,8 +15,10 @@
 //
 // One step = k_box (one thread per slot: box, barostat, piston, grid, FIRE
 // bookkeeping) -> k_drift (kick/drift/wrap, cell key, count, the ensemble's
-// reduction) -> k_scan_a/k_scan_b -> k_scatter -> k_cell_image (per-cell id
-// sort + image) -> k_pair (forces, second kick, the ensemble's reduction).
+// reduction, movers) -> k_stay -> k_arrive -> k_scan_a/k_scan_b -> k_fill
+// -> k_cell_image (per-cell id sort + image) -> k_pair (forces, second kick,
+// the ensemble's reduction) -> k_finalize.  The cell list is updated from the
+// previous one: only particles whose cell changed ("movers") use atomics.
 // Global FP64 scalars are finalised by the last workgroup of a slot's
 // reduction.
 #pragma once
@@ -121,6 +122,7 @@ __global__ void k_box(BatchPtrs P) {
         if (sc.f_do_drift) sc.f_drifted = sc.f_step;
         sc.f_do_pair = (live && sc.f_step < sp.f_max_steps) ? 1 : 0;
     }
+    sc.g_old = sc.g;
     Grid g = make_grid(Lx, Ly, sp.cell_side, sp.rc_max, sp.periodic_y);
     double side = sp.cell_side;
     while (g.ncx * g.ncy > sp.ccap) {   // never for boxes within the allocation bound
@@ -128,6 +130,28 @@ __global__ void k_box(BatchPtrs P) {
         g = make_grid(Lx, Ly, side, sp.rc_max, sp.periodic_y);
     }
     sc.g = g;
+}
+
+// Half update of the Nose-Hoover variable, src/md_env.cu:1668-1673:
+// xi += 0.5 dt (2K - g T0) / Q.
+__device__ inline double nh_half_update(double xi, double K, const SysParams &sp) {
+    return xi + 0.5 * sp.dt * (2.0 * K - sp.g_dof * sp.T0) / sp.Q;
+}
+
+// Appends the calling lanes with mover == true to the batch's mover list: one
+// atomic per wave that has movers.
+__device__ inline void append_movers(const BatchPtrs &P, bool mover, uint32_t pid) {
+    const unsigned long long mask = __ballot(mover);
+    if (mask == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(P.n_movers, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader, 64);
+    if (mover) {
+        const unsigned long long below = mask & ((1ull << lane) - 1ull);
+        P.movers[base + static_cast<uint32_t>(__popcll(below))] = pid;
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -151,9 +175,11 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P) {
     const Grid g = sc.g;
     double red = 0.0;
     double xi1 = 0.0;
-    if (kind == SK_NVT) xi1 = sc.xi + 0.5 * sp.dt * (2.0 * sc.K - sp.g_dof * sp.T0) / sp.Q;
+    if (kind == SK_NVT) xi1 = nh_half_update(sc.xi, sc.K, sp);
     const bool moves = kind == SK_NVE || kind == SK_NVT || kind == SK_NPH || kind == SK_PISTON ||
                        (kind == SK_FIRE && sc.f_do_drift);
+    bool mover = false;
+    uint32_t pid = 0;
     if (li < sp.n) {
         const int i = sp.off + li;
         double2 p = P.x[i];
@@ -232,17 +258,14 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P) {
         const int cx = cell_coord(p.x, g.Lx, g.ncx);
         const int cy = cell_coord(p.y, g.Ly, g.ncy);
         const uint32_t c = static_cast<uint32_t>(sp.coff) + static_cast<uint32_t>(cy) * g.ncx + cx;
+        mover = !sc.list_valid || P.key[i] != c;
         P.key[i] = c;
-        atomicAdd(&P.count[c], 1u);
+        pid = static_cast<uint32_t>(i);
     }
+    append_movers(P, mover, pid);
     if (reduce) {
         double vals[1] = {red};
-        double out[1];
-        if (grid_reduce<1>(vals, 0u, P.partials, P.TB, gb, sp.boff, sp.nblk, &sc.ticket[0], out) &&
-            threadIdx.x == 0) {
-            sc.K_half = out[0];
-            if (kind == SK_NVT) sc.xi_mid = xi1 + 0.5 * sp.dt * (2.0 * out[0] - sp.g_dof * sp.T0) / sp.Q;
-        }
+        block_partials<1>(vals, 0u, P.partials, P.TB, gb);
     }
 }
 
@@ -305,15 +328,62 @@ __global__ void __launch_bounds__(kBlock) k_scan_b(const uint32_t *__restrict__ 
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) cs[ncell] = run;
 }
 
-__global__ void __launch_bounds__(kBlock) k_scatter(BatchPtrs P) {
+// One thread per cell of the new grid (and of the previous grid when it was
+// larger): count[c] = particles of the previous list of cell c that stay in c.
+// Cells beyond the new grid get zero.
+__global__ void __launch_bounds__(kBlock) k_stay(BatchPtrs P) {
     const int gb = xcd_block(blockIdx.x, gridDim.x);
-    const SysParams &sp = P.sp[P.blk2sys[gb]];
-    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
-    if (li >= sp.n) return;
-    const int i = sp.off + li;
-    const uint32_t c = P.key[i];
-    const uint32_t slot = atomicSub(&P.count[c], 1u) - 1u;
-    P.sorted[P.cs[c] + slot] = static_cast<uint32_t>(i);
+    const int r = P.cblk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    const DevScalars &sc = P.sc[r];
+    const int lc = (gb - sp.cboff) * kBlock + threadIdx.x;
+    const int n_new = sc.g.ncx * sc.g.ncy;
+    const int n_old = sc.list_valid ? sc.g_old.ncx * sc.g_old.ncy : 0;
+    // without a previous list (new or reused slot) every cell of the slot is
+    // cleared, whatever an earlier occupant left in it
+    const int limit = sc.list_valid ? (n_new > n_old ? n_new : n_old) : sp.ccap;
+    if (lc >= limit || lc >= sp.ccap) return;
+    const uint32_t c = static_cast<uint32_t>(sp.coff + lc);
+    uint32_t stay = 0;
+    if (sp.n > 0 && lc < n_old && lc < n_new) {
+        const uint32_t s0 = P.cs_old[c], s1 = P.cs_old[c + 1];
+        for (uint32_t k = s0; k < s1; ++k) stay += (P.key[P.sorted_old[k]] == c) ? 1u : 0u;
+    }
+    P.count[c] = stay;
+}
+
+// Movers take an offset inside their new cell, after the stayers.
+__global__ void __launch_bounds__(kBlock) k_arrive(BatchPtrs P) {
+    const uint32_t nm = *P.n_movers;
+    for (uint32_t m = blockIdx.x * kBlock + threadIdx.x; m < nm; m += gridDim.x * kBlock)
+        P.mover_off[m] = atomicAdd(&P.count[P.key[P.movers[m]]], 1u);
+}
+
+// Stayers keep their previous order (ascending id); movers go to their
+// offsets.  One thread per cell of the new grid, then a grid-stride loop over
+// the movers.
+__global__ void __launch_bounds__(kBlock) k_fill(BatchPtrs P) {
+    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int r = P.cblk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    const DevScalars &sc = P.sc[r];
+    const int lc = (gb - sp.cboff) * kBlock + threadIdx.x;
+    const int n_new = sc.g.ncx * sc.g.ncy;
+    const int n_old = sc.list_valid ? sc.g_old.ncx * sc.g_old.ncy : 0;
+    if (sp.n > 0 && lc < n_new && lc < n_old) {
+        const uint32_t c = static_cast<uint32_t>(sp.coff + lc);
+        uint32_t dst = P.cs[c];
+        const uint32_t s0 = P.cs_old[c], s1 = P.cs_old[c + 1];
+        for (uint32_t k = s0; k < s1; ++k) {
+            const uint32_t id = P.sorted_old[k];
+            if (P.key[id] == c) P.sorted[dst++] = id;
+        }
+    }
+    const uint32_t nm = *P.n_movers;
+    for (uint32_t m = blockIdx.x * kBlock + threadIdx.x; m < nm; m += gridDim.x * kBlock) {
+        const uint32_t id = P.movers[m];
+        P.sorted[P.cs[P.key[id]] + P.mover_off[m]] = id;
+    }
 }
 
 // One thread per cell: restore ascending-id order inside the cell (the
@@ -325,9 +395,13 @@ __global__ void __launch_bounds__(kBlock) k_cell_image(BatchPtrs P) {
     const SysParams &sp = P.sp[r];
     const Grid g = P.sc[r].g;
     const int lc = (gb - sp.cboff) * kBlock + threadIdx.x;
+    if (gb == 0 && threadIdx.x == 0) *P.n_movers = 0u;   // every reader of this step's list has run
     if (sp.n <= 0 || lc >= g.ncx * g.ncy) return;
+    if (lc == 0) P.sc[r].list_valid = 1;
     const int c = sp.coff + lc;
     const uint32_t s0 = P.cs[c], s1 = P.cs[c + 1];
+    P.cs_old[c] = s0;
+    if (lc == g.ncx * g.ncy - 1) P.cs_old[c + 1] = s1;
     uint32_t *sorted = P.sorted;
     for (uint32_t k = s0 + 1; k < s1; ++k) {
         const uint32_t id = sorted[k];
@@ -358,6 +432,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_image(BatchPtrs P) {
         }
         P.icls[k] = P.cls[id];
         P.scell[k] = static_cast<uint32_t>(lc);
+        P.sorted_old[k] = id;
     }
 }
 
@@ -696,11 +771,54 @@ __global__ void __launch_bounds__(kBlock) k_pair(BatchPtrs P) {
                         pmode == PM_PISTON || pmode == PM_FIRE;
     if (reduce) {
         double vals[3] = {r0, r1, r2v};
+        block_partials<3>(vals, pmode == PM_FIRE ? 1u : 0u, P.partials, P.TB, gb);
+    }
+}
+
+
+// --------------------------------------------------------------------------
+// per-slot finalisation of the reductions
+// --------------------------------------------------------------------------
+
+enum FinStage : int32_t { FIN_DRIFT = 0, FIN_PAIR = 1, FIN_KINETIC = 2, FIN_WALLS = 3 };
+
+// One workgroup per slot: combines the slot's workgroup partials written by
+// the preceding kernel and updates the slot's scalars.
+__global__ void __launch_bounds__(kBlock) k_finalize(BatchPtrs P, int32_t stage, const int32_t *want) {
+    const int r = blockIdx.x;
+    const SysParams &sp = P.sp[r];
+    DevScalars &sc = P.sc[r];
+    if (sp.n <= 0 || sp.nblk <= 0) return;
+    if (stage == FIN_DRIFT) {
+        if (sp.kind != SK_NVT && sp.kind != SK_NPH) return;
+        double out[1];
+        slot_reduce<1>(0u, P.partials, P.TB, sp.boff, sp.nblk, out);
+        if (threadIdx.x == 0) {
+            sc.K_half = out[0];
+            if (sp.kind == SK_NVT) sc.xi_mid = nh_half_update(nh_half_update(sc.xi, sc.K, sp), out[0], sp);
+        }
+    } else if (stage == FIN_PAIR) {
+        int pmode = sp.pmode;
+        if (pmode == PM_FIRE && !sc.f_do_pair) return;
+        const bool reduce = pmode == PM_OBS || pmode == PM_WALLS || pmode == PM_NVT || pmode == PM_NPH ||
+                            pmode == PM_PISTON || pmode == PM_FIRE;
+        if (!reduce) return;
         double out[3];
-        const unsigned mm = (pmode == PM_FIRE) ? 1u : 0u;
-        if (grid_reduce<3>(vals, mm, P.partials, P.TB, gb, sp.boff, sp.nblk, &sc.ticket[1], out) &&
-            threadIdx.x == 0)
-            finalize_pair(pmode, out, sp, g, sc);
+        slot_reduce<3>(pmode == PM_FIRE ? 1u : 0u, P.partials, P.TB, sp.boff, sp.nblk, out);
+        if (threadIdx.x == 0) finalize_pair(pmode, out, sp, sc.g, sc);
+    } else if (stage == FIN_KINETIC) {
+        if (!want[r]) return;
+        double out[1];
+        slot_reduce<1>(0u, P.partials, P.TB, sp.boff, sp.nblk, out);
+        if (threadIdx.x == 0) sc.K = out[0];
+    } else if (stage == FIN_WALLS) {
+        if (!want[r]) return;
+        double out[2];
+        slot_reduce<2>(0u, P.partials, P.TB, sp.boff, sp.nblk, out);
+        if (threadIdx.x == 0) {
+            sc.F_p = out[0];
+            sc.wallE = out[1];
+        }
     }
 }
 
@@ -723,10 +841,7 @@ __global__ void __launch_bounds__(kBlock) k_kinetic(BatchPtrs P, const int32_t *
         k = 0.5 * sp.pt.m[P.cls[i]] * (q.x * q.x + q.y * q.y);
     }
     double vals[1] = {k};
-    double out[1];
-    if (grid_reduce<1>(vals, 0u, P.partials, P.TB, gb, sp.boff, sp.nblk, &P.sc[r].ticket[2], out) &&
-        threadIdx.x == 0)
-        P.sc[r].K = out[0];
+    block_partials<1>(vals, 0u, P.partials, P.TB, gb);
 }
 
 // Piston wall observables without touching accelerations
@@ -752,12 +867,7 @@ __global__ void __launch_bounds__(kBlock) k_walls(BatchPtrs P, const int32_t *wa
         }
     }
     double vals[2] = {up, e};
-    double out[2];
-    if (grid_reduce<2>(vals, 0u, P.partials, P.TB, gb, sp.boff, sp.nblk, &P.sc[r].ticket[3], out) &&
-        threadIdx.x == 0) {
-        P.sc[r].F_p = out[0];
-        P.sc[r].wallE = out[1];
-    }
+    block_partials<2>(vals, 0u, P.partials, P.TB, gb);
 }
 
 // EnergyMinimizer::initialize_from_frame (src/md_energy_minimizer.cu:343-380):

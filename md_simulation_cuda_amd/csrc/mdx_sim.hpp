@@ -68,6 +68,9 @@ struct SysParams {
 // Device-written scalars of one slot.
 struct DevScalars {
     Grid g;              // current cell grid (box lives in g.Lx, g.Ly)
+    Grid g_old;          // grid of the previous cell list
+    int32_t list_valid;  // the previous cell list (cs_old, sorted_old) holds every particle
+    int32_t pad0;
     double xi;           // Nose-Hoover variable after the last completed half update
     double xi_mid;       // xi of the step in flight (after both half updates)
     double K;            // kinetic energy of the current velocities
@@ -81,7 +84,6 @@ struct DevScalars {
     double f_dt, f_alpha, f_maxF, f_power;
     int32_t f_streak, f_done, f_zero_v, f_step;
     int32_t f_conv_step, f_drifted, f_do_drift, f_do_pair;
-    unsigned ticket[4];
 };
 
 // Pointers of a batch, passed by value to every kernel.
@@ -93,6 +95,9 @@ struct BatchPtrs {
     void *img;
     uint8_t *icls;
     double *partials;       // [3][TB]
+    uint32_t *sorted_old, *cs_old;   // previous cell list
+    uint32_t *movers, *mover_off;    // particles whose cell changed, and their offset in it
+    uint32_t *n_movers;              // mover count of the step (reset by k_cell_image)
     const SysParams *sp;
     DevScalars *sc;
     const int32_t *blk2sys;   // per-particle workgroup -> slot
