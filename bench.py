@@ -49,7 +49,32 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64-accumulating comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="nvt1m", choices=["nvt1m", "ensemble", "serve"],
+                    help="nvt1m: N=1M Nose-Hoover (default line); ensemble: 256 NVT replicas of uneven N "
+                         "in one batch vs per-replica launches; serve: Poisson job stream through the "
+                         "continuous-batching server")
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--per-replica-steps", type=int, default=100)
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times (scale for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain copy kernel of known size: reads and writes mib MiB per launch."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_mib": mib, "launches": 5, "bytes_read_per_launch": n * 4,
+                      "bytes_written_per_launch": n * 4}), flush=True)
 
 
 def make_workload(n):
@@ -115,12 +140,198 @@ def cpu_baseline(frame, Lx, Ly, kw, budget_s):
     }
 
 
+def ensemble_replicas(count, seed=4004):
+    """NVT replicas: N log-uniform in [2048, 65536], T ~ U[0.4, 1.0], eps_AB ~
+    U[0.3, 0.9], density ~ U[0.70, 0.85], square box, jittered lattice start
+    (+-0.05) with Maxwell-Boltzmann velocities at T; replica r uses seed
+    4004 * 1000 + r for its frame."""
+    import numpy as np
+    from md_simulation_cuda_amd import abi
+
+    rng = np.random.default_rng(seed)
+    reps = []
+    for r in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(2048), math.log(65536)))))
+        T = float(rng.uniform(0.4, 1.0))
+        eab = float(rng.uniform(0.3, 0.9))
+        rho = float(rng.uniform(0.70, 0.85))
+        frame, Lx, Ly = abi.gen_lattice(n, n // 2, rho, 0.05, T, 4004 * 1000 + r)
+        reps.append((frame, Lx, Ly, dict(eps_ab=eab, dt=0.005, T_target=T, Q=10.0)))
+    return reps
+
+
+def run_ensemble(args):
+    from md_simulation_cuda_amd import abi
+
+    reps = ensemble_replicas(args.replicas)
+    total_n = sum(len(f) for f, _, _, _ in reps)
+    prec = abi.PREC_COMP32
+    # batched: every replica a slot of one batch, one launch per kernel per step
+    caps = [len(f) for f, _, _, _ in reps]
+    ccaps = [abi.cell_capacity(abi.make_params(Lx, Ly, precision=prec, **kw), 1.0) for f, Lx, Ly, kw in reps]
+    b = abi.Batch(caps, ccaps, precision=prec)
+    for r, (f, Lx, Ly, kw) in enumerate(reps):
+        b.load(r, abi.make_params(Lx, Ly, precision=prec, **kw), f)
+        b.set_kind(r, abi.NVT)
+    b.step(args.warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    value = total_n * args.steps / wall
+    b.timing_enable(True)
+    b.step(min(args.steps, 100))
+    tm = b.timing_read()
+    b.timing_enable(False)
+    b.close()
+    # per-replica launches: each replica its own handle, stream and step graph
+    sims = [abi.Simulation(abi.make_params(Lx, Ly, precision=prec, **kw), f) for f, Lx, Ly, kw in reps]
+    for s in sims:
+        s.step(abi.NVT, 5)
+    for s in sims:
+        s.sync()
+    k = args.per_replica_steps
+    t0 = time.perf_counter()
+    for s in sims:
+        s.step(abi.NVT, k)
+    for s in sims:
+        s.sync()
+    wall_pr = time.perf_counter() - t0
+    for s in sims:
+        s.close()
+    value_pr = total_n * k / wall_pr
+    out = {
+        "metric": "ensemble atom-steps/s (256 NVT replicas, one batch)",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair + fp32 two-sum accumulation, fp64 state",
+        "data": "synthetic (seeded jittered lattices, MB velocities)",
+        "config": {"workload": f"{len(reps)} NVT replicas, N log-uniform [2048, 65536] (sum {total_n}), "
+                               "T U[0.4,1], eps_AB U[0.3,0.9], rho U[0.70,0.85], dt 0.005, Q 10",
+                   "replicas": len(reps), "total_atoms": total_n},
+        "pair_avg_launch_ms": tm.force_ms / max(1, tm.force_launches),
+        "per_replica_launches": {"value": value_pr, "steps": k, "wall_s": wall_pr},
+        "speedup_vs_per_replica": value / value_pr,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_serve(args):
+    """Poisson job stream through the server: kinds NVE/NVT/NPH/FIRE with
+    weights 0.3/0.4/0.2/0.1, N log-uniform in [1024, 262144], steps log-uniform
+    in [500, 20000], the binary LJ family (T U[0.4,1], eps_AB U[0.3,0.9],
+    density U[0.70,0.85]); frames are built when a job arrives."""
+    import numpy as np
+    from md_simulation_cuda_amd import abi
+
+    rng = np.random.default_rng(5005)
+    jobs = []
+    t = 0.0
+    while True:
+        t += float(rng.exponential(1.0 / args.serve_rate))
+        if t >= args.serve_seconds:
+            break
+        kind = int(rng.choice([abi.NVE, abi.NVT, abi.NPH, abi.FIRE], p=[0.3, 0.4, 0.2, 0.1]))
+        n = int(round(math.exp(rng.uniform(math.log(1024), math.log(262144)))))
+        steps = int(round(math.exp(rng.uniform(math.log(500), math.log(20000)))))
+        jobs.append(dict(t=t, kind=kind, n=n, steps=steps, T=float(rng.uniform(0.4, 1.0)),
+                         eab=float(rng.uniform(0.3, 0.9)), rho=float(rng.uniform(0.70, 0.85)),
+                         seed=int(rng.integers(1, 2 ** 31))))
+    fp = abi.FireParams(0.005, 0.05, 0.1, 1.1, 0.5, 5, 1000, 0.0, 0.0)
+    srv = abi.Server()
+    offered = sum(j["n"] * j["steps"] for j in jobs) / args.serve_seconds
+    idx = 0
+    lat = {}
+    done_ids = set()
+    id2job = {}
+    t_start = srv.now()
+    last_print = time.perf_counter()
+    while len(done_ids) < len(jobs):
+        now = srv.now() - t_start
+        while idx < len(jobs) and jobs[idx]["t"] <= now:
+            j = jobs[idx]
+            T = 0.0 if j["kind"] == abi.FIRE else j["T"]
+            f, Lx, Ly = abi.gen_lattice(j["n"], j["n"] // 2, j["rho"], 0.2 if j["kind"] == abi.FIRE else 0.05,
+                                        T, j["seed"])
+            p = abi.make_params(Lx, Ly, eps_ab=j["eab"], dt=0.005, T_target=j["T"], Q=10.0, barostat_mass=50.0)
+            jid = srv.submit(abi.make_job(p, j["kind"], j["steps"], p_ext=1.0, force_tol=1e-6, fire=fp,
+                                          t_arrival=t_start + j["t"]), f)
+            id2job[jid] = j
+            idx += 1
+        nr, nq, fin = srv.tick()
+        for jid in fin:
+            info = srv.info(jid)
+            lat[jid] = (info.t_done - info.t_arrival, id2job[jid]["kind"])
+            srv.result(jid, release=True)
+            done_ids.add(jid)
+        if nr == 0 and nq == 0 and idx < len(jobs):
+            time.sleep(max(0.0, min(0.05, jobs[idx]["t"] - (srv.now() - t_start))))
+        if time.perf_counter() - last_print > 30:
+            last_print = time.perf_counter()
+            print(f"serve: t={srv.now() - t_start:.0f}s submitted {idx}/{len(jobs)} done {len(done_ids)} "
+                  f"running {nr} queued {nq}", file=sys.stderr, flush=True)
+    wall = srv.now() - t_start
+    st = srv.stats()
+    srv.close()
+    L = np.array([v[0] for v in lat.values()])
+    by_kind = {}
+    for name, k in (("nve", abi.NVE), ("nvt", abi.NVT), ("nph", abi.NPH), ("fire", abi.FIRE)):
+        x = np.array([v[0] for v in lat.values() if v[1] == k])
+        if len(x):
+            by_kind[name] = {"jobs": int(len(x)), "p50_s": float(np.percentile(x, 50)),
+                             "p99_s": float(np.percentile(x, 99))}
+    out = {
+        "metric": "served job stream atom-steps/s + p50/p99 job latency",
+        "value": st.atom_steps / wall,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair + fp32 two-sum accumulation (MD), fp32 pair + fp64 accumulation (FIRE)",
+        "data": "synthetic (seeded Poisson arrivals and jittered lattices)",
+        "config": {"workload": f"Poisson {args.serve_rate} jobs/s for {args.serve_seconds} s, "
+                               "NVE/NVT/NPH/FIRE 0.3/0.4/0.2/0.1, N log-uniform [1024, 262144], "
+                               "steps log-uniform [500, 20000]",
+                   "jobs": len(jobs), "offered_atom_steps_per_s": offered},
+        "latency_p50_s": float(np.percentile(L, 50)),
+        "latency_p99_s": float(np.percentile(L, 99)),
+        "latency_by_kind": by_kind,
+        "wall_s": wall,
+        "ticks": st.ticks,
+        "peak_reserved_bytes": st.peak_reserved_bytes,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(multi-GPU slab decomposition is a design note only)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        run_ensemble(args)
+        return
+    if args.workload == "serve":
+        run_serve(args)
+        return
     from md_simulation_cuda_amd import abi
 
     prec = {"comp32": abi.PREC_COMP32, "acc64": abi.PREC_ACC64, "fp64": abi.PREC_FP64}[args.precision]

@@ -231,6 +231,98 @@ int mdx_batch_sync(mdx_batch *b);
 int mdx_batch_set_graph(mdx_batch *b, int32_t enable);
 int mdx_batch_timing_enable(mdx_batch *b, int32_t enable);
 int mdx_batch_timing_read(mdx_batch *b, mdx_timing *out);
+/* FIRE as a slot kind: after mdx_batch_fire_begin the slot takes one FIRE
+ * iteration per mdx_batch_step step (EnergyMinimizer::minimize_frame,
+ * src/md_energy_minimizer.cu:122-272, without the energy trace) while the
+ * other slots do whatever their kind says; max_steps iterations take
+ * max_steps + 1 steps.  The batch must be ACC64 or FP64.  mdx_batch_fire_poll
+ * sets *done once the slot converged or ran out of iterations, and then also
+ * fills result->potential_energy and returns the slot to MDX_IDLE. */
+int mdx_batch_fire_begin(mdx_batch *b, int32_t slot, const mdx_fire_params *fp, double force_tol,
+                         int32_t max_steps);
+int mdx_batch_fire_poll(mdx_batch *b, int32_t slot, int32_t *done, mdx_fire_result *result);
+
+/* ---- job server: continuous batching of MD jobs -----------------------------
+ * No counterpart in the reference, whose drivers (the run/ programs) run one system per
+ * process.  Jobs (NVE, NVT, NPH, piston or FIRE, any N up to the largest slot)
+ * are submitted at any time; each mdx_server_tick admits queued jobs into free
+ * device-resident slots (smallest power-of-two size class that fits, bytes of
+ * admitted slots within hbm_budget_bytes, arrival order without head-of-line
+ * blocking), advances every running job by up to `quantum` steps in the same
+ * launches, records K/U/P of every running MD job, and retires finished jobs
+ * (final frame copied to the host, slot freed).  MD jobs run in a batch of
+ * md_precision, FIRE jobs in an ACC64 batch; both batches' launches overlap on
+ * the device.  A job's result is the same as the same system run alone through
+ * mdx_create + mdx_step (or mdx_fire with fp64_switch = 0). */
+#define MDX_FIRE 4            /* mdx_job.kind for a FIRE minimisation */
+#define MDX_JOB_QUEUED 0
+#define MDX_JOB_RUNNING 1
+#define MDX_JOB_DONE 2
+
+typedef struct mdx_server_params {
+    double hbm_budget_bytes;      /* admission budget for resident slots (default 288e9) */
+    double min_density;           /* sizes the slots' cell capacity (default 0.3)        */
+    double max_box_growth;        /* box growth the slots' cells allow (default 1.4)     */
+    int32_t min_class_log2;       /* smallest slot: 2^min_class_log2 particles (10)      */
+    int32_t max_class_log2;       /* largest slot (18)                                   */
+    int32_t md_slots_per_class;   /* MD slots per size class (6)                         */
+    int32_t fire_slots_per_class; /* FIRE slots per size class (2)                       */
+    int32_t quantum;              /* most steps per tick (100)                           */
+    int32_t record_energies;      /* 1: K/U/P of every running MD job every tick         */
+    int32_t md_precision;         /* mdx_precision of the MD batch (COMP32)              */
+    int32_t device;               /* -1: current                                         */
+} mdx_server_params;
+
+typedef struct mdx_job {
+    mdx_params params;            /* box and interactions; params.precision is ignored   */
+    int32_t kind;                 /* MDX_NVE, MDX_NVT, MDX_NPH, MDX_PISTON or MDX_FIRE   */
+    int32_t nsteps;               /* MD steps, or FIRE max iterations                    */
+    double p_ext;                 /* NPH / piston target pressure                        */
+    double force_tol;             /* FIRE: converged when max|a| < force_tol             */
+    mdx_fire_params fire;         /* FIRE parameters (energy_interval, fp64_switch unused) */
+    double t_arrival;             /* server clock seconds (mdx_server_now); < 0: now     */
+} mdx_job;
+
+typedef struct mdx_energy_record {
+    int64_t step;
+    double K, U, P;
+} mdx_energy_record;
+
+typedef struct mdx_job_info {
+    int64_t id;
+    int32_t state;                /* MDX_JOB_* */
+    int32_t kind;
+    int64_t n, steps_done;
+    double t_arrival, t_admit, t_done;   /* server clock seconds; -1 if not yet */
+    mdx_obs last;                 /* latest K/U/P (MD) or final PE in U (FIRE)         */
+    mdx_fire_result fire;         /* FIRE jobs */
+    int32_t n_records;
+    int32_t pad;
+} mdx_job_info;
+
+typedef struct mdx_server_stats {
+    double atom_steps;            /* sum over ticks of steps x N of every running job   */
+    double reserved_bytes, peak_reserved_bytes;
+    int64_t ticks, completed, queued, slots;
+} mdx_server_counters;
+
+typedef struct mdx_server mdx_server;
+
+int mdx_server_default_params(mdx_server_params *out);
+int mdx_server_create(const mdx_server_params *params, mdx_server **out);
+int mdx_server_destroy(mdx_server *s);
+int mdx_server_now(mdx_server *s, double *seconds);
+int mdx_server_submit(mdx_server *s, const mdx_job *job, const mdx_particle64 *frame, int64_t n,
+                      int64_t *job_id);
+/* One scheduling round; ids of the jobs it finished go to done_ids[0..cap_done). */
+int mdx_server_tick(mdx_server *s, int32_t *n_running, int32_t *n_queued, int64_t *done_ids,
+                    int32_t cap_done, int32_t *n_done);
+int mdx_server_info(mdx_server *s, int64_t job_id, mdx_job_info *out);
+/* Energy records of a job (*n receives the total, at most cap are copied). */
+int mdx_server_records(mdx_server *s, int64_t job_id, mdx_energy_record *out, int32_t cap, int32_t *n);
+/* Final frame of a finished job (original particle order); release != 0 forgets the job. */
+int mdx_server_result(mdx_server *s, int64_t job_id, mdx_particle64 *frame, int64_t cap, int32_t release);
+int mdx_server_stats(mdx_server *s, mdx_server_counters *out);
 
 /* ---- initial frames (host only, no GPU) --------------------------------- */
 

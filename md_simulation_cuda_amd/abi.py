@@ -90,8 +90,62 @@ class InitParams(C.Structure):
     ]
 
 
+FIRE = 4   # job kind of the server
+JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
+
+
+class ServerParams(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_double), ("min_density", C.c_double), ("max_box_growth", C.c_double),
+        ("min_class_log2", C.c_int32), ("max_class_log2", C.c_int32),
+        ("md_slots_per_class", C.c_int32), ("fire_slots_per_class", C.c_int32),
+        ("quantum", C.c_int32), ("record_energies", C.c_int32), ("md_precision", C.c_int32),
+        ("device", C.c_int32),
+    ]
+
+
+class Job(C.Structure):
+    _fields_ = [
+        ("params", Params), ("kind", C.c_int32), ("nsteps", C.c_int32), ("p_ext", C.c_double),
+        ("force_tol", C.c_double), ("fire", FireParams), ("t_arrival", C.c_double),
+    ]
+
+
+class EnergyRecord(C.Structure):
+    _fields_ = [("step", C.c_int64), ("K", C.c_double), ("U", C.c_double), ("P", C.c_double)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [
+        ("id", C.c_int64), ("state", C.c_int32), ("kind", C.c_int32), ("n", C.c_int64),
+        ("steps_done", C.c_int64), ("t_arrival", C.c_double), ("t_admit", C.c_double),
+        ("t_done", C.c_double), ("last", Obs), ("fire", FireResult), ("n_records", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [
+        ("atom_steps", C.c_double), ("reserved_bytes", C.c_double), ("peak_reserved_bytes", C.c_double),
+        ("ticks", C.c_int64), ("completed", C.c_int64), ("queued", C.c_int64), ("slots", C.c_int64),
+    ]
+
+
 _P = C.c_void_p
 _SIGNATURES = {
+    "mdx_batch_fire_begin": (C.c_int, [_P, C.c_int32, C.POINTER(FireParams), C.c_double, C.c_int32]),
+    "mdx_batch_fire_poll": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(FireResult)]),
+    "mdx_server_default_params": (C.c_int, [C.POINTER(ServerParams)]),
+    "mdx_server_create": (C.c_int, [C.POINTER(ServerParams), C.POINTER(_P)]),
+    "mdx_server_destroy": (C.c_int, [_P]),
+    "mdx_server_now": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "mdx_server_submit": (C.c_int, [_P, C.POINTER(Job), _P, C.c_int64, C.POINTER(C.c_int64)]),
+    "mdx_server_tick": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P, C.c_int32,
+                                  C.POINTER(C.c_int32)]),
+    "mdx_server_info": (C.c_int, [_P, C.c_int64, C.POINTER(JobInfo)]),
+    "mdx_server_records": (C.c_int, [_P, C.c_int64, _P, C.c_int32, C.POINTER(C.c_int32)]),
+    "mdx_server_result": (C.c_int, [_P, C.c_int64, _P, C.c_int64, C.c_int32]),
+    "mdx_server_stats": (C.c_int, [_P, C.POINTER(ServerCounters)]),
     "mdx_create": (C.c_int, [C.POINTER(Params), _P, C.c_int64, C.POINTER(_P)]),
     "mdx_destroy": (C.c_int, [_P]),
     "mdx_set_state": (C.c_int, [_P, _P, C.c_int64]),
@@ -381,6 +435,15 @@ class Batch:
         k = min(res.n_energy, cap)
         return res, es[:k].copy(), et[:k].copy()
 
+    def fire_begin(self, slot: int, fp: FireParams, force_tol: float, max_steps: int) -> None:
+        check(self._lib.mdx_batch_fire_begin(self._h, slot, C.byref(fp), force_tol, max_steps))
+
+    def fire_poll(self, slot: int):
+        done = C.c_int32()
+        res = FireResult()
+        check(self._lib.mdx_batch_fire_poll(self._h, slot, C.byref(done), C.byref(res)))
+        return bool(done.value), res
+
     def set_graph(self, on: bool) -> None:
         check(self._lib.mdx_batch_set_graph(self._h, 1 if on else 0))
 
@@ -391,3 +454,92 @@ class Batch:
         t = Timing()
         check(self._lib.mdx_batch_timing_read(self._h, C.byref(t)))
         return t
+
+
+def server_default_params() -> ServerParams:
+    p = ServerParams()
+    check(load().mdx_server_default_params(C.byref(p)))
+    return p
+
+
+def make_job(params: Params, kind: int, nsteps: int, p_ext: float = 0.0, force_tol: float = 1e-6,
+             fire: FireParams | None = None, t_arrival: float = -1.0) -> Job:
+    j = Job()
+    j.params = params
+    j.kind = kind
+    j.nsteps = nsteps
+    j.p_ext = p_ext
+    j.force_tol = force_tol
+    if fire is not None:
+        j.fire = fire
+    j.t_arrival = t_arrival
+    return j
+
+
+class Server:
+    """Owner of one mdx_server: continuous batching of MD jobs."""
+
+    def __init__(self, params: ServerParams | None = None, **kw):
+        self._lib = load()
+        p = params if params is not None else server_default_params()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self.params = p
+        self._n = {}
+        h = C.c_void_p()
+        check(self._lib.mdx_server_create(C.byref(p), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.mdx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def now(self) -> float:
+        t = C.c_double()
+        check(self._lib.mdx_server_now(self._h, C.byref(t)))
+        return t.value
+
+    def submit(self, job: Job, frame: np.ndarray) -> int:
+        frame = np.ascontiguousarray(frame, dtype=PARTICLE_DTYPE)
+        jid = C.c_int64()
+        check(self._lib.mdx_server_submit(self._h, C.byref(job), _ptr(frame), len(frame), C.byref(jid)))
+        self._n[jid.value] = len(frame)
+        return jid.value
+
+    def tick(self, cap_done: int = 256):
+        nr, nq, nd = C.c_int32(), C.c_int32(), C.c_int32()
+        ids = np.zeros(cap_done, np.int64)
+        check(self._lib.mdx_server_tick(self._h, C.byref(nr), C.byref(nq), _ptr(ids), cap_done, C.byref(nd)))
+        return nr.value, nq.value, [int(x) for x in ids[:min(nd.value, cap_done)]]
+
+    def info(self, jid: int) -> JobInfo:
+        out = JobInfo()
+        check(self._lib.mdx_server_info(self._h, jid, C.byref(out)))
+        return out
+
+    def records(self, jid: int) -> np.ndarray:
+        n = C.c_int32()
+        check(self._lib.mdx_server_records(self._h, jid, None, 0, C.byref(n)))
+        out = (EnergyRecord * max(n.value, 1))()
+        check(self._lib.mdx_server_records(self._h, jid, C.cast(out, C.c_void_p), n.value, C.byref(n)))
+        return np.array([(r.step, r.K, r.U, r.P) for r in out[:n.value]],
+                        dtype=[("step", "<i8"), ("K", "<f8"), ("U", "<f8"), ("P", "<f8")])
+
+    def result(self, jid: int, release: bool = True) -> np.ndarray:
+        out = empty_frame(self._n[jid])
+        check(self._lib.mdx_server_result(self._h, jid, _ptr(out), len(out), 1 if release else 0))
+        if release:
+            self._n.pop(jid, None)
+        return out
+
+    def stats(self) -> ServerCounters:
+        out = ServerCounters()
+        check(self._lib.mdx_server_stats(self._h, C.byref(out)))
+        return out

@@ -55,7 +55,7 @@ def build_mdx() -> Path:
     target = OUT / "libmdx.so"
     tmp = OUT / "libmdx.so.tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", str(CSRC / "mdx_api.hip"), str(CSRC / "mdx_batch.hip"),
-          str(CSRC / "mdx_init.cpp"),
+          str(CSRC / "mdx_init.cpp"), str(CSRC / "mdx_server.cpp"),
           "-o", str(tmp)])
     os.replace(tmp, target)
     return target

@@ -334,4 +334,20 @@ int mdx_batch_timing_read(mdx_batch *b, mdx_timing *out) {
     return timing_read(b->b, out);
 }
 
+int mdx_batch_fire_begin(mdx_batch *b, int32_t slot, const mdx_fire_params *fp, double force_tol,
+                         int32_t max_steps) {
+    CHECK_HANDLE(b);
+    if (!fp) return set_error(MDX_ERR_INVALID, "null argument");
+    return b->b.fire_begin(slot, *fp, force_tol, max_steps);
+}
+
+int mdx_batch_fire_poll(mdx_batch *b, int32_t slot, int32_t *done, mdx_fire_result *result) {
+    CHECK_HANDLE(b);
+    if (!done || !result) return set_error(MDX_ERR_INVALID, "null argument");
+    int d = 0;
+    const int rc = b->b.fire_poll(slot, &d, result);
+    *done = d;
+    return rc;
+}
+
 }  // extern "C"

@@ -106,7 +106,7 @@ Batch::~Batch() {
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
                      d_sp, d_sc, d_blk2sys, d_cblk2sys, d_want, sorted_old, cs_old, movers, mover_off,
-                     n_movers};
+                     n_movers, dirty};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
     for (auto *vec : {&force_ev, &step_ev})
@@ -186,10 +186,14 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     MDX_HIP(hipMalloc(&count, (static_cast<size_t>(C) + 1) * 4));
     MDX_HIP(hipMalloc(&sorted_old, n4));
     MDX_HIP(hipMalloc(&cs_old, (static_cast<size_t>(C) + 1) * 4));
-    MDX_HIP(hipMalloc(&movers, n4));
-    MDX_HIP(hipMalloc(&mover_off, n4));
-    MDX_HIP(hipMalloc(&n_movers, 4));
-    MDX_HIP(hipMemset(n_movers, 0, 4));
+    // per drift workgroup: up to kBlock movers and their count
+    const size_t mcap = static_cast<size_t>(std::max(TB, 1)) * kBlock * 4;
+    MDX_HIP(hipMalloc(&movers, mcap));
+    MDX_HIP(hipMalloc(&mover_off, mcap));
+    MDX_HIP(hipMalloc(&n_movers, static_cast<size_t>(std::max(TB, 1)) * 4));
+    MDX_HIP(hipMemset(n_movers, 0, static_cast<size_t>(std::max(TB, 1)) * 4));
+    MDX_HIP(hipMalloc(&dirty, std::max<size_t>(static_cast<size_t>(C), 1)));
+    MDX_HIP(hipMemset(dirty, 0, std::max<size_t>(static_cast<size_t>(C), 1)));
     MDX_HIP(hipMemset(key, 0, n4));
     MDX_HIP(hipMalloc(&cs, (static_cast<size_t>(C) + 1) * 4));
     MDX_HIP(hipMalloc(&bsum, static_cast<size_t>(ceil_div(C, kScanChunk) + 1) * 4));
@@ -224,6 +228,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     ptrs.movers = movers;
     ptrs.mover_off = mover_off;
     ptrs.n_movers = n_movers;
+    ptrs.dirty = dirty;
     ptrs.sp = d_sp;
     ptrs.sc = d_sc;
     ptrs.blk2sys = d_blk2sys;
@@ -270,39 +275,50 @@ int Batch::launch_step_kernels(bool obs) {
             k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs);
         else
             k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs);
-        k_finalize<<<S, kBlock, 0, stream>>>(ptrs, FIN_DRIFT, d_want);
     }
     if (TCB > 0) k_stay<<<TCB, kBlock, 0, stream>>>(ptrs);
-    if (TB > 0) k_arrive<<<std::min(TB, 1024), kBlock, 0, stream>>>(ptrs);
+    if (TB > 0) k_arrive<<<TB, kBlock, 0, stream>>>(ptrs);
     const int nchunk = ceil_div(C, kScanChunk);
     if (nchunk > 0) {
         k_scan_a<<<nchunk, kBlock, 0, stream>>>(count, C, bsum);
         k_scan_b<<<nchunk, kBlock, 0, stream>>>(count, C, bsum, cs);
     }
     if (TCB > 0) k_fill<<<TCB, kBlock, 0, stream>>>(ptrs);
-    if (TCB > 0) {
+    if (TCB > 0) k_cell_sort<<<TCB, kBlock, 0, stream>>>(ptrs);
+    if (TB > 0) {
         if (prec == MDX_PREC_FP64)
-            k_cell_image<double2><<<TCB, kBlock, 0, stream>>>(ptrs);
+            k_slot_image<double2><<<TB, kBlock, 0, stream>>>(ptrs);
         else
-            k_cell_image<float2><<<TCB, kBlock, 0, stream>>>(ptrs);
+            k_slot_image<float2><<<TB, kBlock, 0, stream>>>(ptrs);
     }
     if (TB > 0) {
         const bool timed = timing;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timed && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
             (void)hipEventRecord(e0, stream);
+        const bool uni = uniform_sigma();
         switch (prec) {
         case MDX_PREC_COMP32:
-            if (obs) k_pair<float2, ACC_COMP32, true><<<TB, kBlock, 0, stream>>>(ptrs);
-            else k_pair<float2, ACC_COMP32, false><<<TB, kBlock, 0, stream>>>(ptrs);
+            if (obs) {
+                if (uni) k_pair<float2, ACC_COMP32, true, true><<<TB, kBlock, 0, stream>>>(ptrs);
+                else k_pair<float2, ACC_COMP32, true, false><<<TB, kBlock, 0, stream>>>(ptrs);
+            } else {
+                if (uni) k_pair<float2, ACC_COMP32, false, true><<<TB, kBlock, 0, stream>>>(ptrs);
+                else k_pair<float2, ACC_COMP32, false, false><<<TB, kBlock, 0, stream>>>(ptrs);
+            }
             break;
         case MDX_PREC_ACC64:
-            if (obs) k_pair<float2, ACC_FP64, true><<<TB, kBlock, 0, stream>>>(ptrs);
-            else k_pair<float2, ACC_FP64, false><<<TB, kBlock, 0, stream>>>(ptrs);
+            if (obs) {
+                if (uni) k_pair<float2, ACC_FP64, true, true><<<TB, kBlock, 0, stream>>>(ptrs);
+                else k_pair<float2, ACC_FP64, true, false><<<TB, kBlock, 0, stream>>>(ptrs);
+            } else {
+                if (uni) k_pair<float2, ACC_FP64, false, true><<<TB, kBlock, 0, stream>>>(ptrs);
+                else k_pair<float2, ACC_FP64, false, false><<<TB, kBlock, 0, stream>>>(ptrs);
+            }
             break;
         default:
-            if (obs) k_pair<double2, ACC_FP64, true><<<TB, kBlock, 0, stream>>>(ptrs);
-            else k_pair<double2, ACC_FP64, false><<<TB, kBlock, 0, stream>>>(ptrs);
+            if (obs) k_pair<double2, ACC_FP64, true, false><<<TB, kBlock, 0, stream>>>(ptrs);
+            else k_pair<double2, ACC_FP64, false, false><<<TB, kBlock, 0, stream>>>(ptrs);
             break;
         }
         if (e1) {
@@ -317,8 +333,18 @@ int Batch::launch_step_kernels(bool obs) {
 
 // count passes of the step sequence; replayed from a captured hipGraph unless
 // kernel timing is on.
+bool Batch::uniform_sigma() const {
+    for (const Slot &s : slots)
+        if (s.loaded && !(s.p.sigma_aa == s.p.sigma_bb && s.p.sigma_aa == s.p.sigma_ab)) return false;
+    return true;
+}
+
 int Batch::run_passes(int count_, bool obs) {
     if (count_ <= 0) return MDX_OK;
+    if (graph_uni != uniform_sigma()) {   // captured graphs hold the other pair-kernel variant
+        invalidate_graphs();
+        graph_uni = uniform_sigma();
+    }
     hipEvent_t s0 = nullptr, s1 = nullptr;
     if (timing && hipEventCreate(&s0) == hipSuccess && hipEventCreate(&s1) == hipSuccess)
         (void)hipEventRecord(s0, stream);
@@ -607,6 +633,7 @@ int Batch::step(int nsteps) {
         case SK_NVT: sp.pmode = PM_NVT; break;
         case SK_NPH: sp.pmode = PM_NPH; obs = true; nph = true; break;
         case SK_PISTON: sp.pmode = PM_PISTON; break;
+        case SK_FIRE: sp.pmode = PM_FIRE; break;
         default: return set_error(MDX_ERR_INVALID, "unknown ensemble");
         }
         if (s.kind == SK_NVT && (s.flags & MDX_STEP_MIDDLE_WRAP)) {
@@ -631,6 +658,12 @@ int Batch::step(int nsteps) {
     for (size_t r = 0; r < slots.size(); ++r) {
         Slot &s = slots[r];
         if (!s.loaded || s.kind == SK_NONE) continue;
+        if (s.kind == SK_FIRE) {
+            s.cells_valid = true;
+            s.cells_py = 1;
+            s.K_valid = s.W_valid = false;
+            continue;
+        }
         s.t += s.p.dt * nsteps;
         s.cells_valid = true;
         s.cells_py = s.kind == SK_PISTON ? 0 : 1;
@@ -798,20 +831,18 @@ int Batch::set_precision(int precision) {
     return MDX_OK;
 }
 
-// FIRE on slot r (EnergyMinimizer::minimize_frame, src/md_energy_minimizer.cu:
-// 122-272).  Standard step j runs the drift of iteration j-1 and the force
-// pass of iteration j, so max_steps iterations take max_steps + 1 steps.
-int Batch::fire(int r, const mdx_fire_params &fp, double ftol, int max_steps, mdx_fire_result *res,
-                int32_t *es, double *et, int cap) {
+// Slot r starts FIRE (EnergyMinimizer::minimize_frame, src/md_energy_minimizer.cu:
+// 122-160): zero velocities, dt = dt_init, alpha = alpha0, and the device
+// schedule counters; its kind becomes SK_FIRE.
+int Batch::fire_begin(int r, const mdx_fire_params &fp, double ftol, int max_steps) {
     if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
         return set_error(MDX_ERR_STATE, "slot not loaded");
     if (max_steps <= 0) return set_error(MDX_ERR_INVALID, "EnergyMinimizer: max_steps must be positive.");
+    if (prec == MDX_PREC_COMP32)
+        return set_error(MDX_ERR_INVALID, "FIRE slots need FP64 accelerations: use an ACC64 or FP64 batch");
     MDX_HIP(hipSetDevice(device));
     Slot &s = slots[r];
     s.piston_active = false;
-    // Overlapping starts produce accelerations beyond FP32 range: FIRE keeps
-    // FP64 accelerations (FP32 pairs with FP64 accumulation, or FP64 pairs).
-    if (prec == MDX_PREC_COMP32) MDX_TRY(set_precision(MDX_PREC_ACC64));
     std::fill(h_want.begin(), h_want.end(), 0);
     h_want[r] = 1;
     MDX_HIP(hipMemcpyAsync(d_want, h_want.data(), h_want.size() * 4, hipMemcpyHostToDevice, stream));
@@ -834,6 +865,98 @@ int Batch::fire(int r, const mdx_fire_params &fp, double ftol, int max_steps, md
     sp.N_min = fp.N_min;
     sp.max_step = fp.max_step;
     sp.f_max_steps = max_steps;
+    s.kind = SK_FIRE;
+    s.K_valid = s.W_valid = false;
+    return upload_sp(r);
+}
+
+int Batch::fire_poll(int r, int *done, mdx_fire_result *res) {
+    if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
+        return set_error(MDX_ERR_STATE, "slot not loaded");
+    MDX_HIP(hipSetDevice(device));
+    DevScalars h;
+    MDX_TRY(read_scalars(r, h));
+    const int max_steps = h_sp[r].f_max_steps;
+    const bool converged = h.f_done != 0;
+    // unconverged: max_steps force passes and the drift after the last one
+    const bool finished = converged || (h.f_step >= max_steps && h.f_drifted >= h.f_step);
+    *done = finished ? 1 : 0;
+    std::memset(res, 0, sizeof(*res));
+    res->steps = converged ? h.f_conv_step : (finished ? max_steps : h.f_step);
+    res->converged = converged ? 1 : 0;
+    res->max_force = h.f_maxF;
+    if (finished) {
+        slots[r].kind = SK_NONE;
+        set_idle(h_sp[r], slots[r]);
+        MDX_TRY(upload_sp(r));
+        MDX_TRY(obs_pass(r, 1));   // :270
+        MDX_TRY(read_scalars(r, h));
+        res->potential_energy = h.U;
+    }
+    return MDX_OK;
+}
+
+int Batch::observables_many(const std::vector<int> &which, uint32_t mask, mdx_obs *out) {
+    if (which.empty()) return MDX_OK;
+    for (int r : which)
+        if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded || slots[r].piston_active)
+            return set_error(MDX_ERR_STATE, "observables_many: slot not loaded or in piston mode");
+    MDX_HIP(hipSetDevice(device));
+    if (mask & (MDX_OBS_K | MDX_OBS_P)) {
+        std::vector<int> need;
+        for (int r : which)
+            if (!slots[r].K_valid) need.push_back(r);
+        MDX_TRY(kinetic(need));
+    }
+    if (mask & (MDX_OBS_U | MDX_OBS_P | MDX_OBS_VIRIAL)) MDX_TRY(one_pass(*this, which, 1, PM_OBS));
+    std::vector<DevScalars> h(slots.size());
+    MDX_HIP(hipMemcpyAsync(h.data(), d_sc, h.size() * sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+    MDX_HIP(hipStreamSynchronize(stream));
+    for (size_t q = 0; q < which.size(); ++q) {
+        const int r = which[q];
+        DevScalars &d = h[r];
+        mdx_obs &o = out[q];
+        std::memset(&o, 0, sizeof(o));
+        const double area = d.g.Lx * d.g.Ly;
+        o.K = d.K;
+        if (mask & MDX_OBS_U) o.U = d.U;
+        if (mask & MDX_OBS_VIRIAL) o.W = d.W;
+        if (mask & MDX_OBS_P) {   // src/md_env.cu:2205-2252
+            d.last_P = (area > 0.0) ? (2.0 * d.K + d.W) / (2.0 * area) : 0.0;
+            o.P = d.last_P;
+            MDX_HIP(hipMemcpyAsync(&d_sc[r].last_P, &d.last_P, sizeof(double), hipMemcpyHostToDevice, stream));
+        }
+        o.xi = d.xi;
+        o.p_A = d.p_A;
+        o.area_rate = d.p_A / std::max(slots[r].p.barostat_mass, 1e-12);
+        o.v_p = d.v_p;
+        o.F_p = d.F_p;
+        o.last_P = d.last_P;
+        o.Lx = d.g.Lx;
+        o.Ly = d.g.Ly;
+        o.t = slots[r].t;
+    }
+    MDX_HIP(hipStreamSynchronize(stream));
+    return MDX_OK;
+}
+
+// FIRE on slot r (EnergyMinimizer::minimize_frame, src/md_energy_minimizer.cu:
+// 122-272).  Standard step j runs the drift of iteration j-1 and the force
+// pass of iteration j, so max_steps iterations take max_steps + 1 steps.
+int Batch::fire(int r, const mdx_fire_params &fp, double ftol, int max_steps, mdx_fire_result *res,
+                int32_t *es, double *et, int cap) {
+    if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
+        return set_error(MDX_ERR_STATE, "slot not loaded");
+    if (max_steps <= 0) return set_error(MDX_ERR_INVALID, "EnergyMinimizer: max_steps must be positive.");
+    MDX_HIP(hipSetDevice(device));
+    Slot &s = slots[r];
+    // Overlapping starts produce accelerations beyond FP32 range: FIRE keeps
+    // FP64 accelerations (FP32 pairs with FP64 accumulation, or FP64 pairs).
+    if (prec == MDX_PREC_COMP32) MDX_TRY(set_precision(MDX_PREC_ACC64));
+    MDX_TRY(fire_begin(r, fp, ftol, max_steps));
+    s.kind = SK_NONE;   // this loop steps the slot itself, the others idle
+    DevScalars h;
+    SysParams &sp = h_sp[r];
     int ns = 0, last_pushed = -1;
     auto push = [&](int step, double U) {
         if (ns < cap) {

@@ -58,6 +58,7 @@ struct Batch {
              *bsum = nullptr;
     uint32_t *sorted_old = nullptr, *cs_old = nullptr, *movers = nullptr, *mover_off = nullptr,
              *n_movers = nullptr;
+    uint8_t *dirty = nullptr;
     double *partials = nullptr;
     SysParams *d_sp = nullptr;
     DevScalars *d_sc = nullptr;
@@ -66,6 +67,7 @@ struct Batch {
     // captured step graphs, by [obs]
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     bool use_graph = true;
+    bool graph_uni = true;   // pair-kernel variant the captured graphs hold
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> force_ev, step_ev;
@@ -93,6 +95,14 @@ struct Batch {
                   uint32_t *sorted_ids, int64_t cap_n, int64_t cap_cells);
     int fire(int r, const mdx_fire_params &fp, double ftol, int max_steps, mdx_fire_result *res,
              int32_t *es, double *et, int cap);
+    // FIRE as a per-slot kind of ordinary stepping: after fire_begin the slot
+    // takes one FIRE iteration per step (the device-side schedule) until it
+    // converges or reaches max_steps, alongside whatever the other slots do.
+    int fire_begin(int r, const mdx_fire_params &fp, double ftol, int max_steps);
+    // done = converged or max_steps iterations taken; fills res (PE only when done)
+    int fire_poll(int r, int *done, mdx_fire_result *res);
+    // K/U/P/virial of several slots from one kinetic and one energy pass
+    int observables_many(const std::vector<int> &which, uint32_t mask, mdx_obs *out);
     int set_precision(int precision);
     int read_scalars(int r, DevScalars &h);
     int sync();
@@ -108,6 +118,7 @@ struct Batch {
     int walls(const std::vector<int> &which);
     int obs_pass(int r, int periodic_y);
     void invalidate_graphs();
+    bool uniform_sigma() const;   // every loaded slot has one sigma for all type pairs
     int write_scalar_block(int r, const DevScalars &h);
 };
 

@@ -149,15 +149,38 @@ __device__ void slot_reduce(unsigned maxmask, const double *partials, int stride
     __shared__ double sh[NV][kWaves];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
+    // thread t combines partials t, t + kBlock, t + 2 kBlock, ... in that
+    // order; eight loads per value are in flight before the first add
+    constexpr int kU = 8;
+    double a[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) a[k] = ((maxmask >> k) & 1u) ? -1.0e308 : 0.0;
+    int i = threadIdx.x;
+    for (; i + (kU - 1) * kBlock < nb; i += kU * kBlock) {
+        double q[NV][kU];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int u = 0; u < kU; ++u) q[k][u] = partials[k * stride + base + i + u * kBlock];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const bool mx = (maxmask >> k) & 1u;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) a[k] = mx ? fmax(a[k], q[k][u]) : a[k] + q[k][u];
+        }
+    }
+    for (; i < nb; i += kBlock) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const bool mx = (maxmask >> k) & 1u;
+            const double q = partials[k * stride + base + i];
+            a[k] = mx ? fmax(a[k], q) : a[k] + q;
+        }
+    }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         const bool mx = (maxmask >> k) & 1u;
-        double a = mx ? -1.0e308 : 0.0;
-        for (int i = threadIdx.x; i < nb; i += kBlock) {
-            const double q = partials[k * stride + base + i];
-            a = mx ? fmax(a, q) : a + q;
-        }
-        const double r = mx ? wave_max(a) : wave_sum(a);
+        const double r = mx ? wave_max(a[k]) : wave_sum(a[k]);
         if (lane == 0) sh[k][wid] = r;
     }
     __syncthreads();

@@ -14,13 +14,14 @@
 //   icls, scell             class and slot-local cell of each sorted slot
 //
 // One step = k_box (one thread per slot: box, barostat, piston, grid, FIRE
-// bookkeeping) -> k_drift (kick/drift/wrap, cell key, count, the ensemble's
-// reduction, movers) -> k_stay -> k_arrive -> k_scan_a/k_scan_b -> k_fill
-// -> k_cell_image (per-cell id sort + image) -> k_pair (forces, second kick,
-// the ensemble's reduction) -> k_finalize.  The cell list is updated from the
-// previous one: only particles whose cell changed ("movers") use atomics.
-// Global FP64 scalars are finalised by the last workgroup of a slot's
-// reduction.
+// bookkeeping) -> k_drift (kick/drift/wrap, cell key, the ensemble's
+// reduction partials, movers) -> k_finalize -> k_stay -> k_arrive ->
+// k_scan_a/k_scan_b -> k_fill -> k_cell_sort (id order of the cells that
+// received movers) -> k_slot_image (per-slot image, coalesced) -> k_pair
+// (forces, second kick, the ensemble's reduction partials) -> k_finalize.
+// The cell list is updated from the previous one: only particles whose cell
+// changed ("movers") use atomics.  Global FP64 scalars are combined from the
+// workgroup partials by one workgroup per slot (k_finalize), in a fixed order.
 #pragma once
 #include <math.h>
 
@@ -140,18 +141,27 @@ __device__ inline double nh_half_update(double xi, double K, const SysParams &sp
 
 // Appends the calling lanes with mover == true to the batch's mover list: one
 // atomic per wave that has movers.
-__device__ inline void append_movers(const BatchPtrs &P, bool mover, uint32_t pid) {
+__device__ inline void append_movers(const BatchPtrs &P, bool mover, uint32_t pid, int gb) {
+    // Workgroup-local list: movers[gb * kBlock + j], j < n_movers[gb], in
+    // lane order.  No global atomics (a single global counter serialises
+    // across the XCDs' L2s).
+    __shared__ uint32_t wcount[kWaves];
     const unsigned long long mask = __ballot(mover);
-    if (mask == 0ull) return;
     const int lane = threadIdx.x & 63;
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(P.n_movers, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, leader, 64);
+    const int wid = threadIdx.x >> 6;
+    if (lane == 0) wcount[wid] = static_cast<uint32_t>(__popcll(mask));
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        base += w < wid ? wcount[w] : 0u;
+        total += wcount[w];
+    }
     if (mover) {
         const unsigned long long below = mask & ((1ull << lane) - 1ull);
-        P.movers[base + static_cast<uint32_t>(__popcll(below))] = pid;
+        P.movers[static_cast<uint32_t>(gb) * kBlock + base + static_cast<uint32_t>(__popcll(below))] = pid;
     }
+    if (threadIdx.x == 0) P.n_movers[gb] = total;
 }
 
 // --------------------------------------------------------------------------
@@ -262,7 +272,7 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P) {
         P.key[i] = c;
         pid = static_cast<uint32_t>(i);
     }
-    append_movers(P, mover, pid);
+    append_movers(P, mover, pid, gb);
     if (reduce) {
         double vals[1] = {red};
         block_partials<1>(vals, 0u, P.partials, P.TB, gb);
@@ -328,6 +338,19 @@ __global__ void __launch_bounds__(kBlock) k_scan_b(const uint32_t *__restrict__ 
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) cs[ncell] = run;
 }
 
+// Drift reduction of a slot (Nose-Hoover / NPH kinetic energy after the
+// first half kick), combined from the drift workgroups' partials by one whole
+// workgroup: K_half, and for Nose-Hoover the mid-step xi.
+__device__ inline void finalize_drift(const BatchPtrs &P, const SysParams &sp, DevScalars &sc) {
+    if (sp.n <= 0 || sp.nblk <= 0 || (sp.kind != SK_NVT && sp.kind != SK_NPH)) return;
+    double out[1];
+    slot_reduce<1>(0u, P.partials, P.TB, sp.boff, sp.nblk, out);
+    if (threadIdx.x == 0) {
+        sc.K_half = out[0];
+        if (sp.kind == SK_NVT) sc.xi_mid = nh_half_update(nh_half_update(sc.xi, sc.K, sp), out[0], sp);
+    }
+}
+
 // One thread per cell of the new grid (and of the previous grid when it was
 // larger): count[c] = particles of the previous list of cell c that stay in c.
 // Cells beyond the new grid get zero.
@@ -335,8 +358,9 @@ __global__ void __launch_bounds__(kBlock) k_stay(BatchPtrs P) {
     const int gb = xcd_block(blockIdx.x, gridDim.x);
     const int r = P.cblk2sys[gb];
     const SysParams &sp = P.sp[r];
-    const DevScalars &sc = P.sc[r];
+    DevScalars &sc = P.sc[r];
     const int lc = (gb - sp.cboff) * kBlock + threadIdx.x;
+    if (gb == sp.cboff) finalize_drift(P, sp, sc);   // the slot's first cell workgroup
     const int n_new = sc.g.ncx * sc.g.ncy;
     const int n_old = sc.list_valid ? sc.g_old.ncx * sc.g_old.ncy : 0;
     // without a previous list (new or reused slot) every cell of the slot is
@@ -352,11 +376,17 @@ __global__ void __launch_bounds__(kBlock) k_stay(BatchPtrs P) {
     P.count[c] = stay;
 }
 
-// Movers take an offset inside their new cell, after the stayers.
+// Movers take an offset inside their new cell, after the stayers.  One
+// workgroup per drift workgroup's mover list.
 __global__ void __launch_bounds__(kBlock) k_arrive(BatchPtrs P) {
-    const uint32_t nm = *P.n_movers;
-    for (uint32_t m = blockIdx.x * kBlock + threadIdx.x; m < nm; m += gridDim.x * kBlock)
-        P.mover_off[m] = atomicAdd(&P.count[P.key[P.movers[m]]], 1u);
+    const uint32_t b = blockIdx.x;
+    const uint32_t nm = P.n_movers[b];
+    if (threadIdx.x < nm) {
+        const uint32_t m = b * kBlock + threadIdx.x;
+        const uint32_t c = P.key[P.movers[m]];
+        P.mover_off[m] = atomicAdd(&P.count[c], 1u);
+        P.dirty[c] = 1;   // the cell's order needs restoring
+    }
 }
 
 // Stayers keep their previous order (ascending id); movers go to their
@@ -379,29 +409,34 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchPtrs P) {
             if (P.key[id] == c) P.sorted[dst++] = id;
         }
     }
-    const uint32_t nm = *P.n_movers;
-    for (uint32_t m = blockIdx.x * kBlock + threadIdx.x; m < nm; m += gridDim.x * kBlock) {
-        const uint32_t id = P.movers[m];
-        P.sorted[P.cs[P.key[id]] + P.mover_off[m]] = id;
+    for (int b = blockIdx.x; b < P.TB; b += gridDim.x) {   // every drift workgroup's movers
+        const uint32_t nm = P.n_movers[b];
+        if (threadIdx.x < nm) {
+            const uint32_t m = static_cast<uint32_t>(b) * kBlock + threadIdx.x;
+            const uint32_t id = P.movers[m];
+            P.sorted[P.cs[P.key[id]] + P.mover_off[m]] = id;
+        }
     }
 }
 
-// One thread per cell: restore ascending-id order inside the cell (the
-// scatter's atomics leave it arbitrary), then write the cell-ordered image.
-template <typename IT>
-__global__ void __launch_bounds__(kBlock) k_cell_image(BatchPtrs P) {
+// One thread per cell: cells that received movers restore ascending-id order
+// (stayers are in order, the scatter's atomics leave the arrivals in any
+// order); every cell records its range for the next step's update.
+__global__ void __launch_bounds__(kBlock) k_cell_sort(BatchPtrs P) {
     const int gb = xcd_block(blockIdx.x, gridDim.x);
     const int r = P.cblk2sys[gb];
     const SysParams &sp = P.sp[r];
     const Grid g = P.sc[r].g;
     const int lc = (gb - sp.cboff) * kBlock + threadIdx.x;
-    if (gb == 0 && threadIdx.x == 0) *P.n_movers = 0u;   // every reader of this step's list has run
-    if (sp.n <= 0 || lc >= g.ncx * g.ncy) return;
+    const int ncell = g.ncx * g.ncy;
+    if (sp.n <= 0 || lc >= ncell) return;
     if (lc == 0) P.sc[r].list_valid = 1;
     const int c = sp.coff + lc;
     const uint32_t s0 = P.cs[c], s1 = P.cs[c + 1];
     P.cs_old[c] = s0;
-    if (lc == g.ncx * g.ncy - 1) P.cs_old[c + 1] = s1;
+    if (lc == ncell - 1) P.cs_old[c + 1] = s1;
+    if (!P.dirty[c]) return;
+    P.dirty[c] = 0;
     uint32_t *sorted = P.sorted;
     for (uint32_t k = s0 + 1; k < s1; ++k) {
         const uint32_t id = sorted[k];
@@ -414,26 +449,40 @@ __global__ void __launch_bounds__(kBlock) k_cell_image(BatchPtrs P) {
         }
         sorted[j] = id;
     }
-    const int cx = lc % g.ncx, cy = lc / g.ncx;
-    const double ox = (static_cast<double>(cx) + 0.5) * g.w;
-    const double oy = (static_cast<double>(cy) + 0.5) * g.h;
-    IT *img = static_cast<IT *>(P.img);
-    for (uint32_t k = s0; k < s1; ++k) {
-        const uint32_t id = sorted[k];
-        double2 p = P.x[id];
-        if (sp.wrap_keys) {
-            p.x = pbc_wrap(p.x, g.Lx);
-            if (g.periodic_y) p.y = pbc_wrap(p.y, g.Ly);
-        }
-        if constexpr (std::is_same_v<IT, float2>) {
-            img[k] = make_float2(static_cast<float>(p.x - ox), static_cast<float>(p.y - oy));
-        } else {
-            img[k] = p;
-        }
-        P.icls[k] = P.cls[id];
-        P.scell[k] = static_cast<uint32_t>(lc);
-        P.sorted_old[k] = id;
+}
+
+// One thread per cell-ordered slot: the particle's image (FP32 position
+// relative to its cell centre, or the wrapped FP64 position), class, cell,
+// and its id for the next step's update.  Writes are coalesced.
+template <typename IT>
+__global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
+    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int r = P.blk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
+    if (li >= sp.n) return;
+    const Grid g = P.sc[r].g;
+    const uint32_t k = P.cs[sp.coff] + static_cast<uint32_t>(li);
+    const uint32_t id = P.sorted[k];
+    const uint32_t lc = P.key[id] - static_cast<uint32_t>(sp.coff);
+    double2 p = P.x[id];
+    if (sp.wrap_keys) {
+        p.x = pbc_wrap(p.x, g.Lx);
+        if (g.periodic_y) p.y = pbc_wrap(p.y, g.Ly);
     }
+    IT *img = static_cast<IT *>(P.img);
+    if constexpr (std::is_same_v<IT, float2>) {
+        const uint32_t cy = lc / static_cast<uint32_t>(g.ncx);
+        const uint32_t cx = lc - cy * static_cast<uint32_t>(g.ncx);
+        const double ox = (static_cast<double>(cx) + 0.5) * g.w;
+        const double oy = (static_cast<double>(cy) + 0.5) * g.h;
+        img[k] = make_float2(static_cast<float>(p.x - ox), static_cast<float>(p.y - oy));
+    } else {
+        img[k] = p;
+    }
+    P.icls[k] = P.cls[id];
+    P.scell[k] = lc;
+    P.sorted_old[k] = id;
 }
 
 // --------------------------------------------------------------------------
@@ -568,15 +617,111 @@ __device__ inline void finalize_pair(int pmode, const double (&out)[3], const Sy
     }
 }
 
+// Per-particle constants of the FP32 pair path: "s" entries for partners of
+// the particle's own class, "o" for the AB combination.
+struct PairF32 {
+    float rc2s, rc2o, sig2s, sig2o, e24s, e24o, e4s, e4o, r2_guard;
+};
+
+// FP32 pairs of particle (pix, piy) -- its cell-relative image already moved
+// by minus this neighbour cell's offset -- with the candidates [k0, k1) of one
+// neighbour cell, in ascending slot order.  The cell's pair forces are summed
+// in FP32 and the cell total enters the particle's accumulator once: a
+// compensated (two-sum) add in the COMP32 form, an FP64 add in the ACC64 form
+// (which adds every pair in FP64, as the reference does).  Self is excluded by
+// r^2 > 0.  UNI: all type pairs share sigma (rc and sigma^2 need no select);
+// the arithmetic is identical to the general form.
+template <int ACC, bool OBS, bool UNI>
+__device__ __forceinline__ void pair_cell_f32(const char *__restrict__ imgb, const uint8_t *__restrict__ clsb,
+                                              uint32_t k0, uint32_t k1, float pix, float piy, int ci,
+                                              const Grid &g, const PairTable &pt, const PairF32 &pc,
+                                              ForceSum<ACC> &F, ObsSum &ob, bool want_obs) {
+    float fxc = 0.0f, fyc = 0.0f;
+    for (uint32_t k = k0; k < k1; ++k) {
+        const float2 pj = *reinterpret_cast<const float2 *>(imgb + (k << 3));
+        const int cj = clsb[k];
+        float dx = pix - pj.x;
+        float dy = piy - pj.y;
+        if (g.small_x) dx -= g.Lxf * rintf(dx / g.Lxf);
+        if (g.periodic_y && g.small_y) dy -= g.Lyf * rintf(dy / g.Lyf);
+        const float rr = __builtin_fmaf(dx, dx, dy * dy);
+        const bool same = cj == ci;
+        const float rc2 = UNI ? pc.rc2s : (same ? pc.rc2s : pc.rc2o);
+        if (rr < rc2 && rr > 0.0f) {
+            if (rr < pc.r2_guard) {
+                double fx, fy;
+                pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), pair_index(ci, cj), pt, fx, fy,
+                               ob, want_obs);
+                F.addd(fx, fy);
+            } else {
+                const float sig2 = UNI ? pc.sig2s : (same ? pc.sig2s : pc.sig2o);
+                const float e24 = same ? pc.e24s : pc.e24o;
+                const float r2i = fast_rcp(rr);
+                const float sr2 = sig2 * r2i;
+                const float sr6 = sr2 * sr2 * sr2;
+                const float sr12 = sr6 * sr6;
+                const float tmp = e24 * __builtin_fmaf(2.0f, sr12, -sr6) * r2i;
+                const float fx = tmp * dx, fy = tmp * dy;
+                if constexpr (ACC == ACC_COMP32) {
+                    fxc += fx;
+                    fyc += fy;
+                } else {
+                    F.add(fx, fy);
+                }
+                if constexpr (OBS) {
+                    if (want_obs) {
+                        const float e4 = same ? pc.e4s : pc.e4o;
+                        const float u = e4 * (sr12 - sr6);
+                        const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
+                        ob.u += 0.5 * (static_cast<double>(u) - pt.vc[pair_index(ci, cj)]);
+                        ob.w += 0.5 * (ddx * dfx + ddy * dfy);
+                        ob.lam += 0.5 * (dfx * ddx - dfy * ddy);
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (ACC == ACC_COMP32) F.add(fxc, fyc);
+}
+
+// FP64 pairs of particle pi (wrapped FP64 position) with the candidates
+// [k0, k1) of one neighbour cell shifted by (shx, shy) = +-L across the
+// periodic boundary; the oracle's operation order, so FP64 forces are
+// bit-identical to the oracle's.
+template <bool OBS>
+__device__ __forceinline__ void pair_cell_f64(const double2 *__restrict__ img, const uint8_t *__restrict__ cls,
+                                              uint32_t k0, uint32_t k1, uint32_t self, const double2 &pi, int ci,
+                                              double shx, double shy, const Grid &g, const PairTable &pt,
+                                              ForceSum<ACC_FP64> &F, ObsSum &ob, bool want_obs) {
+    for (uint32_t k = k0; k < k1; ++k) {
+        if (k == self) continue;
+        const double2 pj = img[k];
+        const int cj = cls[k];
+        double dx = (pi.x - pj.x) - shx;
+        double dy = (pi.y - pj.y) - shy;
+        if (g.small_x) {
+            const double d = pi.x - pj.x;
+            dx = d - g.Lx * round(d / g.Lx);
+        }
+        if (g.periodic_y && g.small_y) {
+            const double d = pi.y - pj.y;
+            dy = d - g.Ly * round(d / g.Ly);
+        }
+        double fx, fy;
+        if (pair_fp64<OBS>(dx, dy, pair_index(ci, cj), pt, fx, fy, ob, want_obs)) F.addd(fx, fy);
+    }
+}
+
 // Forces of every particle of every active slot (thread per cell-ordered
-// slot, 3x3 neighbour cells in the oracle's order), then the slot's mode:
-// second kick, FIRE kick, walls, and its reduction.  OBS = true compiles the
-// energy/virial/lambda sums (used when some slot needs them).
-template <typename IT, int ACC, bool OBS>
+// slot, 3x3 neighbour cells in the oracle's order: rows -1, 0, +1, cells -1,
+// 0, +1, ascending id), then the slot's mode: second kick, FIRE kick, walls,
+// and its reduction.  OBS = true compiles the energy/virial/lambda sums (used
+// when some slot needs them); UNI = true when every slot of the batch has one
+// sigma for all type pairs.
+template <typename IT, int ACC, bool OBS, bool UNI>
 __global__ void __launch_bounds__(kBlock) k_pair(BatchPtrs P) {
-    using T = std::conditional_t<std::is_same_v<IT, float2>, float, double>;
     using AT = std::conditional_t<ACC == ACC_COMP32, float2, double2>;
-    constexpr bool kFp64 = std::is_same_v<T, double>;
+    constexpr bool kFp64 = std::is_same_v<IT, double2>;
     const int gb = xcd_block(blockIdx.x, gridDim.x);
     const int r = P.blk2sys[gb];
     const SysParams &sp = P.sp[r];
@@ -591,108 +736,59 @@ __global__ void __launch_bounds__(kBlock) k_pair(BatchPtrs P) {
     double r0 = 0.0, r1 = 0.0, r2v = 0.0;
     if (li < sp.n) {
         const uint32_t s = P.cs[sp.coff] + static_cast<uint32_t>(li);
-        const IT *img = static_cast<const IT *>(P.img);
         const uint32_t lc = P.scell[s];
         const int cx = static_cast<int>(lc % static_cast<uint32_t>(g.ncx));
         const int cy = static_cast<int>(lc / static_cast<uint32_t>(g.ncx));
-        const IT pi = img[s];
         const int ci = P.icls[s];
         ForceSum<kFp64 ? ACC_FP64 : ACC> F;
         ObsSum ob;
-        float rc2s = 0.f, rc2o = 0.f, sig2s = 0.f, sig2o = 0.f, e24s = 0.f, e24o = 0.f, e4s = 0.f, e4o = 0.f;
+        const IT pi = static_cast<const IT *>(P.img)[s];
+        PairF32 pc{};
         if constexpr (!kFp64) {
-            rc2s = pt.rc2f[ci];
-            rc2o = pt.rc2f[2];
-            sig2s = pt.sig2f[ci];
-            sig2o = pt.sig2f[2];
-            e24s = pt.eps24f[ci];
-            e24o = pt.eps24f[2];
-            e4s = pt.eps4f[ci];
-            e4o = pt.eps4f[2];
+            pc.rc2s = pt.rc2f[ci];
+            pc.rc2o = pt.rc2f[2];
+            pc.sig2s = pt.sig2f[ci];
+            pc.sig2o = pt.sig2f[2];
+            pc.e24s = pt.eps24f[ci];
+            pc.e24o = pt.eps24f[2];
+            pc.e4s = pt.eps4f[ci];
+            pc.e4o = pt.eps4f[2];
+            pc.r2_guard = pt.r2_guard;
         }
         for (int oy = -1; oy <= 1; ++oy) {
             int yy = cy + oy;
-            T sy = 0;
+            double sy = 0.0;
             if (g.periodic_y) {
                 if (yy < 0) {
                     yy += g.ncy;
-                    sy = -static_cast<T>(g.Ly);
+                    sy = -g.Ly;
                 } else if (yy >= g.ncy) {
                     yy -= g.ncy;
-                    sy = static_cast<T>(g.Ly);
+                    sy = g.Ly;
                 }
             } else if (yy < 0 || yy >= g.ncy) {
                 continue;
             }
-            const T shy = kFp64 ? sy : static_cast<T>(oy) * static_cast<T>(g.hf);
+            // the row's three cells (periodic in x), their ranges loaded together
             const int rowbase = sp.coff + yy * g.ncx;
-            for (int ox = -1; ox <= 1; ++ox) {
-                int xx = cx + ox;
-                T sx = 0;
-                if (xx < 0) {
-                    xx += g.ncx;
-                    sx = -static_cast<T>(g.Lx);
-                } else if (xx >= g.ncx) {
-                    xx -= g.ncx;
-                    sx = static_cast<T>(g.Lx);
-                }
-                const T shx = kFp64 ? sx : static_cast<T>(ox) * static_cast<T>(g.wf);
-                const uint32_t k0 = P.cs[rowbase + xx], k1 = P.cs[rowbase + xx + 1];
-                for (uint32_t k = k0; k < k1; ++k) {
-                    if (k == s) continue;
-                    const IT pj = img[k];
-                    const int cj = P.icls[k];
-                    if constexpr (kFp64) {
-                        double dx = (pi.x - pj.x) - shx;
-                        double dy = (pi.y - pj.y) - shy;
-                        if (g.small_x) {
-                            const double d = pi.x - pj.x;
-                            dx = d - g.Lx * round(d / g.Lx);
-                        }
-                        if (g.periodic_y && g.small_y) {
-                            const double d = pi.y - pj.y;
-                            dy = d - g.Ly * round(d / g.Ly);
-                        }
-                        double fx, fy;
-                        if (pair_fp64<OBS>(dx, dy, pair_index(ci, cj), pt, fx, fy, ob, want_obs)) F.addd(fx, fy);
-                    } else {
-                        float dx = (pi.x - pj.x) - shx;
-                        float dy = (pi.y - pj.y) - shy;
-                        if (g.small_x) dx -= g.Lxf * rintf(dx / g.Lxf);
-                        if (g.periodic_y && g.small_y) dy -= g.Lyf * rintf(dy / g.Lyf);
-                        const float rr = __builtin_fmaf(dx, dx, dy * dy);
-                        const bool same = cj == ci;
-                        const float rc2 = same ? rc2s : rc2o;
-                        if (rr < rc2 && rr > 0.0f) {
-                            if (rr < pt.r2_guard) {
-                                double fx, fy;
-                                pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), pair_index(ci, cj),
-                                               pt, fx, fy, ob, want_obs);
-                                F.addd(fx, fy);
-                            } else {
-                                const float sig2 = same ? sig2s : sig2o;
-                                const float e24 = same ? e24s : e24o;
-                                const float r2i = fast_rcp(rr);
-                                const float sr2 = sig2 * r2i;
-                                const float sr6 = sr2 * sr2 * sr2;
-                                const float sr12 = sr6 * sr6;
-                                const float tmp = e24 * __builtin_fmaf(2.0f, sr12, -sr6) * r2i;
-                                const float fx = tmp * dx, fy = tmp * dy;
-                                F.add(fx, fy);
-                                if constexpr (OBS) {
-                                    if (want_obs) {
-                                        const float e4 = same ? e4s : e4o;
-                                        const float u = e4 * (sr12 - sr6);
-                                        const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
-                                        ob.u += 0.5 * (static_cast<double>(u) - pt.vc[pair_index(ci, cj)]);
-                                        ob.w += 0.5 * (ddx * dfx + ddy * dfy);
-                                        ob.lam += 0.5 * (dfx * ddx - dfy * ddy);
-                                    }
-                                }
-                            }
-                        }
-                    }
-                }
+            const int xl = cx == 0 ? g.ncx - 1 : cx - 1;
+            const int xr = cx == g.ncx - 1 ? 0 : cx + 1;
+            const uint32_t a0 = P.cs[rowbase + xl], a1 = P.cs[rowbase + xl + 1];
+            const uint32_t b0 = P.cs[rowbase + cx], b1 = P.cs[rowbase + cx + 1];
+            const uint32_t c0 = P.cs[rowbase + xr], c1 = P.cs[rowbase + xr + 1];
+            if constexpr (kFp64) {
+                const double2 *img = static_cast<const double2 *>(P.img);
+                const double sxl = cx == 0 ? -g.Lx : 0.0;
+                const double sxr = cx == g.ncx - 1 ? g.Lx : 0.0;
+                pair_cell_f64<OBS>(img, P.icls, a0, a1, s, pi, ci, sxl, sy, g, pt, F, ob, want_obs);
+                pair_cell_f64<OBS>(img, P.icls, b0, b1, s, pi, ci, 0.0, sy, g, pt, F, ob, want_obs);
+                pair_cell_f64<OBS>(img, P.icls, c0, c1, s, pi, ci, sxr, sy, g, pt, F, ob, want_obs);
+            } else {
+                const char *imgb = static_cast<const char *>(P.img);
+                const float piy = pi.y - static_cast<float>(oy) * g.hf;
+                pair_cell_f32<ACC, OBS, UNI>(imgb, P.icls, a0, a1, pi.x + g.wf, piy, ci, g, pt, pc, F, ob, want_obs);
+                pair_cell_f32<ACC, OBS, UNI>(imgb, P.icls, b0, b1, pi.x, piy, ci, g, pt, pc, F, ob, want_obs);
+                pair_cell_f32<ACC, OBS, UNI>(imgb, P.icls, c0, c1, pi.x - g.wf, piy, ci, g, pt, pc, F, ob, want_obs);
             }
         }
         const uint32_t id = P.sorted[s];
@@ -780,7 +876,7 @@ __global__ void __launch_bounds__(kBlock) k_pair(BatchPtrs P) {
 // per-slot finalisation of the reductions
 // --------------------------------------------------------------------------
 
-enum FinStage : int32_t { FIN_DRIFT = 0, FIN_PAIR = 1, FIN_KINETIC = 2, FIN_WALLS = 3 };
+enum FinStage : int32_t { FIN_PAIR = 1, FIN_KINETIC = 2, FIN_WALLS = 3 };
 
 // One workgroup per slot: combines the slot's workgroup partials written by
 // the preceding kernel and updates the slot's scalars.
@@ -789,15 +885,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(BatchPtrs P, int32_t stage,
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
     if (sp.n <= 0 || sp.nblk <= 0) return;
-    if (stage == FIN_DRIFT) {
-        if (sp.kind != SK_NVT && sp.kind != SK_NPH) return;
-        double out[1];
-        slot_reduce<1>(0u, P.partials, P.TB, sp.boff, sp.nblk, out);
-        if (threadIdx.x == 0) {
-            sc.K_half = out[0];
-            if (sp.kind == SK_NVT) sc.xi_mid = nh_half_update(nh_half_update(sc.xi, sc.K, sp), out[0], sp);
-        }
-    } else if (stage == FIN_PAIR) {
+    if (stage == FIN_PAIR) {
         int pmode = sp.pmode;
         if (pmode == PM_FIRE && !sc.f_do_pair) return;
         const bool reduce = pmode == PM_OBS || pmode == PM_WALLS || pmode == PM_NVT || pmode == PM_NPH ||

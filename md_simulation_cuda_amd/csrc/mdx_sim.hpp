@@ -96,8 +96,9 @@ struct BatchPtrs {
     uint8_t *icls;
     double *partials;       // [3][TB]
     uint32_t *sorted_old, *cs_old;   // previous cell list
-    uint32_t *movers, *mover_off;    // particles whose cell changed, and their offset in it
-    uint32_t *n_movers;              // mover count of the step (reset by k_cell_image)
+    uint32_t *movers, *mover_off;    // per drift workgroup gb: [gb * kBlock, + n_movers[gb]) particles whose cell changed, and their offset in it
+    uint32_t *n_movers;              // per drift workgroup: its mover count this step
+    uint8_t *dirty;                  // cells that received movers this step
     const SysParams *sp;
     DevScalars *sc;
     const int32_t *blk2sys;   // per-particle workgroup -> slot

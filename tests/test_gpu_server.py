@@ -1,0 +1,139 @@
+"""Job server (continuous batching): jobs of every kind, submitted while
+others run, give the same frames and energies as the same systems run alone
+through the single-system API; admission respects the HBM budget."""
+import sys
+
+import numpy as np
+import pytest
+
+from md_simulation_cuda_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+FP = abi.FireParams(0.005, 0.025, 0.1, 1.1, 0.5, 5, 1000, 0.0, 0.0)
+
+
+def job_specs():
+    """(kind, n, rho, T, eps_ab, nsteps, p_ext, seed)"""
+    return [
+        (abi.NVT, 1500, 0.80, 0.7, 0.5, 120, 0.0, 21),
+        (abi.NVE, 900, 0.75, 0.6, 0.3, 75, 0.0, 22),
+        (abi.NPH, 2000, 0.80, 0.8, 0.6, 90, 1.0, 23),
+        (abi.FIRE, 700, 0.85, 0.0, 0.5, 150, 0.0, 24),
+        (abi.NVT, 3000, 0.70, 0.5, 0.9, 60, 0.0, 25),
+        (abi.NVE, 600, 0.82, 1.0, 0.4, 200, 0.0, 26),
+        (abi.FIRE, 1100, 0.80, 0.0, 0.7, 90, 0.0, 27),
+        (abi.NPH, 1200, 0.78, 0.7, 0.5, 45, 1.5, 28),
+        (abi.NVT, 800, 0.80, 0.9, 0.5, 33, 0.0, 29),
+    ]
+
+
+def make(spec):
+    kind, n, rho, T, eab, nsteps, pext, seed = spec
+    if kind == abi.FIRE:
+        f, Lx, Ly = abi.gen_lattice(n, n // 2, rho, 0.2, 0.0, seed)
+    else:
+        f, Lx, Ly = abi.gen_lattice(n, n // 2, rho, 0.05, T, seed)
+    kw = dict(eps_ab=eab, dt=0.005, T_target=max(T, 0.1), Q=10.0, barostat_mass=50.0)
+    return f, Lx, Ly, kw
+
+
+def alone(spec, f, Lx, Ly, kw):
+    kind, n, rho, T, eab, nsteps, pext, seed = spec
+    s = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), f)
+    if kind == abi.FIRE:
+        res, _, _ = s.fire(FP, 1e-6, nsteps)
+        out = s.collect(), res
+    else:
+        s.step(kind, nsteps, p_ext=pext)
+        out = s.collect(), s.observables()
+    s.close()
+    return out
+
+
+def run_server(specs, frames, budget=None, stagger=True):
+    kw = dict(min_class_log2=9, max_class_log2=12, md_slots_per_class=2, fire_slots_per_class=1, quantum=32)
+    if budget is not None:
+        kw["hbm_budget_bytes"] = budget
+    srv = abi.Server(**kw)
+    ids = {}
+    pending = list(range(len(specs)))
+    ticks = 0
+    done = set()
+    while len(done) < len(specs):
+        # submit a few jobs per tick, so later jobs arrive while others run
+        k = 3 if stagger else len(pending)
+        for i in pending[:k]:
+            kind, n, rho, T, eab, nsteps, pext, seed = specs[i]
+            f, Lx, Ly, pkw = frames[i]
+            job = abi.make_job(abi.make_params(Lx, Ly, **pkw), kind, nsteps, p_ext=pext, force_tol=1e-6,
+                               fire=FP)
+            ids[i] = srv.submit(job, f)
+        pending = pending[k:]
+        nr, nq, fin = srv.tick()
+        done.update(fin)
+        ticks += 1
+        assert ticks < 10000
+    return srv, ids
+
+
+def test_server_results_equal_alone_runs():
+    specs = job_specs()
+    frames = [make(s) for s in specs]
+    say("server mix: %d jobs" % len(specs))
+    srv, ids = run_server(specs, frames)
+    st = srv.stats()
+    assert st.completed == len(specs)
+    for i, spec in enumerate(specs):
+        f, Lx, Ly, kw = frames[i]
+        info = srv.info(ids[i])
+        assert info.state == abi.JOB_DONE
+        rec = srv.records(ids[i])
+        got = srv.result(ids[i])
+        ref, ro = alone(spec, f, Lx, Ly, kw)
+        say(f"job {i} kind={spec[0]} n={spec[1]} steps={spec[5]}")
+        assert np.array_equal(got["pos"], ref["pos"]), i
+        assert np.array_equal(got["vel"], ref["vel"]), i
+        if spec[0] == abi.FIRE:
+            assert info.fire.steps == ro.steps and info.fire.converged == ro.converged
+            assert info.fire.max_force == ro.max_force
+            assert info.last.U == ro.potential_energy
+        else:
+            assert info.steps_done == spec[5]
+            assert len(rec) >= 1 and rec["step"][-1] == spec[5]
+            assert np.all(np.diff(rec["step"]) > 0)
+            assert info.last.U == ro.U and info.last.K == ro.K, i
+    srv.close()
+
+
+def test_server_admission_respects_budget():
+    specs = job_specs()[:6]
+    frames = [make(s) for s in specs]
+    # room for about two of the largest slots at a time
+    budget = 2.5 * 4096 * 120.0
+    say("server budget %.0f bytes" % budget)
+    srv, ids = run_server(specs, frames, budget=budget, stagger=False)
+    st = srv.stats()
+    assert st.completed == len(specs)
+    assert 0 < st.peak_reserved_bytes <= budget
+    assert st.reserved_bytes == 0
+    srv.close()
+
+
+def test_server_rejects_oversized_and_bad_jobs():
+    srv = abi.Server(min_class_log2=9, max_class_log2=10, md_slots_per_class=1, fire_slots_per_class=0)
+    f, Lx, Ly = abi.gen_lattice(2000, 1000, 0.8, 0.05, 0.7, 3)
+    with pytest.raises(abi.MdxError):
+        srv.submit(abi.make_job(abi.make_params(Lx, Ly), abi.NVT, 10), f)
+    f, Lx, Ly = abi.gen_lattice(500, 250, 0.8, 0.05, 0.7, 3)
+    with pytest.raises(abi.MdxError):
+        srv.submit(abi.make_job(abi.make_params(Lx, Ly), abi.FIRE, 10), f)   # no FIRE slots
+    with pytest.raises(abi.MdxError):
+        srv.submit(abi.make_job(abi.make_params(Lx, Ly), abi.NVT, 0), f)
+    srv.close()
