@@ -199,6 +199,38 @@ int mdx_timing_read(mdx_sim *sim, mdx_timing *out);
 
 
 
+/* ---- tuning of the pair pass ---------------------------------------------
+ * No counterpart in the reference (one fixed launch shape, no cell list).
+ * cell_scale: cell side >= cell_scale * cutoff * sigma_max (>= 1).
+ * lanes_per_particle: 1, 2 or 4 lanes share one particle's neighbour cells
+ * (64, 32 or 16 particles per wavefront); the FP64 form always uses 1.
+ * Changing either recomputes the accelerations; FP32 results then follow the
+ * new summation order (same tolerance, bit-identical to a fresh run with the
+ * same tuning). */
+typedef struct mdx_tuning {
+    double cell_scale;
+    int32_t lanes_per_particle;
+    int32_t pad;
+} mdx_tuning;
+
+#define MDX_TUNE_MAX 16
+typedef struct mdx_tune_report {
+    mdx_tuning best;
+    double best_ms_per_step;
+    int32_t n_tried;
+    int32_t pad;
+    mdx_tuning tried[MDX_TUNE_MAX];
+    double ms_per_step[MDX_TUNE_MAX];   /* measured with HIP events over `steps` steps */
+} mdx_tune_report;
+
+int mdx_set_tuning(mdx_sim *sim, const mdx_tuning *t);
+int mdx_get_tuning(mdx_sim *sim, mdx_tuning *out);
+/* Autotuner: on a copy of the system, times `steps` steps of ensemble `kind`
+ * for every candidate (cell_scale 1.0, 1.1, 1.25 x lanes_per_particle 1, 2,
+ * 4), then applies the fastest to the handle.  The handle's state is not
+ * advanced. */
+int mdx_autotune(mdx_sim *sim, int32_t kind, int32_t steps, mdx_tune_report *out);
+
 /* ---- batches of independent systems ---------------------------------------
  * No counterpart in the reference, which runs each sweep point (temperature,
  * epsilon_AB, lambda) as a separate job.  A batch holds nslots slots; slot r
@@ -241,6 +273,11 @@ int mdx_batch_timing_read(mdx_batch *b, mdx_timing *out);
 int mdx_batch_fire_begin(mdx_batch *b, int32_t slot, const mdx_fire_params *fp, double force_tol,
                          int32_t max_steps);
 int mdx_batch_fire_poll(mdx_batch *b, int32_t slot, int32_t *done, mdx_fire_result *result);
+/* Tuning of every loaded slot (cell_scale) and of the batch's pair launches;
+ * mdx_batch_autotune times the candidates on a copy of the batch with each
+ * slot's current kind (idle slots as NVE) and applies the fastest. */
+int mdx_batch_set_tuning(mdx_batch *b, const mdx_tuning *t);
+int mdx_batch_autotune(mdx_batch *b, int32_t steps, mdx_tune_report *out);
 
 /* ---- job server: continuous batching of MD jobs -----------------------------
  * No counterpart in the reference, whose drivers (the run/ programs) run one system per

@@ -90,6 +90,22 @@ class InitParams(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("cell_scale", C.c_double), ("lanes_per_particle", C.c_int32), ("pad", C.c_int32)]
+
+
+TUNE_MAX = 16
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("best", Tuning), ("best_ms_per_step", C.c_double), ("n_tried", C.c_int32), ("pad", C.c_int32),
+                ("tried", Tuning * TUNE_MAX), ("ms_per_step", C.c_double * TUNE_MAX)]
+
+    def table(self):
+        return [(self.tried[i].cell_scale, self.tried[i].lanes_per_particle, self.ms_per_step[i])
+                for i in range(self.n_tried)]
+
+
 FIRE = 4   # job kind of the server
 JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
 
@@ -133,6 +149,11 @@ class ServerCounters(C.Structure):
 
 _P = C.c_void_p
 _SIGNATURES = {
+    "mdx_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "mdx_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "mdx_autotune": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(TuneReport)]),
+    "mdx_batch_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "mdx_batch_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(TuneReport)]),
     "mdx_batch_fire_begin": (C.c_int, [_P, C.c_int32, C.POINTER(FireParams), C.c_double, C.c_int32]),
     "mdx_batch_fire_poll": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(FireResult)]),
     "mdx_server_default_params": (C.c_int, [C.POINTER(ServerParams)]),
@@ -358,6 +379,20 @@ class Simulation:
     def set_graph(self, on: bool) -> None:
         check(self._lib.mdx_set_graph(self._h, 1 if on else 0))
 
+    def set_tuning(self, cell_scale: float = 1.0, lanes_per_particle: int = 1) -> None:
+        t = Tuning(cell_scale, lanes_per_particle, 0)
+        check(self._lib.mdx_set_tuning(self._h, C.byref(t)))
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        check(self._lib.mdx_get_tuning(self._h, C.byref(t)))
+        return t
+
+    def autotune(self, kind: int, steps: int = 20) -> TuneReport:
+        rep = TuneReport()
+        check(self._lib.mdx_autotune(self._h, kind, steps, C.byref(rep)))
+        return rep
+
     def timing_enable(self, on: bool) -> None:
         check(self._lib.mdx_timing_enable(self._h, 1 if on else 0))
 
@@ -446,6 +481,15 @@ class Batch:
 
     def set_graph(self, on: bool) -> None:
         check(self._lib.mdx_batch_set_graph(self._h, 1 if on else 0))
+
+    def set_tuning(self, cell_scale: float = 1.0, lanes_per_particle: int = 1) -> None:
+        t = Tuning(cell_scale, lanes_per_particle, 0)
+        check(self._lib.mdx_batch_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, steps: int = 20) -> TuneReport:
+        rep = TuneReport()
+        check(self._lib.mdx_batch_autotune(self._h, steps, C.byref(rep)))
+        return rep
 
     def timing_enable(self, on: bool) -> None:
         check(self._lib.mdx_batch_timing_enable(self._h, 1 if on else 0))

@@ -334,6 +334,39 @@ int mdx_batch_timing_read(mdx_batch *b, mdx_timing *out) {
     return timing_read(b->b, out);
 }
 
+int mdx_set_tuning(mdx_sim *sim, const mdx_tuning *t) {
+    CHECK_HANDLE(sim);
+    if (!t) return set_error(MDX_ERR_INVALID, "null argument");
+    return sim->b.set_tuning(*t);
+}
+
+int mdx_get_tuning(mdx_sim *sim, mdx_tuning *out) {
+    CHECK_HANDLE(sim);
+    if (!out) return set_error(MDX_ERR_INVALID, "null argument");
+    *out = sim->b.tuning();
+    return MDX_OK;
+}
+
+int mdx_autotune(mdx_sim *sim, int32_t kind, int32_t steps, mdx_tune_report *out) {
+    CHECK_HANDLE(sim);
+    if (!out) return set_error(MDX_ERR_INVALID, "null argument");
+    const int k = to_kind(kind);
+    if (k < 0) return set_error(MDX_ERR_INVALID, "unknown ensemble");
+    return sim->b.autotune(steps, out, k);
+}
+
+int mdx_batch_set_tuning(mdx_batch *b, const mdx_tuning *t) {
+    CHECK_HANDLE(b);
+    if (!t) return set_error(MDX_ERR_INVALID, "null argument");
+    return b->b.set_tuning(*t);
+}
+
+int mdx_batch_autotune(mdx_batch *b, int32_t steps, mdx_tune_report *out) {
+    CHECK_HANDLE(b);
+    if (!out) return set_error(MDX_ERR_INVALID, "null argument");
+    return b->b.autotune(steps, out, -1);
+}
+
 int mdx_batch_fire_begin(mdx_batch *b, int32_t slot, const mdx_fire_params *fp, double force_tol,
                          int32_t max_steps) {
     CHECK_HANDLE(b);

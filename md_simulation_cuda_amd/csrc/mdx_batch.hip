@@ -106,7 +106,7 @@ Batch::~Batch() {
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
                      d_sp, d_sc, d_blk2sys, d_cblk2sys, d_want, sorted_old, cs_old, movers, mover_off,
-                     n_movers, dirty};
+                     n_movers, arrivals};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
     for (auto *vec : {&force_ev, &step_ev})
@@ -192,8 +192,8 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     MDX_HIP(hipMalloc(&mover_off, mcap));
     MDX_HIP(hipMalloc(&n_movers, static_cast<size_t>(std::max(TB, 1)) * 4));
     MDX_HIP(hipMemset(n_movers, 0, static_cast<size_t>(std::max(TB, 1)) * 4));
-    MDX_HIP(hipMalloc(&dirty, std::max<size_t>(static_cast<size_t>(C), 1)));
-    MDX_HIP(hipMemset(dirty, 0, std::max<size_t>(static_cast<size_t>(C), 1)));
+    MDX_HIP(hipMalloc(&arrivals, std::max<size_t>(static_cast<size_t>(C), 1) * 4));
+    MDX_HIP(hipMemset(arrivals, 0, std::max<size_t>(static_cast<size_t>(C), 1) * 4));
     MDX_HIP(hipMemset(key, 0, n4));
     MDX_HIP(hipMalloc(&cs, (static_cast<size_t>(C) + 1) * 4));
     MDX_HIP(hipMalloc(&bsum, static_cast<size_t>(ceil_div(C, kScanChunk) + 1) * 4));
@@ -228,7 +228,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     ptrs.movers = movers;
     ptrs.mover_off = mover_off;
     ptrs.n_movers = n_movers;
-    ptrs.dirty = dirty;
+    ptrs.arrivals = arrivals;
     ptrs.sp = d_sp;
     ptrs.sc = d_sc;
     ptrs.blk2sys = d_blk2sys;
@@ -266,24 +266,33 @@ int Batch::sync() {
     return MDX_OK;
 }
 
+// FP32 pair kernel with lpp lanes per particle (1, 2 or 4).
+template <int ACC, bool OBS, bool UNI>
+static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &p) {
+    switch (lpp) {
+    case 2: k_pair<float2, ACC, OBS, UNI, 2><<<grid, kBlock * 2, 0, st>>>(p); break;
+    case 4: k_pair<float2, ACC, OBS, UNI, 4><<<grid, kBlock * 4, 0, st>>>(p); break;
+    default: k_pair<float2, ACC, OBS, UNI, 1><<<grid, kBlock, 0, st>>>(p); break;
+    }
+}
+
 // Launch one step of every slot (their kind/pmode in d_sp).
 int Batch::launch_step_kernels(bool obs) {
     const int S = static_cast<int>(slots.size());
-    k_box<<<ceil_div(S, 64), 64, 0, stream>>>(ptrs);
+    k_box<<<S, kBlock, 0, stream>>>(ptrs);
     if (TB > 0) {
         if (prec == MDX_PREC_COMP32)
             k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs);
         else
             k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs);
     }
-    if (TCB > 0) k_stay<<<TCB, kBlock, 0, stream>>>(ptrs);
-    if (TB > 0) k_arrive<<<TB, kBlock, 0, stream>>>(ptrs);
+    if (TCB + TB > 0) k_stay_arrive<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
     const int nchunk = ceil_div(C, kScanChunk);
     if (nchunk > 0) {
-        k_scan_a<<<nchunk, kBlock, 0, stream>>>(count, C, bsum);
-        k_scan_b<<<nchunk, kBlock, 0, stream>>>(count, C, bsum, cs);
+        k_scan_a<<<nchunk, kBlock, 0, stream>>>(count, arrivals, C, bsum);
+        k_scan_b<<<nchunk, kBlock, 0, stream>>>(count, arrivals, C, bsum, cs);
     }
-    if (TCB > 0) k_fill<<<TCB, kBlock, 0, stream>>>(ptrs);
+    if (TCB + TB > 0) k_fill<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
     if (TCB > 0) k_cell_sort<<<TCB, kBlock, 0, stream>>>(ptrs);
     if (TB > 0) {
         if (prec == MDX_PREC_FP64)
@@ -300,32 +309,31 @@ int Batch::launch_step_kernels(bool obs) {
         switch (prec) {
         case MDX_PREC_COMP32:
             if (obs) {
-                if (uni) k_pair<float2, ACC_COMP32, true, true><<<TB, kBlock, 0, stream>>>(ptrs);
-                else k_pair<float2, ACC_COMP32, true, false><<<TB, kBlock, 0, stream>>>(ptrs);
+                if (uni) launch_pair_f32<ACC_COMP32, true, true>(lpp, TB, stream, ptrs);
+                else launch_pair_f32<ACC_COMP32, true, false>(lpp, TB, stream, ptrs);
             } else {
-                if (uni) k_pair<float2, ACC_COMP32, false, true><<<TB, kBlock, 0, stream>>>(ptrs);
-                else k_pair<float2, ACC_COMP32, false, false><<<TB, kBlock, 0, stream>>>(ptrs);
+                if (uni) launch_pair_f32<ACC_COMP32, false, true>(lpp, TB, stream, ptrs);
+                else launch_pair_f32<ACC_COMP32, false, false>(lpp, TB, stream, ptrs);
             }
             break;
         case MDX_PREC_ACC64:
             if (obs) {
-                if (uni) k_pair<float2, ACC_FP64, true, true><<<TB, kBlock, 0, stream>>>(ptrs);
-                else k_pair<float2, ACC_FP64, true, false><<<TB, kBlock, 0, stream>>>(ptrs);
+                if (uni) launch_pair_f32<ACC_FP64, true, true>(lpp, TB, stream, ptrs);
+                else launch_pair_f32<ACC_FP64, true, false>(lpp, TB, stream, ptrs);
             } else {
-                if (uni) k_pair<float2, ACC_FP64, false, true><<<TB, kBlock, 0, stream>>>(ptrs);
-                else k_pair<float2, ACC_FP64, false, false><<<TB, kBlock, 0, stream>>>(ptrs);
+                if (uni) launch_pair_f32<ACC_FP64, false, true>(lpp, TB, stream, ptrs);
+                else launch_pair_f32<ACC_FP64, false, false>(lpp, TB, stream, ptrs);
             }
             break;
-        default:
-            if (obs) k_pair<double2, ACC_FP64, true, false><<<TB, kBlock, 0, stream>>>(ptrs);
-            else k_pair<double2, ACC_FP64, false, false><<<TB, kBlock, 0, stream>>>(ptrs);
+        default:   // the FP64 form keeps one lane per particle (the oracle's order)
+            if (obs) k_pair<double2, ACC_FP64, true, false, 1><<<TB, kBlock, 0, stream>>>(ptrs);
+            else k_pair<double2, ACC_FP64, false, false, 1><<<TB, kBlock, 0, stream>>>(ptrs);
             break;
         }
         if (e1) {
             (void)hipEventRecord(e1, stream);
             force_ev.emplace_back(e0, e1);
         }
-        k_finalize<<<S, kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
     }
     MDX_HIP(hipGetLastError());
     return MDX_OK;
@@ -368,6 +376,9 @@ int Batch::run_passes(int count_, bool obs) {
     } else {
         for (int k = 0; k < count_; ++k) MDX_TRY(launch_step_kernels(obs));
     }
+    // the last pass's pair reduction (earlier ones are combined by the next k_box)
+    k_finalize<<<static_cast<int>(slots.size()), kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
+    MDX_HIP(hipGetLastError());
     if (s1) {
         (void)hipEventRecord(s1, stream);
         step_ev.emplace_back(s0, s1);
@@ -829,6 +840,126 @@ int Batch::set_precision(int precision) {
     if (!pis.empty()) MDX_TRY(compute_forces(pis, 0, true));
     MDX_HIP(hipStreamSynchronize(stream));
     return MDX_OK;
+}
+
+mdx_tuning Batch::tuning() const {
+    mdx_tuning t{};
+    t.cell_scale = cell_scale;
+    t.lanes_per_particle = lpp;
+    return t;
+}
+
+int Batch::set_tuning(const mdx_tuning &t) {
+    if (!(t.cell_scale >= 1.0) || !std::isfinite(t.cell_scale))
+        return set_error(MDX_ERR_INVALID, "cell_scale must be finite and >= 1");
+    if (t.lanes_per_particle != 1 && t.lanes_per_particle != 2 && t.lanes_per_particle != 4)
+        return set_error(MDX_ERR_INVALID, "lanes_per_particle must be 1, 2 or 4");
+    MDX_HIP(hipSetDevice(device));
+    MDX_HIP(hipStreamSynchronize(stream));
+    const int new_lpp = prec == MDX_PREC_FP64 ? 1 : t.lanes_per_particle;
+    if (new_lpp != lpp) {
+        lpp = new_lpp;
+        invalidate_graphs();
+    }
+    cell_scale = t.cell_scale;
+    std::vector<int> per, pis;
+    for (size_t r = 0; r < slots.size(); ++r) {
+        Slot &s = slots[r];
+        if (!s.loaded) continue;
+        const double side = t.cell_scale * s.rc_max;
+        DevScalars h;
+        MDX_TRY(read_scalars(static_cast<int>(r), h));
+        const Grid g = make_grid(h.g.Lx, h.g.Ly, side, s.rc_max, 1);
+        if (static_cast<int64_t>(g.ncx) * g.ncy > s.ccap)
+            return set_error(MDX_ERR_CAPACITY, "cell_scale gives more cells than the slot holds");
+        s.p.cell_scale = t.cell_scale;
+        h_sp[r].cell_side = side;
+        MDX_TRY(upload_sp(static_cast<int>(r)));
+        (s.piston_active ? pis : per).push_back(static_cast<int>(r));
+    }
+    // accelerations in the new summation order, as a fresh load would have them
+    if (!per.empty()) MDX_TRY(compute_forces(per, 1, false));
+    if (!pis.empty()) MDX_TRY(compute_forces(pis, 0, true));
+    MDX_HIP(hipStreamSynchronize(stream));
+    return MDX_OK;
+}
+
+int Batch::autotune(int steps, mdx_tune_report *rep, int kind_override) {
+    if (steps <= 0) return set_error(MDX_ERR_INVALID, "autotune: steps must be positive");
+    MDX_HIP(hipSetDevice(device));
+    std::memset(rep, 0, sizeof(*rep));
+    // a copy of the batch at the current state: same capacities and precision,
+    // every loaded slot with its current frame, box and kind
+    Batch c;
+    std::vector<int64_t> caps;
+    std::vector<int32_t> ccaps;
+    for (const Slot &s : slots) {
+        caps.push_back(s.n_cap);
+        ccaps.push_back(s.ccap);
+    }
+    MDX_TRY(c.init(static_cast<int>(slots.size()), caps.data(), ccaps.data(), prec, device));
+    bool any = false;
+    for (size_t r = 0; r < slots.size(); ++r) {
+        const Slot &s = slots[r];
+        if (!s.loaded) continue;
+        std::vector<mdx_particle64> f(s.n);
+        MDX_TRY(collect(static_cast<int>(r), f.data(), s.n));
+        DevScalars h;
+        MDX_TRY(read_scalars(static_cast<int>(r), h));
+        mdx_params p = s.p;
+        p.Lx = h.g.Lx;
+        p.Ly = h.g.Ly;
+        MDX_TRY(c.load(static_cast<int>(r), p, f.data(), s.n));
+        int k = kind_override >= 0 ? kind_override : s.kind;
+        if (k == SK_NONE || k == SK_FIRE || k == SK_KEYS) k = SK_NVE;
+        MDX_TRY(c.set_kind(static_cast<int>(r), k, s.p_ext, s.flags & ~MDX_STEP_MIDDLE_WRAP));
+        any = true;
+    }
+    if (!any) return set_error(MDX_ERR_STATE, "autotune: no loaded slot");
+    const double scales[3] = {1.0, 1.1, 1.25};
+    const int lanes[3] = {1, 2, 4};
+    double best = 1e300;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MDX_HIP(hipEventCreate(&e0));
+    MDX_HIP(hipEventCreate(&e1));
+    int rc = MDX_OK;
+    for (double cs : scales) {
+        for (int l : lanes) {
+            if (prec == MDX_PREC_FP64 && l != 1) continue;
+            if (rep->n_tried >= MDX_TUNE_MAX) break;
+            mdx_tuning t{};
+            t.cell_scale = cs;
+            t.lanes_per_particle = l;
+            rc = c.set_tuning(t);
+            if (rc == MDX_ERR_CAPACITY) continue;   // too many cells for a slot: not a candidate
+            if (rc != MDX_OK) break;
+            if ((rc = c.step(3)) != MDX_OK) break;   // warm-up: graph capture, caches
+            (void)hipEventRecord(e0, c.stream);
+            if ((rc = c.step(steps)) != MDX_OK) break;
+            (void)hipEventRecord(e1, c.stream);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                rc = set_error(MDX_ERR_HIP, "autotune: event wait failed");
+                break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double per = static_cast<double>(ms) / steps;
+            rep->tried[rep->n_tried] = t;
+            rep->ms_per_step[rep->n_tried] = per;
+            ++rep->n_tried;
+            if (per < best) {
+                best = per;
+                rep->best = t;
+                rep->best_ms_per_step = per;
+            }
+        }
+        if (rc != MDX_OK && rc != MDX_ERR_CAPACITY) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != MDX_OK && rc != MDX_ERR_CAPACITY) return rc;
+    if (rep->n_tried == 0) return set_error(MDX_ERR_STATE, "autotune: no candidate fits the slots");
+    return set_tuning(rep->best);
 }
 
 // Slot r starts FIRE (EnergyMinimizer::minimize_frame, src/md_energy_minimizer.cu:

@@ -58,7 +58,7 @@ struct Batch {
              *bsum = nullptr;
     uint32_t *sorted_old = nullptr, *cs_old = nullptr, *movers = nullptr, *mover_off = nullptr,
              *n_movers = nullptr;
-    uint8_t *dirty = nullptr;
+    uint32_t *arrivals = nullptr;
     double *partials = nullptr;
     SysParams *d_sp = nullptr;
     DevScalars *d_sc = nullptr;
@@ -68,6 +68,8 @@ struct Batch {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     bool use_graph = true;
     bool graph_uni = true;   // pair-kernel variant the captured graphs hold
+    int lpp = 1;             // lanes per particle of the FP32 pair kernel (1, 2, 4)
+    double cell_scale = 1.0; // last cell_scale applied by set_tuning (0: per-slot params)
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> force_ev, step_ev;
@@ -103,6 +105,12 @@ struct Batch {
     int fire_poll(int r, int *done, mdx_fire_result *res);
     // K/U/P/virial of several slots from one kinetic and one energy pass
     int observables_many(const std::vector<int> &which, uint32_t mask, mdx_obs *out);
+    // pair-pass tuning: cell size of every loaded slot and lanes per particle
+    int set_tuning(const mdx_tuning &t);
+    mdx_tuning tuning() const;
+    // times the candidates on a copy of the batch (kind_override >= 0: every
+    // loaded slot steps that kind) and applies the fastest
+    int autotune(int steps, mdx_tune_report *rep, int kind_override);
     int set_precision(int precision);
     int read_scalars(int r, DevScalars &h);
     int sync();

@@ -117,10 +117,10 @@ __device__ inline double wave_max(double v) {
 // component at partials[k * stride + gblk].  Component k is a max when bit k of
 // maxmask is set, a sum otherwise.  A later kernel (slot_reduce) combines a
 // slot's partials in index order; the kernel boundary orders the stores, so no
-// device-scope fence or atomic is needed.  Requires blockDim.x == kBlock.
-template <int NV>
+// device-scope fence or atomic is needed.  Requires blockDim.x == 64 * NW.
+template <int NV, int NW = kWaves>
 __device__ void block_partials(double (&v)[NV], unsigned maxmask, double *partials, int stride, int gblk) {
-    __shared__ double sh[NV][kWaves];
+    __shared__ double sh[NV][NW];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
 #pragma unroll
@@ -135,7 +135,7 @@ __device__ void block_partials(double (&v)[NV], unsigned maxmask, double *partia
         for (int k = 0; k < NV; ++k) {
             const bool mx = (maxmask >> k) & 1u;
             double b = sh[k][0];
-            for (int w = 1; w < kWaves; ++w) b = mx ? fmax(b, sh[k][w]) : b + sh[k][w];
+            for (int w = 1; w < NW; ++w) b = mx ? fmax(b, sh[k][w]) : b + sh[k][w];
             partials[k * stride + gblk] = b;
         }
     }

@@ -6,22 +6,25 @@
 //   a         float2 (compensated FP32 form) or double2 (FP64-accumulating forms)
 //   cls       uint8         type class: 0, 1, or 2 for any other type
 //   key       uint32        global cell index of each particle
-//   count     uint32[C]     per-cell occupancy of the list being built
+//   count     uint32[C]     per-cell stayers of the list being built
+//   arrivals  uint32[C]     per-cell movers arriving this step (zero at rest)
 //   cs        uint32[C+1]   cell start: exclusive prefix of count over all cells
 //   sorted    uint32        particle ids in cell order, ascending id per cell
 //   img       float2/double2, in cell order: cell-centre-relative FP32
 //                           position, or the wrapped FP64 position
 //   icls, scell             class and slot-local cell of each sorted slot
 //
-// One step = k_box (one thread per slot: box, barostat, piston, grid, FIRE
-// bookkeeping) -> k_drift (kick/drift/wrap, cell key, the ensemble's
-// reduction partials, movers) -> k_finalize -> k_stay -> k_arrive ->
-// k_scan_a/k_scan_b -> k_fill -> k_cell_sort (id order of the cells that
-// received movers) -> k_slot_image (per-slot image, coalesced) -> k_pair
-// (forces, second kick, the ensemble's reduction partials) -> k_finalize.
-// The cell list is updated from the previous one: only particles whose cell
+// One step = k_box (one workgroup per slot: the previous pair pass's
+// reduction, then box, barostat, piston, grid, FIRE bookkeeping) -> k_drift
+// (kick/drift/wrap, cell key, the ensemble's reduction partials, movers) ->
+// k_stay_arrive (stayers per cell, the drift reduction, movers' arrival
+// index) -> k_scan_a/k_scan_b -> k_fill -> k_cell_sort (id order of the cells
+// that received movers) -> k_slot_image (per-slot image, coalesced) -> k_pair
+// (forces, second kick, the ensemble's reduction partials).  A run of steps
+// ends with k_finalize, which combines the last pair pass's partials.  The
+// cell list is updated from the previous one: only particles whose cell
 // changed ("movers") use atomics.  Global FP64 scalars are combined from the
-// workgroup partials by one workgroup per slot (k_finalize), in a fixed order.
+// workgroup partials by one workgroup per slot, in a fixed order.
 #pragma once
 #include <math.h>
 
@@ -46,16 +49,33 @@ __device__ inline double2 load_acc(const void *a, int i) {
 // per-slot scalar kernel
 // --------------------------------------------------------------------------
 
-// One thread per slot, first kernel of a step.  NPH: area barostat of
-// src/md_env.cu:1824-1877.  Piston: src/md_env.cu:1950-1980.  FIRE: decides
-// whether the pending drift and the next force pass run.  Every slot: the
-// cell grid of its (possibly new) box.
-__global__ void k_box(BatchPtrs P) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= P.S) return;
+__device__ inline void finalize_pair(int pmode, const double (&out)[3], const SysParams &sp, const Grid &g,
+                                     DevScalars &sc);
+
+// Combines the partials of the slot's last pair pass (one whole workgroup)
+// and applies its mode's scalar update (finalize_pair).
+__device__ inline void finalize_pair_pending(const BatchPtrs &P, const SysParams &sp, DevScalars &sc) {
+    const int pmode = sp.pmode;
+    double out[3];
+    slot_reduce<3>(pmode == PM_FIRE ? 1u : 0u, P.partials, P.TB, sp.boff, sp.nblk, out);
+    if (threadIdx.x == 0) {
+        finalize_pair(pmode, out, sp, sc.g, sc);
+        sc.pair_pending = 0;
+    }
+}
+
+// One workgroup per slot, first kernel of a step.  It first combines the
+// previous pass's pair reduction, then thread 0 updates the slot's box.
+// NPH: area barostat of src/md_env.cu:1824-1877.  Piston:
+// src/md_env.cu:1950-1980.  FIRE: decides whether the pending drift and the
+// next force pass run.  Every slot: the cell grid of its (possibly new) box.
+__global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P) {
+    const int r = blockIdx.x;
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
     if (sp.n <= 0) return;
+    if (sc.pair_pending && sp.nblk > 0) finalize_pair_pending(P, sp, sc);
+    if (threadIdx.x != 0) return;
     double Lx = sc.g.Lx, Ly = sc.g.Ly;
     if (sp.kind == SK_NPH) {
         const double area_old = Lx * Ly;
@@ -283,14 +303,15 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P) {
 // cell list: scan, scatter, per-cell sort + image
 // --------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(kBlock) k_scan_a(const uint32_t *__restrict__ count, int ncell,
+__global__ void __launch_bounds__(kBlock) k_scan_a(const uint32_t *__restrict__ count,
+                                                   const uint32_t *__restrict__ arrivals, int ncell,
                                                    uint32_t *__restrict__ bsum) {
     __shared__ uint32_t sh[kWaves];
     const int base = blockIdx.x * kScanChunk + threadIdx.x * 4;
     uint32_t s = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-        if (base + j < ncell) s += count[base + j];
+        if (base + j < ncell) s += count[base + j] + arrivals[base + j];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
@@ -301,7 +322,8 @@ __global__ void __launch_bounds__(kBlock) k_scan_a(const uint32_t *__restrict__ 
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_b(const uint32_t *__restrict__ count, int ncell,
+__global__ void __launch_bounds__(kBlock) k_scan_b(const uint32_t *__restrict__ count,
+                                                   const uint32_t *__restrict__ arrivals, int ncell,
                                                    const uint32_t *__restrict__ bsum, uint32_t *__restrict__ cs) {
     __shared__ uint32_t shw[kWaves];
     __shared__ uint32_t shp[kWaves];
@@ -315,7 +337,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_b(const uint32_t *__restrict__ 
     uint32_t tsum = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        c[j] = (base + j < ncell) ? count[base + j] : 0u;
+        c[j] = (base + j < ncell) ? count[base + j] + arrivals[base + j] : 0u;
         tsum += c[j];
     }
     uint32_t inc = tsum;
@@ -351,11 +373,23 @@ __device__ inline void finalize_drift(const BatchPtrs &P, const SysParams &sp, D
     }
 }
 
-// One thread per cell of the new grid (and of the previous grid when it was
-// larger): count[c] = particles of the previous list of cell c that stay in c.
-// Cells beyond the new grid get zero.
-__global__ void __launch_bounds__(kBlock) k_stay(BatchPtrs P) {
-    const int gb = xcd_block(blockIdx.x, gridDim.x);
+// Workgroups [0, ncb): one thread per cell of the new grid (and of the
+// previous grid when it was larger): count[c] = particles of the previous list
+// of cell c that stay in c; cells beyond the new grid get zero.  The slot's
+// first cell workgroup also combines the drift reduction.  Workgroups
+// [ncb, ncb + TB): drift workgroup b's movers take an arrival index in their
+// new cell (atomics on distinct cells).  The two parts touch different arrays.
+__global__ void __launch_bounds__(kBlock) k_stay_arrive(BatchPtrs P, int ncb) {
+    if (static_cast<int>(blockIdx.x) >= ncb) {
+        const uint32_t b = blockIdx.x - ncb;
+        const uint32_t nm = P.n_movers[b];
+        if (threadIdx.x < nm) {
+            const uint32_t m = b * kBlock + threadIdx.x;
+            P.mover_off[m] = atomicAdd(&P.arrivals[P.key[P.movers[m]]], 1u);
+        }
+        return;
+    }
+    const int gb = xcd_block(blockIdx.x, ncb);
     const int r = P.cblk2sys[gb];
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
@@ -376,24 +410,23 @@ __global__ void __launch_bounds__(kBlock) k_stay(BatchPtrs P) {
     P.count[c] = stay;
 }
 
-// Movers take an offset inside their new cell, after the stayers.  One
-// workgroup per drift workgroup's mover list.
-__global__ void __launch_bounds__(kBlock) k_arrive(BatchPtrs P) {
-    const uint32_t b = blockIdx.x;
-    const uint32_t nm = P.n_movers[b];
-    if (threadIdx.x < nm) {
-        const uint32_t m = b * kBlock + threadIdx.x;
-        const uint32_t c = P.key[P.movers[m]];
-        P.mover_off[m] = atomicAdd(&P.count[c], 1u);
-        P.dirty[c] = 1;   // the cell's order needs restoring
+// Workgroups [0, ncb): one thread per cell of the new grid copies the cell's
+// stayers in their previous order (ascending id) to the start of the cell.
+// Workgroups [ncb, ncb + TB): drift workgroup b's movers go after the stayers
+// of their new cell, at their arrival index.
+__global__ void __launch_bounds__(kBlock) k_fill(BatchPtrs P, int ncb) {
+    if (static_cast<int>(blockIdx.x) >= ncb) {
+        const uint32_t b = blockIdx.x - ncb;
+        const uint32_t nm = P.n_movers[b];
+        if (threadIdx.x < nm) {
+            const uint32_t m = b * kBlock + threadIdx.x;
+            const uint32_t id = P.movers[m];
+            const uint32_t c = P.key[id];
+            P.sorted[P.cs[c] + P.count[c] + P.mover_off[m]] = id;
+        }
+        return;
     }
-}
-
-// Stayers keep their previous order (ascending id); movers go to their
-// offsets.  One thread per cell of the new grid, then a grid-stride loop over
-// the movers.
-__global__ void __launch_bounds__(kBlock) k_fill(BatchPtrs P) {
-    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int gb = xcd_block(blockIdx.x, ncb);
     const int r = P.cblk2sys[gb];
     const SysParams &sp = P.sp[r];
     const DevScalars &sc = P.sc[r];
@@ -407,14 +440,6 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchPtrs P) {
         for (uint32_t k = s0; k < s1; ++k) {
             const uint32_t id = P.sorted_old[k];
             if (P.key[id] == c) P.sorted[dst++] = id;
-        }
-    }
-    for (int b = blockIdx.x; b < P.TB; b += gridDim.x) {   // every drift workgroup's movers
-        const uint32_t nm = P.n_movers[b];
-        if (threadIdx.x < nm) {
-            const uint32_t m = static_cast<uint32_t>(b) * kBlock + threadIdx.x;
-            const uint32_t id = P.movers[m];
-            P.sorted[P.cs[P.key[id]] + P.mover_off[m]] = id;
         }
     }
 }
@@ -435,8 +460,8 @@ __global__ void __launch_bounds__(kBlock) k_cell_sort(BatchPtrs P) {
     const uint32_t s0 = P.cs[c], s1 = P.cs[c + 1];
     P.cs_old[c] = s0;
     if (lc == ncell - 1) P.cs_old[c + 1] = s1;
-    if (!P.dirty[c]) return;
-    P.dirty[c] = 0;
+    if (P.arrivals[c] == 0) return;
+    P.arrivals[c] = 0;
     uint32_t *sorted = P.sorted;
     for (uint32_t k = s0 + 1; k < s1; ++k) {
         const uint32_t id = sorted[k];
@@ -631,13 +656,16 @@ struct PairF32 {
 // (which adds every pair in FP64, as the reference does).  Self is excluded by
 // r^2 > 0.  UNI: all type pairs share sigma (rc and sigma^2 need no select);
 // the arithmetic is identical to the general form.
-template <int ACC, bool OBS, bool UNI>
+//
+template <int ACC, bool OBS, bool UNI, int LPP>
 __device__ __forceinline__ void pair_cell_f32(const char *__restrict__ imgb, const uint8_t *__restrict__ clsb,
                                               uint32_t k0, uint32_t k1, float pix, float piy, int ci,
                                               const Grid &g, const PairTable &pt, const PairF32 &pc,
                                               ForceSum<ACC> &F, ObsSum &ob, bool want_obs) {
     float fxc = 0.0f, fyc = 0.0f;
-    for (uint32_t k = k0; k < k1; ++k) {
+    // with LPP lanes per particle, lane q of the group takes candidates
+    // k0 + q, k0 + q + LPP, ...
+    for (uint32_t k = k0 + (LPP > 1 ? threadIdx.x % LPP : 0u); k < k1; k += LPP) {
         const float2 pj = *reinterpret_cast<const float2 *>(imgb + (k << 3));
         const int cj = clsb[k];
         float dx = pix - pj.x;
@@ -712,16 +740,45 @@ __device__ __forceinline__ void pair_cell_f64(const double2 *__restrict__ img, c
     }
 }
 
-// Forces of every particle of every active slot (thread per cell-ordered
+// Sum of the LPP lanes' partial forces and observables of one particle
+// (xor butterfly inside the wave; every lane of the group ends with the same
+// totals: the two-sum error term is exact, the other additions commute).
+template <int LPP, int ACC>
+__device__ inline void combine_lanes(ForceSum<ACC> &F, ObsSum &ob) {
+#pragma unroll
+    for (int o = 1; o < LPP; o <<= 1) {
+        if constexpr (ACC == ACC_COMP32) {
+            const float sx = __shfl_xor(F.fx.s, o, 64), cx = __shfl_xor(F.fx.c, o, 64);
+            const float sy = __shfl_xor(F.fy.s, o, 64), cy = __shfl_xor(F.fy.c, o, 64);
+            F.fx.add(sx);
+            F.fx.c += cx;
+            F.fy.add(sy);
+            F.fy.c += cy;
+            F.gx += __shfl_xor(F.gx, o, 64);
+            F.gy += __shfl_xor(F.gy, o, 64);
+        } else {
+            F.fx += __shfl_xor(F.fx, o, 64);
+            F.fy += __shfl_xor(F.fy, o, 64);
+        }
+        ob.u += __shfl_xor(ob.u, o, 64);
+        ob.w += __shfl_xor(ob.w, o, 64);
+        ob.lam += __shfl_xor(ob.lam, o, 64);
+    }
+}
+
+// Forces of every particle of every active slot (LPP lanes per cell-ordered
 // slot, 3x3 neighbour cells in the oracle's order: rows -1, 0, +1, cells -1,
 // 0, +1, ascending id), then the slot's mode: second kick, FIRE kick, walls,
 // and its reduction.  OBS = true compiles the energy/virial/lambda sums (used
 // when some slot needs them); UNI = true when every slot of the batch has one
-// sigma for all type pairs.
-template <typename IT, int ACC, bool OBS, bool UNI>
-__global__ void __launch_bounds__(kBlock) k_pair(BatchPtrs P) {
+// sigma for all type pairs.  A workgroup (kBlock * LPP threads) covers the
+// same kBlock particles whatever LPP is; LPP > 1 only for the FP32 pair forms.
+template <typename IT, int ACC, bool OBS, bool UNI, int LPP>
+__global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P) {
     using AT = std::conditional_t<ACC == ACC_COMP32, float2, double2>;
     constexpr bool kFp64 = std::is_same_v<IT, double2>;
+    static_assert(!kFp64 || LPP == 1, "the FP64 form keeps the oracle's sequential order");
+    static_assert(LPP == 1 || LPP == 2 || LPP == 4, "lanes per particle");
     const int gb = xcd_block(blockIdx.x, gridDim.x);
     const int r = P.blk2sys[gb];
     const SysParams &sp = P.sp[r];
@@ -732,16 +789,17 @@ __global__ void __launch_bounds__(kBlock) k_pair(BatchPtrs P) {
     const bool want_obs = OBS && (pmode == PM_OBS || pmode == PM_NPH);
     const Grid g = sc.g;
     const PairTable &pt = sp.pt;
-    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
+    const int li = (gb - sp.boff) * kBlock + static_cast<int>(threadIdx.x) / LPP;
+    const bool lead = LPP == 1 || threadIdx.x % LPP == 0;
     double r0 = 0.0, r1 = 0.0, r2v = 0.0;
+    ForceSum<kFp64 ? ACC_FP64 : ACC> F;
+    ObsSum ob;
     if (li < sp.n) {
         const uint32_t s = P.cs[sp.coff] + static_cast<uint32_t>(li);
         const uint32_t lc = P.scell[s];
         const int cx = static_cast<int>(lc % static_cast<uint32_t>(g.ncx));
         const int cy = static_cast<int>(lc / static_cast<uint32_t>(g.ncx));
         const int ci = P.icls[s];
-        ForceSum<kFp64 ? ACC_FP64 : ACC> F;
-        ObsSum ob;
         const IT pi = static_cast<const IT *>(P.img)[s];
         PairF32 pc{};
         if constexpr (!kFp64) {
@@ -786,11 +844,18 @@ __global__ void __launch_bounds__(kBlock) k_pair(BatchPtrs P) {
             } else {
                 const char *imgb = static_cast<const char *>(P.img);
                 const float piy = pi.y - static_cast<float>(oy) * g.hf;
-                pair_cell_f32<ACC, OBS, UNI>(imgb, P.icls, a0, a1, pi.x + g.wf, piy, ci, g, pt, pc, F, ob, want_obs);
-                pair_cell_f32<ACC, OBS, UNI>(imgb, P.icls, b0, b1, pi.x, piy, ci, g, pt, pc, F, ob, want_obs);
-                pair_cell_f32<ACC, OBS, UNI>(imgb, P.icls, c0, c1, pi.x - g.wf, piy, ci, g, pt, pc, F, ob, want_obs);
+                pair_cell_f32<ACC, OBS, UNI, LPP>(imgb, P.icls, a0, a1, pi.x + g.wf, piy, ci, g, pt, pc, F, ob,
+                                                  want_obs);
+                pair_cell_f32<ACC, OBS, UNI, LPP>(imgb, P.icls, b0, b1, pi.x, piy, ci, g, pt, pc, F, ob, want_obs);
+                pair_cell_f32<ACC, OBS, UNI, LPP>(imgb, P.icls, c0, c1, pi.x - g.wf, piy, ci, g, pt, pc, F, ob,
+                                                  want_obs);
             }
         }
+        if constexpr (LPP > 1) combine_lanes<LPP, kFp64 ? ACC_FP64 : ACC>(F, ob);
+    }
+    if (li < sp.n && lead) {
+        const uint32_t s = P.cs[sp.coff] + static_cast<uint32_t>(li);
+        const int ci = P.icls[s];
         const uint32_t id = P.sorted[s];
         double axd, ayd;
         if constexpr (kFp64 || ACC == ACC_FP64) {
@@ -867,7 +932,8 @@ __global__ void __launch_bounds__(kBlock) k_pair(BatchPtrs P) {
                         pmode == PM_PISTON || pmode == PM_FIRE;
     if (reduce) {
         double vals[3] = {r0, r1, r2v};
-        block_partials<3>(vals, pmode == PM_FIRE ? 1u : 0u, P.partials, P.TB, gb);
+        block_partials<3, kWaves * LPP>(vals, pmode == PM_FIRE ? 1u : 0u, P.partials, P.TB, gb);
+        if (gb == sp.boff && threadIdx.x == 0) sc.pair_pending = 1;
     }
 }
 
@@ -886,14 +952,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(BatchPtrs P, int32_t stage,
     DevScalars &sc = P.sc[r];
     if (sp.n <= 0 || sp.nblk <= 0) return;
     if (stage == FIN_PAIR) {
-        int pmode = sp.pmode;
-        if (pmode == PM_FIRE && !sc.f_do_pair) return;
-        const bool reduce = pmode == PM_OBS || pmode == PM_WALLS || pmode == PM_NVT || pmode == PM_NPH ||
-                            pmode == PM_PISTON || pmode == PM_FIRE;
-        if (!reduce) return;
-        double out[3];
-        slot_reduce<3>(pmode == PM_FIRE ? 1u : 0u, P.partials, P.TB, sp.boff, sp.nblk, out);
-        if (threadIdx.x == 0) finalize_pair(pmode, out, sp, sc.g, sc);
+        if (sc.pair_pending) finalize_pair_pending(P, sp, sc);
     } else if (stage == FIN_KINETIC) {
         if (!want[r]) return;
         double out[1];

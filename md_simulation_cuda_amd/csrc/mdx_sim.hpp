@@ -70,7 +70,7 @@ struct DevScalars {
     Grid g;              // current cell grid (box lives in g.Lx, g.Ly)
     Grid g_old;          // grid of the previous cell list
     int32_t list_valid;  // the previous cell list (cs_old, sorted_old) holds every particle
-    int32_t pad0;
+    int32_t pair_pending;  // the last pair pass left partials that k_box / k_finalize combine
     double xi;           // Nose-Hoover variable after the last completed half update
     double xi_mid;       // xi of the step in flight (after both half updates)
     double K;            // kinetic energy of the current velocities
@@ -96,9 +96,11 @@ struct BatchPtrs {
     uint8_t *icls;
     double *partials;       // [3][TB]
     uint32_t *sorted_old, *cs_old;   // previous cell list
-    uint32_t *movers, *mover_off;    // per drift workgroup gb: [gb * kBlock, + n_movers[gb]) particles whose cell changed, and their offset in it
-    uint32_t *n_movers;              // per drift workgroup: its mover count this step
-    uint8_t *dirty;                  // cells that received movers this step
+    // per drift workgroup gb: movers[gb * kBlock + j], j < n_movers[gb], are
+    // the particles whose cell changed; mover_off their arrival index
+    uint32_t *movers, *mover_off;
+    uint32_t *n_movers;
+    uint32_t *arrivals;              // per cell: movers that arrived this step (zero at rest)
     const SysParams *sp;
     DevScalars *sc;
     const int32_t *blk2sys;   // per-particle workgroup -> slot

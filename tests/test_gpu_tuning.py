@@ -1,0 +1,128 @@
+"""Pair-pass tuning (cell size, lanes per particle) and the autotuner: every
+tuning stays within the FP32 force tolerance of the FP64 oracle, a tuned
+system equals a fresh system given the same tuning bit for bit (alone and in
+a batch), and the autotuner leaves the state where it was."""
+import sys
+
+import numpy as np
+import pytest
+
+from md_simulation_cuda_amd import abi
+from oracle import oracle as O
+from tests.helpers import lattice_frame, near_cutoff_mask
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+TUNINGS = [(1.0, 1), (1.0, 2), (1.0, 4), (1.1, 1), (1.25, 2), (1.25, 4)]
+
+
+@pytest.mark.parametrize("cs,lanes", TUNINGS)
+@pytest.mark.parametrize("prec", [abi.PREC_COMP32, abi.PREC_ACC64])
+def test_tuned_forces_within_tolerance(cs, lanes, prec):
+    f, Lx, Ly = lattice_frame(4096, 0.8, seed=1001, jitter=0.05, T=0.7)
+    say(f"tuned forces cs={cs} lanes={lanes} prec={prec}")
+    g = abi.Simulation(abi.make_params(Lx, Ly, precision=prec, eps_ab=0.5), f)
+    g.set_tuning(cs, lanes)
+    t = g.get_tuning()
+    assert t.cell_scale == cs and t.lanes_per_particle == lanes
+    acc = g.compute_forces(True)
+    o = O.OracleSim(O.make_params(Lx, Ly, eps_ab=0.5), f)
+    acc_o, _, _, _ = o.pair_pass(True)
+    rms = np.sqrt((acc_o ** 2).sum(1).mean())
+    err = np.sqrt(((acc - acc_o) ** 2).sum(1))
+    band = near_cutoff_mask(f, Lx, Ly, 2.5, (1.0, 1.0, 1.0))
+    fe, fr = err[~band].max() / rms, np.sqrt((err ** 2).mean()) / rms
+    say(f"  force max rel {fe:.2e} rms rel {fr:.2e}")
+    ob = g.observables()
+    du = abs(ob.U - o.U()) / abs(o.U())
+    dp = abs(ob.P - o.pressure()) / abs(o.pressure())
+    say(f"  U rel {du:.2e} P rel {dp:.2e}")
+    g.close()
+    # the FP32 image is relative to the cell centre, so its rounding (and the
+    # force error) grows with the cell side: force max 1.3e-5 and pressure
+    # 6.5e-6 at cell_scale 1.1 on this frame, against 1e-5 and 2e-6 at the
+    # minimum cell
+    big = cs > 1.0
+    assert fe <= (2.5e-5 if big else 1e-5)
+    assert fr <= 2e-6 * cs * cs
+    assert du <= (2e-6 if big else 1e-6)
+    assert dp <= (2e-5 if big else 5e-6)
+
+
+def test_tuned_equals_fresh_with_same_tuning_alone_and_in_batch():
+    f, Lx, Ly = lattice_frame(3000, 0.75, seed=5, jitter=0.05, T=0.6, aspect=2.0)
+    kw = dict(eps_ab=0.5, dt=0.005, T_target=0.6, Q=10.0)
+    say("tuned vs fresh")
+    # tuning applied after a default load ...
+    b = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+    b.set_tuning(1.1, 2)
+    b.step(abi.NVT, 30)
+    fb = b.collect()
+    # ... equals a load that asked for the cell size, then the same tuning
+    c = abi.Simulation(abi.make_params(Lx, Ly, cell_scale=1.1, **kw), f)
+    c.set_tuning(1.1, 2)
+    c.step(abi.NVT, 30)
+    fc = c.collect()
+    assert np.array_equal(fb["pos"], fc["pos"]) and np.array_equal(fb["vel"], fc["vel"])
+    # and the same system in a batch beside another slot, same tuning
+    f2, L2x, L2y = lattice_frame(1200, 0.8, seed=6, jitter=0.05, T=0.7)
+    bt = abi.Batch([3000, 1200], [abi.cell_capacity(abi.make_params(Lx, Ly), 1.0),
+                                  abi.cell_capacity(abi.make_params(L2x, L2y), 1.0)])
+    bt.load(0, abi.make_params(Lx, Ly, **kw), f)
+    bt.load(1, abi.make_params(L2x, L2y, eps_ab=0.4, dt=0.005, T_target=0.7, Q=10.0), f2)
+    bt.set_tuning(1.1, 2)
+    bt.set_kind(0, abi.NVT)
+    bt.set_kind(1, abi.NVE)
+    bt.step(30)
+    fd = bt.collect(0)
+    assert np.array_equal(fb["pos"], fd["pos"]) and np.array_equal(fb["vel"], fd["vel"])
+    # a different lane split sums in another order: equal within tolerance only
+    e = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+    e.set_tuning(1.1, 4)
+    e.step(abi.NVT, 30)
+    fe = e.collect()
+    assert np.abs(fe["pos"] - fb["pos"]).max() < 1e-3
+    for h in (b, c, bt, e):
+        h.close()
+
+
+def test_autotune_reports_candidates_and_keeps_state():
+    f, Lx, Ly = lattice_frame(20000, 0.8, seed=7, jitter=0.05, T=0.7)
+    say("autotune")
+    s = abi.Simulation(abi.make_params(Lx, Ly, eps_ab=0.5, dt=0.005, T_target=0.7, Q=10.0), f)
+    s.step(abi.NVT, 5)
+    before = s.collect()
+    rep = s.autotune(abi.NVT, 10)
+    say(f"  tried {rep.table()} best {rep.best.cell_scale} {rep.best.lanes_per_particle}")
+    assert rep.n_tried == 9
+    assert all(m > 0 for _, _, m in rep.table())
+    assert rep.best_ms_per_step == min(m for _, _, m in rep.table())
+    t = s.get_tuning()
+    assert t.cell_scale == rep.best.cell_scale and t.lanes_per_particle == rep.best.lanes_per_particle
+    after = s.collect()
+    assert np.array_equal(before["pos"], after["pos"]) and np.array_equal(before["vel"], after["vel"])
+    s.step(abi.NVT, 20)
+    ob = s.observables()
+    assert np.isfinite(ob.K) and np.isfinite(ob.U)
+    s.close()
+
+
+def test_fp64_keeps_one_lane():
+    f, Lx, Ly = lattice_frame(1500, 0.8, seed=8, jitter=0.05, T=0.7)
+    s = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_FP64, eps_ab=0.5), f)
+    s.set_tuning(1.0, 4)
+    assert s.get_tuning().lanes_per_particle == 1
+    o = O.OracleSim(O.make_params(Lx, Ly, eps_ab=0.5), f)
+    acc_o, _, _, _ = o.pair_pass(True)
+    assert np.array_equal(s.compute_forces(True), acc_o)
+    with pytest.raises(abi.MdxError):
+        s.set_tuning(0.9, 1)
+    with pytest.raises(abi.MdxError):
+        s.set_tuning(1.0, 3)
+    s.close()
