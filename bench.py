@@ -55,7 +55,7 @@ def parse():
                          "in one batch vs per-replica launches; serve: Poisson job stream through the "
                          "continuous-batching server; fire_nph: FIRE from a random start at N=262144 "
                          "then NPH; nve4k: NVE N=4096 with the FP64 oracle timed beside it")
-    ap.add_argument("--tune-steps", type=int, default=20,
+    ap.add_argument("--tune-steps", type=int, default=40,
                     help="steps per autotuner candidate before the timed run (0: default tuning)")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--per-replica-steps", type=int, default=100)
@@ -106,7 +106,8 @@ def run_gpu(frame, Lx, Ly, kw, prec, steps, warmup, tune_steps, timed_steps=200)
     if tune_steps > 0:
         rep = sim.autotune(abi.NVT, tune_steps)
         tune = {"cell_scale": rep.best.cell_scale, "lanes_per_particle": rep.best.lanes_per_particle,
-                "candidates_ms_per_step": [[c, l, round(m, 5)] for c, l, m in rep.table()]}
+                "skin": rep.best.skin,
+                "candidates_ms_per_step": [[c, l, k, round(m, 5)] for c, l, k, m in rep.table()]}
     sim.step(abi.NVT, warmup)
     sim.sync()
     t0 = time.perf_counter()
@@ -121,6 +122,30 @@ def run_gpu(frame, Lx, Ly, kw, prec, steps, warmup, tune_steps, timed_steps=200)
     final = sim.collect()
     sim.close()
     return wall, tm, obs, tune, final
+
+
+def pair_traffic(tune):
+    """HBM bytes per pair-kernel launch at this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same kernel variant
+    (scaled by the copy-kernel calibration), or None when no summary matches."""
+    skin = (tune or {}).get("skin", 0.0)
+    lanes = (tune or {}).get("lanes_per_particle", 1)
+    name = "nvt_1m_list_pair_traffic.md" if skin > 0 else "nvt_1m_cell_path_pair_traffic.md"
+    path = ROOT / "profiles" / name
+    if not path.exists():
+        return None
+    vals = {}
+    for line in path.read_text().splitlines():
+        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "skin", "lanes_per_particle"):
+            k, v = line.split(":", 1)
+            vals[k] = float(v)
+    if "traffic_bytes_per_launch" not in vals:
+        return None
+    if skin > 0 and (vals.get("skin") != skin or vals.get("lanes_per_particle") != lanes):
+        return None
+    if skin == 0 and lanes != 1:
+        return None
+    return vals["traffic_bytes_per_launch"]
 
 
 def accuracy(frame, Lx, Ly, kw):
@@ -245,7 +270,8 @@ def run_ensemble(args):
     if args.tune_steps > 0:
         rep = b.autotune(args.tune_steps)
         tune = {"cell_scale": rep.best.cell_scale, "lanes_per_particle": rep.best.lanes_per_particle,
-                "candidates_ms_per_step": [[c, l, round(m, 5)] for c, l, m in rep.table()]}
+                "skin": rep.best.skin,
+                "candidates_ms_per_step": [[c, l, k, round(m, 5)] for c, l, k, m in rep.table()]}
     b.step(args.warmup)
     b.sync()
     t0 = time.perf_counter()
@@ -262,7 +288,7 @@ def run_ensemble(args):
     sims = [abi.Simulation(abi.make_params(Lx, Ly, precision=prec, **kw), f) for f, Lx, Ly, kw in reps]
     if tune is not None:   # the batch's tuning, so both sides run the same kernels
         for s in sims:
-            s.set_tuning(tune["cell_scale"], tune["lanes_per_particle"])
+            s.set_tuning(tune["cell_scale"], tune["lanes_per_particle"], tune["skin"])
     for s in sims:
         s.step(abi.NVT, 5)
     for s in sims:
@@ -575,7 +601,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": pair_traffic(tune) if (n == 1048576 and prec == abi.PREC_COMP32) else None,
             "bytes_per_atom": PAIR_BYTES_PER_ATOM[prec],
             "avg_launch_ms": pair_ms,
             "launches": tm.force_launches,

@@ -204,16 +204,22 @@ int mdx_timing_read(mdx_sim *sim, mdx_timing *out);
  * cell_scale: cell side >= cell_scale * cutoff * sigma_max (>= 1).
  * lanes_per_particle: 1, 2 or 4 lanes share one particle's neighbour cells
  * (64, 32 or 16 particles per wavefront); the FP64 form always uses 1.
+ * skin: with skin > 0, NVE and NVT slots in the FP32 forms keep a list of the
+ * partners within cutoff + skin and rebuild it (and the cell list, with cells
+ * of at least cutoff + skin) at the step where some particle has moved more
+ * than skin / 2 since the last build; the pairs evaluated are exactly those
+ * within the cutoff, as without the list.  skin = 0: cell list every step.
  * Changing either recomputes the accelerations; FP32 results then follow the
  * new summation order (same tolerance, bit-identical to a fresh run with the
  * same tuning). */
 typedef struct mdx_tuning {
     double cell_scale;
+    double skin;                 /* > 0: neighbour-list mode (FP32 forms, NVE/NVT slots) */
     int32_t lanes_per_particle;
     int32_t pad;
 } mdx_tuning;
 
-#define MDX_TUNE_MAX 16
+#define MDX_TUNE_MAX 32
 typedef struct mdx_tune_report {
     mdx_tuning best;
     double best_ms_per_step;
@@ -226,9 +232,9 @@ typedef struct mdx_tune_report {
 int mdx_set_tuning(mdx_sim *sim, const mdx_tuning *t);
 int mdx_get_tuning(mdx_sim *sim, mdx_tuning *out);
 /* Autotuner: on a copy of the system, times `steps` steps of ensemble `kind`
- * for every candidate (cell_scale 1.0, 1.1, 1.25 x lanes_per_particle 1, 2,
- * 4), then applies the fastest to the handle.  The handle's state is not
- * advanced. */
+ * for every candidate (cell_scale 1.0, 1.1 x lanes_per_particle 1, 2, 4 x skin
+ * 0, 0.3, 0.5; FP64 form: skin 0, one lane), then applies the fastest to the
+ * handle.  The handle's state is not advanced. */
 int mdx_autotune(mdx_sim *sim, int32_t kind, int32_t steps, mdx_tune_report *out);
 
 /* ---- batches of independent systems ---------------------------------------

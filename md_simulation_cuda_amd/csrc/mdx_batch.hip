@@ -106,7 +106,7 @@ Batch::~Batch() {
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
                      d_sp, d_sc, d_blk2sys, d_cblk2sys, d_want, sorted_old, cs_old, movers, mover_off,
-                     n_movers, arrivals};
+                     n_movers, arrivals, key_list, x0, nlist, ncount};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
     for (auto *vec : {&force_ev, &step_ev})
@@ -236,6 +236,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     ptrs.TB = TB;
     ptrs.S = nslots;
     ptrs.C = C;
+    ptrs.N = static_cast<int32_t>(N);
     return MDX_OK;
 }
 
@@ -299,6 +300,7 @@ int Batch::launch_step_kernels(bool obs) {
             k_slot_image<double2><<<TB, kBlock, 0, stream>>>(ptrs);
         else
             k_slot_image<float2><<<TB, kBlock, 0, stream>>>(ptrs);
+        if (nlist && prec != MDX_PREC_FP64) k_list_build<<<TB, kBlock, 0, stream>>>(ptrs);
     }
     if (TB > 0) {
         const bool timed = timing;
@@ -390,6 +392,7 @@ namespace {
 void set_idle(SysParams &sp, const Slot &s) {
     sp.kind = SK_NONE;
     sp.pmode = PM_NONE;
+    sp.list_mode = 0;
     sp.periodic_y = s.cells_py;
     sp.wrap_keys = 1;
     sp.middle_wrap = 0;
@@ -424,7 +427,7 @@ int Batch::load(int r, const mdx_params &p, const mdx_particle64 *frame, int64_t
     s.kind = SK_NONE;
     s.flags = 0;
     sp.n = static_cast<int32_t>(n);
-    sp.cell_side = side;
+    sp.cell_side = cell_side_for(s);
     sp.rc_max = rc_max;
     sp.dt = p.dt;
     sp.T0 = p.T_target;
@@ -647,6 +650,10 @@ int Batch::step(int nsteps) {
         case SK_FIRE: sp.pmode = PM_FIRE; break;
         default: return set_error(MDX_ERR_INVALID, "unknown ensemble");
         }
+        sp.list_mode = (nlist && prec != MDX_PREC_FP64 && (s.kind == SK_NVE || s.kind == SK_NVT)) ? 1 : 0;
+        sp.list_cap = list_cap;
+        sp.skin_half2 = 0.25 * skin * skin;
+        sp.r_list = s.rc_max + skin;
         if (s.kind == SK_NVT && (s.flags & MDX_STEP_MIDDLE_WRAP)) {
             DevScalars h;
             MDX_TRY(read_scalars(static_cast<int>(r), h));
@@ -845,8 +852,31 @@ int Batch::set_precision(int precision) {
 mdx_tuning Batch::tuning() const {
     mdx_tuning t{};
     t.cell_scale = cell_scale;
+    t.skin = skin;
     t.lanes_per_particle = lpp;
     return t;
+}
+
+double Batch::cell_side_for(const Slot &s) const {
+    const double side = std::max(s.p.cell_scale, 1.0) * s.rc_max;
+    return skin > 0.0 ? std::max(side, s.rc_max + skin) : side;
+}
+
+int Batch::ensure_list_arrays() {
+    if (nlist) return MDX_OK;
+    if (N >= (int64_t{1} << 28)) return set_error(MDX_ERR_CAPACITY, "neighbour lists need fewer than 2^28 particles");
+    const size_t n = static_cast<size_t>(std::max<int64_t>(N, 1));
+    MDX_HIP(hipMalloc(&key_list, n * 4));
+    MDX_HIP(hipMalloc(&x0, n * sizeof(double2)));
+    MDX_HIP(hipMalloc(&ncount, n * 4));
+    MDX_HIP(hipMalloc(&nlist, n * static_cast<size_t>(list_cap) * 4));
+    MDX_HIP(hipMemset(ncount, 0, n * 4));
+    ptrs.key_list = key_list;
+    ptrs.x0 = x0;
+    ptrs.ncount = ncount;
+    ptrs.nlist = nlist;
+    invalidate_graphs();   // the step sequence gains the list build
+    return MDX_OK;
 }
 
 int Batch::set_tuning(const mdx_tuning &t) {
@@ -854,6 +884,7 @@ int Batch::set_tuning(const mdx_tuning &t) {
         return set_error(MDX_ERR_INVALID, "cell_scale must be finite and >= 1");
     if (t.lanes_per_particle != 1 && t.lanes_per_particle != 2 && t.lanes_per_particle != 4)
         return set_error(MDX_ERR_INVALID, "lanes_per_particle must be 1, 2 or 4");
+    if (!(t.skin >= 0.0) || !(t.skin <= 4.0)) return set_error(MDX_ERR_INVALID, "skin must be in [0, 4]");
     MDX_HIP(hipSetDevice(device));
     MDX_HIP(hipStreamSynchronize(stream));
     const int new_lpp = prec == MDX_PREC_FP64 ? 1 : t.lanes_per_particle;
@@ -861,20 +892,33 @@ int Batch::set_tuning(const mdx_tuning &t) {
         lpp = new_lpp;
         invalidate_graphs();
     }
-    cell_scale = t.cell_scale;
-    std::vector<int> per, pis;
+    const double new_skin = prec == MDX_PREC_FP64 ? 0.0 : t.skin;
+    if (new_skin > 0.0) MDX_TRY(ensure_list_arrays());
+    // the cell sides must fit the slots before anything changes
     for (size_t r = 0; r < slots.size(); ++r) {
-        Slot &s = slots[r];
+        const Slot &s = slots[r];
         if (!s.loaded) continue;
-        const double side = t.cell_scale * s.rc_max;
+        double side = std::max(t.cell_scale, 1.0) * s.rc_max;
+        if (new_skin > 0.0) side = std::max(side, s.rc_max + new_skin);
         DevScalars h;
         MDX_TRY(read_scalars(static_cast<int>(r), h));
         const Grid g = make_grid(h.g.Lx, h.g.Ly, side, s.rc_max, 1);
         if (static_cast<int64_t>(g.ncx) * g.ncy > s.ccap)
             return set_error(MDX_ERR_CAPACITY, "cell_scale gives more cells than the slot holds");
+    }
+    cell_scale = t.cell_scale;
+    skin = new_skin;
+    std::vector<int> per, pis;
+    for (size_t r = 0; r < slots.size(); ++r) {
+        Slot &s = slots[r];
+        if (!s.loaded) continue;
         s.p.cell_scale = t.cell_scale;
-        h_sp[r].cell_side = side;
+        h_sp[r].cell_side = cell_side_for(s);
         MDX_TRY(upload_sp(static_cast<int>(r)));
+        DevScalars h;   // a list that overflowed gets another chance under the new tuning
+        MDX_TRY(read_scalars(static_cast<int>(r), h));
+        h.list_overflow = 0;
+        MDX_TRY(write_scalar_block(static_cast<int>(r), h));
         (s.piston_active ? pis : per).push_back(static_cast<int>(r));
     }
     // accelerations in the new summation order, as a fresh load would have them
@@ -916,19 +960,23 @@ int Batch::autotune(int steps, mdx_tune_report *rep, int kind_override) {
         any = true;
     }
     if (!any) return set_error(MDX_ERR_STATE, "autotune: no loaded slot");
-    const double scales[3] = {1.0, 1.1, 1.25};
+    const double scales[2] = {1.0, 1.1};
     const int lanes[3] = {1, 2, 4};
+    const double skins[3] = {0.0, 0.3, 0.5};
     double best = 1e300;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MDX_HIP(hipEventCreate(&e0));
     MDX_HIP(hipEventCreate(&e1));
     int rc = MDX_OK;
     for (double cs : scales) {
-        for (int l : lanes) {
-            if (prec == MDX_PREC_FP64 && l != 1) continue;
+        for (int cand = 0; cand < 9; ++cand) {
+            const int l = lanes[cand % 3];
+            const double sk = skins[cand / 3];
+            if (prec == MDX_PREC_FP64 && (l != 1 || sk > 0.0)) continue;
             if (rep->n_tried >= MDX_TUNE_MAX) break;
             mdx_tuning t{};
             t.cell_scale = cs;
+            t.skin = sk;
             t.lanes_per_particle = l;
             rc = c.set_tuning(t);
             if (rc == MDX_ERR_CAPACITY) continue;   // too many cells for a slot: not a candidate

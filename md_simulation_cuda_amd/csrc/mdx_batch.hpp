@@ -69,7 +69,11 @@ struct Batch {
     bool use_graph = true;
     bool graph_uni = true;   // pair-kernel variant the captured graphs hold
     int lpp = 1;             // lanes per particle of the FP32 pair kernel (1, 2, 4)
-    double cell_scale = 1.0; // last cell_scale applied by set_tuning (0: per-slot params)
+    double cell_scale = 1.0; // last cell_scale applied by set_tuning
+    double skin = 0.0;       // neighbour-list skin (0: cell list every step)
+    int list_cap = 64;       // neighbour-list entries per particle
+    uint32_t *key_list = nullptr, *nlist = nullptr, *ncount = nullptr;
+    double2 *x0 = nullptr;
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> force_ev, step_ev;
@@ -127,6 +131,8 @@ struct Batch {
     int obs_pass(int r, int periodic_y);
     void invalidate_graphs();
     bool uniform_sigma() const;   // every loaded slot has one sigma for all type pairs
+    double cell_side_for(const Slot &s) const;   // cell side of a slot under the batch's tuning
+    int ensure_list_arrays();
     int write_scalar_block(int r, const DevScalars &h);
 };
 

@@ -151,6 +151,11 @@ __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P) {
         g = make_grid(Lx, Ly, side, sp.rc_max, sp.periodic_y);
     }
     sc.g = g;
+    // the neighbour list needs cells of at least r_list in a periodic box;
+    // outside list mode every step rebuilds the cell list
+    const bool on = sp.list_mode && !sc.list_overflow && g.periodic_y && g.w >= sp.r_list && g.h >= sp.r_list;
+    sc.list_on = on ? 1 : 0;
+    sc.rebuild = (!on || !sc.list_built) ? 1 : 0;
 }
 
 // Half update of the Nose-Hoover variable, src/md_env.cu:1668-1673:
@@ -288,7 +293,22 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P) {
         const int cx = cell_coord(p.x, g.Lx, g.ncx);
         const int cy = cell_coord(p.y, g.Ly, g.ncy);
         const uint32_t c = static_cast<uint32_t>(sp.coff) + static_cast<uint32_t>(cy) * g.ncx + cx;
-        mover = !sc.list_valid || P.key[i] != c;
+        // movers are relative to the keys the current cell list was built
+        // from: key_list after a list-mode build, else the previous step's key
+        mover = !sc.list_valid || (sc.list_built ? P.key_list[i] : P.key[i]) != c;
+        if (sc.list_on && sc.list_built) {
+            // a particle beyond skin / 2 of its build position makes this step
+            // rebuild the lists
+            {
+                const double2 p0 = P.x0[i];
+                double dx = p.x - p0.x, dy = p.y - p0.y;
+                if (dx > 0.5 * g.Lx) dx -= g.Lx;
+                else if (dx < -0.5 * g.Lx) dx += g.Lx;
+                if (dy > 0.5 * g.Ly) dy -= g.Ly;
+                else if (dy < -0.5 * g.Ly) dy += g.Ly;
+                if (dx * dx + dy * dy > sp.skin_half2) sc.rebuild = 1;   // same value from every writer
+            }
+        }
         P.key[i] = c;
         pid = static_cast<uint32_t>(i);
     }
@@ -382,6 +402,7 @@ __device__ inline void finalize_drift(const BatchPtrs &P, const SysParams &sp, D
 __global__ void __launch_bounds__(kBlock) k_stay_arrive(BatchPtrs P, int ncb) {
     if (static_cast<int>(blockIdx.x) >= ncb) {
         const uint32_t b = blockIdx.x - ncb;
+        if (!P.sc[P.blk2sys[b]].rebuild) return;
         const uint32_t nm = P.n_movers[b];
         if (threadIdx.x < nm) {
             const uint32_t m = b * kBlock + threadIdx.x;
@@ -395,6 +416,9 @@ __global__ void __launch_bounds__(kBlock) k_stay_arrive(BatchPtrs P, int ncb) {
     DevScalars &sc = P.sc[r];
     const int lc = (gb - sp.cboff) * kBlock + threadIdx.x;
     if (gb == sp.cboff) finalize_drift(P, sp, sc);   // the slot's first cell workgroup
+    // a slot that keeps its lists keeps count = cell occupancy (set by
+    // k_cell_sort), so the scan reproduces its cell starts
+    if (!sc.rebuild) return;
     const int n_new = sc.g.ncx * sc.g.ncy;
     const int n_old = sc.list_valid ? sc.g_old.ncx * sc.g_old.ncy : 0;
     // without a previous list (new or reused slot) every cell of the slot is
@@ -417,6 +441,7 @@ __global__ void __launch_bounds__(kBlock) k_stay_arrive(BatchPtrs P, int ncb) {
 __global__ void __launch_bounds__(kBlock) k_fill(BatchPtrs P, int ncb) {
     if (static_cast<int>(blockIdx.x) >= ncb) {
         const uint32_t b = blockIdx.x - ncb;
+        if (!P.sc[P.blk2sys[b]].rebuild) return;
         const uint32_t nm = P.n_movers[b];
         if (threadIdx.x < nm) {
             const uint32_t m = b * kBlock + threadIdx.x;
@@ -430,6 +455,7 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchPtrs P, int ncb) {
     const int r = P.cblk2sys[gb];
     const SysParams &sp = P.sp[r];
     const DevScalars &sc = P.sc[r];
+    if (!sc.rebuild) return;
     const int lc = (gb - sp.cboff) * kBlock + threadIdx.x;
     const int n_new = sc.g.ncx * sc.g.ncy;
     const int n_old = sc.list_valid ? sc.g_old.ncx * sc.g_old.ncy : 0;
@@ -454,12 +480,16 @@ __global__ void __launch_bounds__(kBlock) k_cell_sort(BatchPtrs P) {
     const Grid g = P.sc[r].g;
     const int lc = (gb - sp.cboff) * kBlock + threadIdx.x;
     const int ncell = g.ncx * g.ncy;
-    if (sp.n <= 0 || lc >= ncell) return;
-    if (lc == 0) P.sc[r].list_valid = 1;
+    if (sp.n <= 0 || lc >= ncell || !P.sc[r].rebuild) return;
+    if (lc == 0) {
+        P.sc[r].list_valid = 1;
+        P.sc[r].list_built = 0;   // built from this step's keys; k_list_build marks a list-mode build
+    }
     const int c = sp.coff + lc;
     const uint32_t s0 = P.cs[c], s1 = P.cs[c + 1];
     P.cs_old[c] = s0;
     if (lc == ncell - 1) P.cs_old[c + 1] = s1;
+    P.count[c] = s1 - s0;   // occupancy, for steps that keep this list
     if (P.arrivals[c] == 0) return;
     P.arrivals[c] = 0;
     uint32_t *sorted = P.sorted;
@@ -486,10 +516,14 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
     const SysParams &sp = P.sp[r];
     const int li = (gb - sp.boff) * kBlock + threadIdx.x;
     if (li >= sp.n) return;
-    const Grid g = P.sc[r].g;
+    const DevScalars &sc = P.sc[r];
+    const Grid g = sc.g;
+    const bool reb = sc.rebuild;
     const uint32_t k = P.cs[sp.coff] + static_cast<uint32_t>(li);
     const uint32_t id = P.sorted[k];
-    const uint32_t lc = P.key[id] - static_cast<uint32_t>(sp.coff);
+    // the particle's cell in the list: the current one on a rebuild, else the
+    // one of the last build (the particle may have left it since)
+    const uint32_t lc = reb ? P.key[id] - static_cast<uint32_t>(sp.coff) : P.scell[k];
     double2 p = P.x[id];
     if (sp.wrap_keys) {
         p.x = pbc_wrap(p.x, g.Lx);
@@ -499,15 +533,82 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
     if constexpr (std::is_same_v<IT, float2>) {
         const uint32_t cy = lc / static_cast<uint32_t>(g.ncx);
         const uint32_t cx = lc - cy * static_cast<uint32_t>(g.ncx);
-        const double ox = (static_cast<double>(cx) + 0.5) * g.w;
-        const double oy = (static_cast<double>(cy) + 0.5) * g.h;
-        img[k] = make_float2(static_cast<float>(p.x - ox), static_cast<float>(p.y - oy));
+        double dx = p.x - (static_cast<double>(cx) + 0.5) * g.w;
+        double dy = p.y - (static_cast<double>(cy) + 0.5) * g.h;
+        if (!reb) {   // minimum image: the particle may have crossed the box edge
+            if (dx > 0.5 * g.Lx) dx -= g.Lx;
+            else if (dx < -0.5 * g.Lx) dx += g.Lx;
+            if (dy > 0.5 * g.Ly) dy -= g.Ly;
+            else if (dy < -0.5 * g.Ly) dy += g.Ly;
+        }
+        img[k] = make_float2(static_cast<float>(dx), static_cast<float>(dy));
     } else {
         img[k] = p;
     }
-    P.icls[k] = P.cls[id];
-    P.scell[k] = lc;
-    P.sorted_old[k] = id;
+    if (reb) {
+        P.icls[k] = P.cls[id];
+        P.scell[k] = lc;
+        P.sorted_old[k] = id;
+        if (sc.list_on) {
+            P.x0[id] = p;
+            P.key_list[id] = P.key[id];
+        }
+    }
+}
+
+// Neighbour list of every cell-ordered slot of the slots that rebuild in list
+// mode: the partners within r_list in the 3x3 neighbour cells, in the pair
+// kernel's order (rows -1, 0, +1; cells -1, 0, +1; ascending id).  An entry is
+// the partner's slot index with the neighbour cell's offset code
+// (ox + 1) | (oy + 1) << 2 in the top four bits.
+__global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
+    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int r = P.blk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    DevScalars &sc = P.sc[r];
+    if (!sc.list_on || !sc.rebuild || sp.n <= 0) return;
+    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
+    if (li >= sp.n) return;
+    const Grid g = sc.g;
+    const uint32_t s = P.cs[sp.coff] + static_cast<uint32_t>(li);
+    const uint32_t lc = P.scell[s];
+    const int cx = static_cast<int>(lc % static_cast<uint32_t>(g.ncx));
+    const int cy = static_cast<int>(lc / static_cast<uint32_t>(g.ncx));
+    const float2 *img = static_cast<const float2 *>(P.img);
+    const float2 pi = img[s];
+    const float rl2 = static_cast<float>(sp.r_list * sp.r_list);
+    const uint32_t cap = static_cast<uint32_t>(sp.list_cap);
+    const size_t stride = static_cast<size_t>(P.N);
+    uint32_t cnt = 0;
+    bool over = false;
+    for (int oy = -1; oy <= 1; ++oy) {
+        int yy = cy + oy;
+        if (yy < 0) yy += g.ncy;
+        else if (yy >= g.ncy) yy -= g.ncy;
+        const int rowbase = sp.coff + yy * g.ncx;
+        const float piy = pi.y - static_cast<float>(oy) * g.hf;
+        for (int ox = -1; ox <= 1; ++ox) {
+            int xx = cx + ox;
+            if (xx < 0) xx += g.ncx;
+            else if (xx >= g.ncx) xx -= g.ncx;
+            const float pix = pi.x - static_cast<float>(ox) * g.wf;
+            const uint32_t code = static_cast<uint32_t>((ox + 1) | ((oy + 1) << 2)) << 28;
+            const uint32_t k0 = P.cs[rowbase + xx], k1 = P.cs[rowbase + xx + 1];
+            for (uint32_t k = k0; k < k1; ++k) {
+                if (k == s) continue;
+                const float2 pj = img[k];
+                const float dx = pix - pj.x, dy = piy - pj.y;
+                if (__builtin_fmaf(dx, dx, dy * dy) < rl2) {
+                    if (cnt < cap) P.nlist[cnt * stride + s] = k | code;
+                    else over = true;
+                    ++cnt;
+                }
+            }
+        }
+    }
+    P.ncount[s] = cnt < cap ? cnt : cap;
+    if (over) sc.list_overflow = 1;   // this step's pair pass uses the cells instead
+    if (li == 0) sc.list_built = 1;
 }
 
 // --------------------------------------------------------------------------
@@ -712,6 +813,76 @@ __device__ __forceinline__ void pair_cell_f32(const char *__restrict__ imgb, con
     if constexpr (ACC == ACC_COMP32) F.add(fxc, fyc);
 }
 
+// FP32 pairs of cell-ordered slot s from its neighbour list (list mode):
+// entries in the cell order of the build, each carrying its neighbour cell's
+// offset, so the displacement is formed exactly as in pair_cell_f32.  Pair
+// forces are summed in FP32 over runs of eight entries, each run entering the
+// compensated accumulator once (COMP32); ACC64 adds every pair in FP64.
+template <int ACC, bool OBS, bool UNI, int LPP>
+__device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, float2 pi, int ci, const Grid &g,
+                                              const PairTable &pt, const PairF32 &pc, ForceSum<ACC> &F,
+                                              ObsSum &ob, bool want_obs) {
+    const char *imgb = static_cast<const char *>(P.img);
+    const uint32_t n = P.ncount[s];
+    const size_t stride = static_cast<size_t>(P.N);
+    float fxc = 0.0f, fyc = 0.0f;
+    uint32_t run = 0;
+    for (uint32_t e = (LPP > 1 ? threadIdx.x % LPP : 0u); e < n; e += LPP) {
+        const uint32_t v = P.nlist[e * stride + s];
+        const uint32_t k = v & 0x0FFFFFFFu;
+        const float ox = static_cast<float>(static_cast<int>((v >> 28) & 3u) - 1);
+        const float oy = static_cast<float>(static_cast<int>(v >> 30) - 1);
+        const float2 pj = *reinterpret_cast<const float2 *>(imgb + (k << 3));
+        const int cj = P.icls[k];
+        const float dx = __builtin_fmaf(-ox, g.wf, pi.x) - pj.x;
+        const float dy = __builtin_fmaf(-oy, g.hf, pi.y) - pj.y;
+        const float rr = __builtin_fmaf(dx, dx, dy * dy);
+        const bool same = cj == ci;
+        const float rc2 = UNI ? pc.rc2s : (same ? pc.rc2s : pc.rc2o);
+        if (rr < rc2 && rr > 0.0f) {
+            if (rr < pc.r2_guard) {
+                double fx, fy;
+                pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), pair_index(ci, cj), pt, fx, fy,
+                               ob, want_obs);
+                F.addd(fx, fy);
+            } else {
+                const float sig2 = UNI ? pc.sig2s : (same ? pc.sig2s : pc.sig2o);
+                const float e24 = same ? pc.e24s : pc.e24o;
+                const float r2i = fast_rcp(rr);
+                const float sr2 = sig2 * r2i;
+                const float sr6 = sr2 * sr2 * sr2;
+                const float sr12 = sr6 * sr6;
+                const float tmp = e24 * __builtin_fmaf(2.0f, sr12, -sr6) * r2i;
+                const float fx = tmp * dx, fy = tmp * dy;
+                if constexpr (ACC == ACC_COMP32) {
+                    fxc += fx;
+                    fyc += fy;
+                } else {
+                    F.add(fx, fy);
+                }
+                if constexpr (OBS) {
+                    if (want_obs) {
+                        const float e4 = same ? pc.e4s : pc.e4o;
+                        const float u = e4 * (sr12 - sr6);
+                        const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
+                        ob.u += 0.5 * (static_cast<double>(u) - pt.vc[pair_index(ci, cj)]);
+                        ob.w += 0.5 * (ddx * dfx + ddy * dfy);
+                        ob.lam += 0.5 * (dfx * ddx - dfy * ddy);
+                    }
+                }
+            }
+        }
+        if constexpr (ACC == ACC_COMP32) {
+            if (++run == 8u) {
+                F.add(fxc, fyc);
+                fxc = fyc = 0.0f;
+                run = 0;
+            }
+        }
+    }
+    if constexpr (ACC == ACC_COMP32) F.add(fxc, fyc);
+}
+
 // FP64 pairs of particle pi (wrapped FP64 position) with the candidates
 // [k0, k1) of one neighbour cell shifted by (shx, shy) = +-L across the
 // periodic boundary; the oracle's operation order, so FP64 forces are
@@ -813,7 +984,14 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P) {
             pc.e4o = pt.eps4f[2];
             pc.r2_guard = pt.r2_guard;
         }
-        for (int oy = -1; oy <= 1; ++oy) {
+        bool by_list = false;
+        if constexpr (!kFp64) {
+            if (sc.list_on && !sc.list_overflow) {
+                pair_list_f32<ACC, OBS, UNI, LPP>(P, s, pi, ci, g, pt, pc, F, ob, want_obs);
+                by_list = true;
+            }
+        }
+        for (int oy = -1; oy <= 1 && !by_list; ++oy) {
             int yy = cy + oy;
             double sy = 0.0;
             if (g.periodic_y) {

@@ -62,6 +62,13 @@ struct SysParams {
     // FIRE
     double ftol, f_inc, f_dec, dt_max, alpha0, max_step;
     int32_t N_min, f_max_steps;
+    // neighbour-list mode (FP32 forms, NVE/NVT): each particle lists the
+    // partners within r_list = rc_max + skin; the list (and the cell list it is
+    // built from) is rebuilt at the step where some particle of the slot has
+    // moved more than skin / 2 since the last build
+    int32_t list_mode, list_cap;
+    double skin_half2;   // (skin / 2)^2
+    double r_list;       // rc_max + skin
     PairTable pt;
 };
 
@@ -84,6 +91,11 @@ struct DevScalars {
     double f_dt, f_alpha, f_maxF, f_power;
     int32_t f_streak, f_done, f_zero_v, f_step;
     int32_t f_conv_step, f_drifted, f_do_drift, f_do_pair;
+    // neighbour-list mode
+    int32_t list_on;        // this step uses the neighbour list
+    int32_t rebuild;        // this step rebuilds the cell list (always outside list mode)
+    int32_t list_built;     // the cell list was built from key_list (a list-mode build); nlist and x0 match it
+    int32_t list_overflow;  // a particle had more than list_cap partners: list mode off
 };
 
 // Pointers of a batch, passed by value to every kernel.
@@ -101,6 +113,12 @@ struct BatchPtrs {
     uint32_t *movers, *mover_off;
     uint32_t *n_movers;
     uint32_t *arrivals;              // per cell: movers that arrived this step (zero at rest)
+    // neighbour-list mode: key and position of each particle at the last
+    // build; per cell-ordered slot s, entry e of its list at nlist[e * N + s]
+    // (partner slot | cell offset code << 28) and its length ncount[s]
+    uint32_t *key_list;
+    double2 *x0;
+    uint32_t *nlist, *ncount;
     const SysParams *sp;
     DevScalars *sc;
     const int32_t *blk2sys;   // per-particle workgroup -> slot
@@ -108,7 +126,7 @@ struct BatchPtrs {
     int32_t TB;               // total per-particle workgroups
     int32_t S;                // slots
     int32_t C;                // total cells
-    int32_t pad;
+    int32_t N;                // total particle capacity (stride of nlist)
 };
 
 // Canonical grid of a box (the oracle evaluates the same expressions).
