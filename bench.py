@@ -57,6 +57,8 @@ def parse():
                          "then NPH; nve4k: NVE N=4096 with the FP64 oracle timed beside it")
     ap.add_argument("--tune-steps", type=int, default=40,
                     help="steps per autotuner candidate before the timed run (0: default tuning)")
+    ap.add_argument("--tuning", default=None, metavar="CELL_SCALE,LANES,SKIN[,WINDOW]",
+                    help="use this tuning instead of the autotuner (for profiles)")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--per-replica-steps", type=int, default=100)
     ap.add_argument("--serve-seconds", type=float, default=30.0)
@@ -96,18 +98,23 @@ def in_cutoff_pairs_per_atom(rho):
     return rho * math.pi * 2.5 ** 2
 
 
-def run_gpu(frame, Lx, Ly, kw, prec, steps, warmup, tune_steps, timed_steps=200):
+def run_gpu(frame, Lx, Ly, kw, prec, steps, warmup, tune_steps, fixed_tuning=None, timed_steps=200):
     """Timed run (each step replayed from a captured hipGraph), then a shorter
     eager run with HIP events around every pair-kernel launch."""
     from md_simulation_cuda_amd import abi
 
     sim = abi.Simulation(abi.make_params(Lx, Ly, precision=prec, **kw), frame)
     tune = None
-    if tune_steps > 0:
+    if fixed_tuning is not None:
+        cs, lanes, skin, window = fixed_tuning
+        sim.set_tuning(cs, lanes, skin, window)
+        tune = {"cell_scale": cs, "lanes_per_particle": lanes, "skin": skin, "window": window,
+                "candidates_ms_per_step": None}
+    elif tune_steps > 0:
         rep = sim.autotune(abi.NVT, tune_steps)
         tune = {"cell_scale": rep.best.cell_scale, "lanes_per_particle": rep.best.lanes_per_particle,
-                "skin": rep.best.skin,
-                "candidates_ms_per_step": [[c, l, k, round(m, 5)] for c, l, k, m in rep.table()]}
+                "skin": rep.best.skin, "window": rep.best.window,
+                "candidates_ms_per_step": [[c, l, k, w, round(m, 5)] for c, l, k, w, m in rep.table()]}
     sim.step(abi.NVT, warmup)
     sim.sync()
     t0 = time.perf_counter()
@@ -270,8 +277,8 @@ def run_ensemble(args):
     if args.tune_steps > 0:
         rep = b.autotune(args.tune_steps)
         tune = {"cell_scale": rep.best.cell_scale, "lanes_per_particle": rep.best.lanes_per_particle,
-                "skin": rep.best.skin,
-                "candidates_ms_per_step": [[c, l, k, round(m, 5)] for c, l, k, m in rep.table()]}
+                "skin": rep.best.skin, "window": rep.best.window,
+                "candidates_ms_per_step": [[c, l, k, w, round(m, 5)] for c, l, k, w, m in rep.table()]}
     b.step(args.warmup)
     b.sync()
     t0 = time.perf_counter()
@@ -286,9 +293,9 @@ def run_ensemble(args):
     b.close()
     # per-replica launches: each replica its own handle, stream and step graph
     sims = [abi.Simulation(abi.make_params(Lx, Ly, precision=prec, **kw), f) for f, Lx, Ly, kw in reps]
-    if tune is not None:   # the batch's tuning, so both sides run the same kernels
+    if tune is not None:   # the batch's kernels; no rebuild windows (a host sync per window and handle)
         for s in sims:
-            s.set_tuning(tune["cell_scale"], tune["lanes_per_particle"], tune["skin"])
+            s.set_tuning(tune["cell_scale"], tune["lanes_per_particle"], tune["skin"], 0)
     for s in sims:
         s.step(abi.NVT, 5)
     for s in sims:
@@ -566,12 +573,21 @@ def main():
     prec = {"comp32": abi.PREC_COMP32, "acc64": abi.PREC_ACC64, "fp64": abi.PREC_FP64}[args.precision]
     frame, Lx, Ly, kw = make_workload(args.n)
     n = len(frame)
-    wall, tm, obs, tune, final = run_gpu(frame, Lx, Ly, kw, prec, args.steps, args.warmup, args.tune_steps)
+    fixed = None
+    if args.tuning:
+        parts = args.tuning.split(",") + ["0"]
+        fixed = (float(parts[0]), int(parts[1]), float(parts[2]), int(parts[3]))
+    wall, tm, obs, tune, final = run_gpu(frame, Lx, Ly, kw, prec, args.steps, args.warmup, args.tune_steps, fixed)
     value = n * args.steps / wall
     ms_step = wall * 1e3 / args.steps
     pair_ms = tm.force_ms / max(1, tm.force_launches)
     rho = n / (Lx * Ly)
-    pair_bytes = PAIR_BYTES_PER_ATOM[prec] * n
+    skin = (tune or {}).get("skin", 0.0) or 0.0
+    # list mode: the pair kernel reads the list length (4 B) and one 4-B entry
+    # per partner within cutoff + skin instead of walking cell starts
+    list_len = rho * math.pi * (2.5 + skin) ** 2 if skin > 0 else 0.0
+    bytes_per_atom = PAIR_BYTES_PER_ATOM[prec] + (4.0 + 4.0 * list_len - 1.0 if skin > 0 else 0.0)
+    pair_bytes = bytes_per_atom * n
     achieved = pair_bytes / (pair_ms * 1e-3) / 1e9
     flops = FLOP_PER_PAIR * in_cutoff_pairs_per_atom(rho) * n
     out = {
@@ -602,7 +618,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": pair_traffic(tune) if (n == 1048576 and prec == abi.PREC_COMP32) else None,
-            "bytes_per_atom": PAIR_BYTES_PER_ATOM[prec],
+            "bytes_per_atom": bytes_per_atom,
+            "mean_list_entries": list_len,
             "avg_launch_ms": pair_ms,
             "launches": tm.force_launches,
             "eager_ms_per_step": tm.step_ms / max(1, tm.steps),
@@ -627,7 +644,7 @@ def main():
     }
     if not args.no_plain and prec == abi.PREC_COMP32:
         wall2, tm2, _, tune2, _ = run_gpu(frame, Lx, Ly, kw, abi.PREC_ACC64, args.steps, args.warmup,
-                                          args.tune_steps)
+                                          args.tune_steps, fixed)
         out["plain_fp64_accumulation"] = {
             "value": n * args.steps / wall2,
             "ms_per_step": wall2 * 1e3 / args.steps,

@@ -209,14 +209,20 @@ int mdx_timing_read(mdx_sim *sim, mdx_timing *out);
  * of at least cutoff + skin) at the step where some particle has moved more
  * than skin / 2 since the last build; the pairs evaluated are exactly those
  * within the cutoff, as without the list.  skin = 0: cell list every step.
- * Changing either recomputes the accelerations; FP32 results then follow the
- * new summation order (same tolerance, bit-identical to a fresh run with the
- * same tuning). */
+ * window: with window W > 1 and every stepping slot in list mode, steps run in
+ * windows of W: the first rebuilds the lists, the other W - 1 run only the
+ * drift, image and pair kernels; a slot in which some particle moved beyond
+ * skin / 2 inside the window is put back to the window's start and redone
+ * with on-demand rebuilds.  Rebuild points depend only on W and the slot's own
+ * particles.
+ * Changing the tuning recomputes the accelerations; FP32 results then follow
+ * the new summation order (same tolerance, bit-identical to a fresh run with
+ * the same tuning). */
 typedef struct mdx_tuning {
     double cell_scale;
     double skin;                 /* > 0: neighbour-list mode (FP32 forms, NVE/NVT slots) */
     int32_t lanes_per_particle;
-    int32_t pad;
+    int32_t window;              /* list mode: steps per rebuild window (0: rebuild on demand) */
 } mdx_tuning;
 
 #define MDX_TUNE_MAX 32
@@ -232,9 +238,10 @@ typedef struct mdx_tune_report {
 int mdx_set_tuning(mdx_sim *sim, const mdx_tuning *t);
 int mdx_get_tuning(mdx_sim *sim, mdx_tuning *out);
 /* Autotuner: on a copy of the system, times `steps` steps of ensemble `kind`
- * for every candidate (cell_scale 1.0, 1.1 x lanes_per_particle 1, 2, 4 x skin
- * 0, 0.3, 0.5; FP64 form: skin 0, one lane), then applies the fastest to the
- * handle.  The handle's state is not advanced. */
+ * for every candidate (without lists: cell_scale 1.0, 1.1 x lanes 1, 2, 4;
+ * with lists: skin 0.3, 0.5, 0.8 x lanes 1, 2 x window 0, 16; FP64 form: cell
+ * list, one lane), then applies the fastest to the handle.  The handle's
+ * state is not advanced. */
 int mdx_autotune(mdx_sim *sim, int32_t kind, int32_t steps, mdx_tune_report *out);
 
 /* ---- batches of independent systems ---------------------------------------

@@ -92,7 +92,7 @@ class InitParams(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("cell_scale", C.c_double), ("skin", C.c_double), ("lanes_per_particle", C.c_int32),
-                ("pad", C.c_int32)]
+                ("window", C.c_int32)]
 
 
 TUNE_MAX = 32
@@ -103,9 +103,9 @@ class TuneReport(C.Structure):
                 ("tried", Tuning * TUNE_MAX), ("ms_per_step", C.c_double * TUNE_MAX)]
 
     def table(self):
-        """(cell_scale, lanes_per_particle, skin, ms_per_step) of every candidate."""
+        """(cell_scale, lanes_per_particle, skin, window, ms_per_step) of every candidate."""
         return [(self.tried[i].cell_scale, self.tried[i].lanes_per_particle, self.tried[i].skin,
-                 self.ms_per_step[i]) for i in range(self.n_tried)]
+                 self.tried[i].window, self.ms_per_step[i]) for i in range(self.n_tried)]
 
 
 FIRE = 4   # job kind of the server
@@ -381,8 +381,9 @@ class Simulation:
     def set_graph(self, on: bool) -> None:
         check(self._lib.mdx_set_graph(self._h, 1 if on else 0))
 
-    def set_tuning(self, cell_scale: float = 1.0, lanes_per_particle: int = 1, skin: float = 0.0) -> None:
-        t = Tuning(cell_scale, skin, lanes_per_particle, 0)
+    def set_tuning(self, cell_scale: float = 1.0, lanes_per_particle: int = 1, skin: float = 0.0,
+                   window: int = 0) -> None:
+        t = Tuning(cell_scale, skin, lanes_per_particle, window)
         check(self._lib.mdx_set_tuning(self._h, C.byref(t)))
 
     def get_tuning(self) -> Tuning:
@@ -484,8 +485,9 @@ class Batch:
     def set_graph(self, on: bool) -> None:
         check(self._lib.mdx_batch_set_graph(self._h, 1 if on else 0))
 
-    def set_tuning(self, cell_scale: float = 1.0, lanes_per_particle: int = 1, skin: float = 0.0) -> None:
-        t = Tuning(cell_scale, skin, lanes_per_particle, 0)
+    def set_tuning(self, cell_scale: float = 1.0, lanes_per_particle: int = 1, skin: float = 0.0,
+                   window: int = 0) -> None:
+        t = Tuning(cell_scale, skin, lanes_per_particle, window)
         check(self._lib.mdx_batch_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, steps: int = 20) -> TuneReport:

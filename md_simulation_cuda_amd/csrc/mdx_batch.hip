@@ -106,7 +106,7 @@ Batch::~Batch() {
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
                      d_sp, d_sc, d_blk2sys, d_cblk2sys, d_want, sorted_old, cs_old, movers, mover_off,
-                     n_movers, arrivals, key_list, x0, nlist, ncount};
+                     n_movers, arrivals, key_list, x0, nlist, ncount, x_bak, v_bak, a_bak, sc_bak};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
     for (auto *vec : {&force_ev, &step_ev})
@@ -118,11 +118,12 @@ Batch::~Batch() {
 }
 
 void Batch::invalidate_graphs() {
-    for (auto &g : graph)
-        if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
+    for (auto &row : graph)
+        for (auto &g : row)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
 }
 
 int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int precision, int device_ordinal) {
@@ -267,6 +268,17 @@ int Batch::sync() {
     return MDX_OK;
 }
 
+namespace {
+void set_idle(SysParams &sp, const Slot &s) {
+    sp.kind = SK_NONE;
+    sp.pmode = PM_NONE;
+    sp.list_mode = 0;
+    sp.periodic_y = s.cells_py;
+    sp.wrap_keys = 1;
+    sp.middle_wrap = 0;
+}
+}  // namespace
+
 // FP32 pair kernel with lpp lanes per particle (1, 2 or 4).
 template <int ACC, bool OBS, bool UNI>
 static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &p) {
@@ -278,29 +290,32 @@ static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &
 }
 
 // Launch one step of every slot (their kind/pmode in d_sp).
-int Batch::launch_step_kernels(bool obs) {
+int Batch::launch_step_kernels(bool obs, int mode) {
     const int S = static_cast<int>(slots.size());
-    k_box<<<S, kBlock, 0, stream>>>(ptrs);
+    const bool lean = mode == STEP_LEAN;
+    k_box<<<S, kBlock, 0, stream>>>(ptrs, mode);
     if (TB > 0) {
         if (prec == MDX_PREC_COMP32)
-            k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs);
+            k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs, lean ? 1 : 0);
         else
-            k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs);
+            k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs, lean ? 1 : 0);
     }
-    if (TCB + TB > 0) k_stay_arrive<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
-    const int nchunk = ceil_div(C, kScanChunk);
-    if (nchunk > 0) {
-        k_scan_a<<<nchunk, kBlock, 0, stream>>>(count, arrivals, C, bsum);
-        k_scan_b<<<nchunk, kBlock, 0, stream>>>(count, arrivals, C, bsum, cs);
+    if (!lean) {   // cell list (and, in list mode, neighbour lists) of the slots that rebuild
+        if (TCB + TB > 0) k_stay_arrive<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
+        const int nchunk = ceil_div(C, kScanChunk);
+        if (nchunk > 0) {
+            k_scan_a<<<nchunk, kBlock, 0, stream>>>(count, arrivals, C, bsum);
+            k_scan_b<<<nchunk, kBlock, 0, stream>>>(count, arrivals, C, bsum, cs);
+        }
+        if (TCB + TB > 0) k_fill<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
+        if (TCB > 0) k_cell_sort<<<TCB, kBlock, 0, stream>>>(ptrs);
     }
-    if (TCB + TB > 0) k_fill<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
-    if (TCB > 0) k_cell_sort<<<TCB, kBlock, 0, stream>>>(ptrs);
     if (TB > 0) {
         if (prec == MDX_PREC_FP64)
             k_slot_image<double2><<<TB, kBlock, 0, stream>>>(ptrs);
         else
             k_slot_image<float2><<<TB, kBlock, 0, stream>>>(ptrs);
-        if (nlist && prec != MDX_PREC_FP64) k_list_build<<<TB, kBlock, 0, stream>>>(ptrs);
+        if (!lean && nlist && prec != MDX_PREC_FP64) k_list_build<<<TB, kBlock, 0, stream>>>(ptrs);
     }
     if (TB > 0) {
         const bool timed = timing;
@@ -349,21 +364,16 @@ bool Batch::uniform_sigma() const {
     return true;
 }
 
-int Batch::run_passes(int count_, bool obs) {
+// count steps of one mode, replayed from a captured hipGraph unless kernel
+// timing is on.
+int Batch::replay(bool obs, int mode, int count_) {
     if (count_ <= 0) return MDX_OK;
-    if (graph_uni != uniform_sigma()) {   // captured graphs hold the other pair-kernel variant
-        invalidate_graphs();
-        graph_uni = uniform_sigma();
-    }
-    hipEvent_t s0 = nullptr, s1 = nullptr;
-    if (timing && hipEventCreate(&s0) == hipSuccess && hipEventCreate(&s1) == hipSuccess)
-        (void)hipEventRecord(s0, stream);
     if (use_graph && !timing) {
-        hipGraphExec_t &ge = graph[obs ? 1 : 0];
+        hipGraphExec_t &ge = graph[obs ? 1 : 0][mode];
         if (!ge) {
             hipGraph_t g = nullptr;
             MDX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            const int rc = launch_step_kernels(obs);
+            const int rc = launch_step_kernels(obs, mode);
             hipError_t e = hipStreamEndCapture(stream, &g);
             if (rc != MDX_OK) {
                 if (g) (void)hipGraphDestroy(g);
@@ -376,11 +386,42 @@ int Batch::run_passes(int count_, bool obs) {
         }
         for (int k = 0; k < count_; ++k) MDX_HIP(hipGraphLaunch(ge, stream));
     } else {
-        for (int k = 0; k < count_; ++k) MDX_TRY(launch_step_kernels(obs));
+        for (int k = 0; k < count_; ++k) MDX_TRY(launch_step_kernels(obs, mode));
     }
-    // the last pass's pair reduction (earlier ones are combined by the next k_box)
-    k_finalize<<<static_cast<int>(slots.size()), kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
-    MDX_HIP(hipGetLastError());
+    return MDX_OK;
+}
+
+// Windows apply when the batch is tuned for them and every stepping slot is a
+// list-mode slot (NVE / NVT in an FP32 form).
+bool Batch::windows_apply() const {
+    if (window < 2 || !nlist || prec == MDX_PREC_FP64) return false;
+    bool any = false;
+    for (size_t r = 0; r < slots.size(); ++r) {
+        if (!slots[r].loaded || h_sp[r].kind == SK_NONE) continue;
+        if (!h_sp[r].list_mode || h_sp[r].middle_wrap) return false;
+        any = true;
+    }
+    return any;
+}
+
+// count passes of the step sequence; a run ends with the combination of the
+// last pair pass's reduction (earlier ones are combined by the next k_box).
+int Batch::run_passes(int count_, bool obs) {
+    if (count_ <= 0) return MDX_OK;
+    if (graph_uni != uniform_sigma()) {   // captured graphs hold the other pair-kernel variant
+        invalidate_graphs();
+        graph_uni = uniform_sigma();
+    }
+    hipEvent_t s0 = nullptr, s1 = nullptr;
+    if (timing && hipEventCreate(&s0) == hipSuccess && hipEventCreate(&s1) == hipSuccess)
+        (void)hipEventRecord(s0, stream);
+    if (!obs && windows_apply()) {
+        MDX_TRY(run_windows(count_));
+    } else {
+        MDX_TRY(replay(obs, STEP_NORMAL, count_));
+        k_finalize<<<static_cast<int>(slots.size()), kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
+        MDX_HIP(hipGetLastError());
+    }
     if (s1) {
         (void)hipEventRecord(s1, stream);
         step_ev.emplace_back(s0, s1);
@@ -388,16 +429,77 @@ int Batch::run_passes(int count_, bool obs) {
     return MDX_OK;
 }
 
-namespace {
-void set_idle(SysParams &sp, const Slot &s) {
-    sp.kind = SK_NONE;
-    sp.pmode = PM_NONE;
-    sp.list_mode = 0;
-    sp.periodic_y = s.cells_py;
-    sp.wrap_keys = 1;
-    sp.middle_wrap = 0;
+// Rebuild windows (list mode): snapshot, one rebuilding step, window - 1 lean
+// steps, then the slots whose lean steps were not covered by their lists
+// (a particle beyond skin / 2, or no usable list) go back to the snapshot and
+// redo the window on the normal path, alone.
+int Batch::run_windows(int count_) {
+    const int S = static_cast<int>(slots.size());
+    const size_t n = static_cast<size_t>(N);
+    const size_t asz = prec == MDX_PREC_COMP32 ? sizeof(float2) : sizeof(double2);
+    if (!x_bak) {
+        MDX_HIP(hipMalloc(&x_bak, n * sizeof(double2)));
+        MDX_HIP(hipMalloc(&v_bak, n * sizeof(double2)));
+        MDX_HIP(hipMalloc(&a_bak, n * sizeof(double2)));
+        MDX_HIP(hipMalloc(&sc_bak, static_cast<size_t>(S) * sizeof(DevScalars)));
+    }
+    std::vector<DevScalars> cur(S), bak(S);
+    std::vector<SysParams> keep = h_sp;
+    for (int left = count_; left > 0;) {
+        const int w = std::min(window, left);
+        MDX_HIP(hipMemcpyAsync(x_bak, x, n * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+        MDX_HIP(hipMemcpyAsync(v_bak, v, n * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+        MDX_HIP(hipMemcpyAsync(a_bak, a, n * asz, hipMemcpyDeviceToDevice, stream));
+        MDX_HIP(hipMemcpyAsync(sc_bak, d_sc, S * sizeof(DevScalars), hipMemcpyDeviceToDevice, stream));
+        MDX_TRY(replay(false, STEP_REBUILD, 1));
+        MDX_TRY(replay(false, STEP_LEAN, w - 1));
+        k_finalize<<<S, kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
+        MDX_HIP(hipGetLastError());
+        MDX_HIP(hipMemcpyAsync(cur.data(), d_sc, S * sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+        MDX_HIP(hipStreamSynchronize(stream));
+        std::vector<int> redo;
+        for (int r = 0; r < S; ++r)
+            if (slots[r].loaded && keep[r].kind != SK_NONE && cur[r].violated) redo.push_back(r);
+        if (!redo.empty()) {
+            MDX_HIP(hipMemcpyAsync(bak.data(), sc_bak, S * sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+            MDX_HIP(hipStreamSynchronize(stream));
+            for (int r = 0; r < S; ++r) set_idle(h_sp[r], slots[r]);
+            for (int r : redo) {
+                const size_t off = static_cast<size_t>(h_sp[r].off), cnt = static_cast<size_t>(h_sp[r].n);
+                MDX_HIP(hipMemcpyAsync(x + off, x_bak + off, cnt * sizeof(double2), hipMemcpyDeviceToDevice,
+                                       stream));
+                MDX_HIP(hipMemcpyAsync(v + off, v_bak + off, cnt * sizeof(double2), hipMemcpyDeviceToDevice,
+                                       stream));
+                MDX_HIP(hipMemcpyAsync(static_cast<char *>(a) + off * asz, static_cast<char *>(a_bak) + off * asz,
+                                       cnt * asz, hipMemcpyDeviceToDevice, stream));
+                // the physics of the window start, the cell-list state of now
+                DevScalars h = bak[r];
+                h.g = cur[r].g;
+                h.g_old = cur[r].g_old;
+                h.list_valid = cur[r].list_valid;
+                h.list_built = cur[r].list_built;
+                h.list_overflow = cur[r].list_overflow;
+                h.pair_pending = 0;
+                h.violated = 0;
+                MDX_TRY(write_scalar_block(r, h));
+                h_sp[r] = keep[r];
+            }
+            MDX_TRY(upload_all_sp());
+            MDX_TRY(replay(false, STEP_NORMAL, w));
+            k_finalize<<<S, kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
+            MDX_HIP(hipGetLastError());
+            h_sp = keep;
+            MDX_TRY(upload_all_sp());
+        }
+        if (std::any_of(cur.begin(), cur.end(), [](const DevScalars &q) { return q.violated != 0; })) {
+            k_clear_violations<<<ceil_div(S, kBlock), kBlock, 0, stream>>>(ptrs);
+            MDX_HIP(hipGetLastError());
+        }
+        left -= w;
+    }
+    return MDX_OK;
 }
-}  // namespace
+
 
 int Batch::load(int r, const mdx_params &p, const mdx_particle64 *frame, int64_t n) {
     if (r < 0 || r >= static_cast<int>(slots.size())) return set_error(MDX_ERR_INVALID, "slot out of range");
@@ -854,6 +956,7 @@ mdx_tuning Batch::tuning() const {
     t.cell_scale = cell_scale;
     t.skin = skin;
     t.lanes_per_particle = lpp;
+    t.window = window;
     return t;
 }
 
@@ -885,6 +988,7 @@ int Batch::set_tuning(const mdx_tuning &t) {
     if (t.lanes_per_particle != 1 && t.lanes_per_particle != 2 && t.lanes_per_particle != 4)
         return set_error(MDX_ERR_INVALID, "lanes_per_particle must be 1, 2 or 4");
     if (!(t.skin >= 0.0) || !(t.skin <= 4.0)) return set_error(MDX_ERR_INVALID, "skin must be in [0, 4]");
+    if (t.window < 0 || t.window > 4096) return set_error(MDX_ERR_INVALID, "window must be in [0, 4096]");
     MDX_HIP(hipSetDevice(device));
     MDX_HIP(hipStreamSynchronize(stream));
     const int new_lpp = prec == MDX_PREC_FP64 ? 1 : t.lanes_per_particle;
@@ -908,6 +1012,7 @@ int Batch::set_tuning(const mdx_tuning &t) {
     }
     cell_scale = t.cell_scale;
     skin = new_skin;
+    window = new_skin > 0.0 ? t.window : 0;
     std::vector<int> per, pis;
     for (size_t r = 0; r < slots.size(); ++r) {
         Slot &s = slots[r];
@@ -960,48 +1065,45 @@ int Batch::autotune(int steps, mdx_tune_report *rep, int kind_override) {
         any = true;
     }
     if (!any) return set_error(MDX_ERR_STATE, "autotune: no loaded slot");
-    const double scales[2] = {1.0, 1.1};
-    const int lanes[3] = {1, 2, 4};
-    const double skins[3] = {0.0, 0.3, 0.5};
+    // candidates: cell list every step (cell size x lanes), or neighbour lists
+    // (skin x lanes x rebuild window)
+    std::vector<mdx_tuning> cands;
+    for (double cs : {1.0, 1.1})
+        for (int l : {1, 2, 4}) cands.push_back(mdx_tuning{cs, 0.0, l, 0});
+    if (prec != MDX_PREC_FP64)
+        for (double sk : {0.3, 0.5, 0.8})
+            for (int l : {1, 2})
+                for (int w : {0, 16}) cands.push_back(mdx_tuning{1.0, sk, l, w});
     double best = 1e300;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MDX_HIP(hipEventCreate(&e0));
     MDX_HIP(hipEventCreate(&e1));
     int rc = MDX_OK;
-    for (double cs : scales) {
-        for (int cand = 0; cand < 9; ++cand) {
-            const int l = lanes[cand % 3];
-            const double sk = skins[cand / 3];
-            if (prec == MDX_PREC_FP64 && (l != 1 || sk > 0.0)) continue;
-            if (rep->n_tried >= MDX_TUNE_MAX) break;
-            mdx_tuning t{};
-            t.cell_scale = cs;
-            t.skin = sk;
-            t.lanes_per_particle = l;
-            rc = c.set_tuning(t);
-            if (rc == MDX_ERR_CAPACITY) continue;   // too many cells for a slot: not a candidate
-            if (rc != MDX_OK) break;
-            if ((rc = c.step(3)) != MDX_OK) break;   // warm-up: graph capture, caches
-            (void)hipEventRecord(e0, c.stream);
-            if ((rc = c.step(steps)) != MDX_OK) break;
-            (void)hipEventRecord(e1, c.stream);
-            if (hipEventSynchronize(e1) != hipSuccess) {
-                rc = set_error(MDX_ERR_HIP, "autotune: event wait failed");
-                break;
-            }
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            const double per = static_cast<double>(ms) / steps;
-            rep->tried[rep->n_tried] = t;
-            rep->ms_per_step[rep->n_tried] = per;
-            ++rep->n_tried;
-            if (per < best) {
-                best = per;
-                rep->best = t;
-                rep->best_ms_per_step = per;
-            }
+    for (const mdx_tuning &t : cands) {
+        if (prec == MDX_PREC_FP64 && t.lanes_per_particle != 1) continue;
+        if (rep->n_tried >= MDX_TUNE_MAX) break;
+        rc = c.set_tuning(t);
+        if (rc == MDX_ERR_CAPACITY) continue;   // too many cells for a slot: not a candidate
+        if (rc != MDX_OK) break;
+        if ((rc = c.step(3)) != MDX_OK) break;   // warm-up: graph capture, caches
+        (void)hipEventRecord(e0, c.stream);
+        if ((rc = c.step(steps)) != MDX_OK) break;
+        (void)hipEventRecord(e1, c.stream);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = set_error(MDX_ERR_HIP, "autotune: event wait failed");
+            break;
         }
-        if (rc != MDX_OK && rc != MDX_ERR_CAPACITY) break;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double per = static_cast<double>(ms) / steps;
+        rep->tried[rep->n_tried] = t;
+        rep->ms_per_step[rep->n_tried] = per;
+        ++rep->n_tried;
+        if (per < best) {
+            best = per;
+            rep->best = t;
+            rep->best_ms_per_step = per;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -65,13 +65,17 @@ struct Batch {
     int32_t *d_blk2sys = nullptr, *d_cblk2sys = nullptr, *d_want = nullptr;
     BatchPtrs ptrs{};
     // captured step graphs, by [obs]
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};   // [obs][StepMode]
     bool use_graph = true;
     bool graph_uni = true;   // pair-kernel variant the captured graphs hold
     int lpp = 1;             // lanes per particle of the FP32 pair kernel (1, 2, 4)
     double cell_scale = 1.0; // last cell_scale applied by set_tuning
     double skin = 0.0;       // neighbour-list skin (0: cell list every step)
     int list_cap = 64;       // neighbour-list entries per particle
+    int window = 0;          // list mode: steps per rebuild window
+    double2 *x_bak = nullptr, *v_bak = nullptr;   // window start state
+    void *a_bak = nullptr;
+    DevScalars *sc_bak = nullptr;
     uint32_t *key_list = nullptr, *nlist = nullptr, *ncount = nullptr;
     double2 *x0 = nullptr;
     // timing
@@ -122,8 +126,11 @@ struct Batch {
     // internals
     int upload_sp(int r);
     int upload_all_sp();
-    int launch_step_kernels(bool obs);
+    int launch_step_kernels(bool obs, int mode);
     int run_passes(int count, bool obs);
+    int replay(bool obs, int mode, int count);
+    int run_windows(int count);
+    bool windows_apply() const;
     int pre_step();
     int rebuild_cells(const std::vector<int> &which, int periodic_y);
     int kinetic(const std::vector<int> &which);

@@ -17,10 +17,12 @@
 // One step = k_box (one workgroup per slot: the previous pair pass's
 // reduction, then box, barostat, piston, grid, FIRE bookkeeping) -> k_drift
 // (kick/drift/wrap, cell key, the ensemble's reduction partials, movers) ->
-// k_stay_arrive (stayers per cell, the drift reduction, movers' arrival
-// index) -> k_scan_a/k_scan_b -> k_fill -> k_cell_sort (id order of the cells
-// that received movers) -> k_slot_image (per-slot image, coalesced) -> k_pair
-// (forces, second kick, the ensemble's reduction partials).  A run of steps
+// k_stay_arrive (stayers per cell, movers' arrival index) ->
+// k_scan_a/k_scan_b -> k_fill -> k_cell_sort (id order of the cells that
+// received movers) -> k_slot_image (the drift reduction; per-slot image,
+// coalesced) -> k_list_build (neighbour lists, list mode) -> k_pair (forces,
+// second kick, the ensemble's reduction partials).  A lean step (list mode,
+// no rebuild) is k_box -> k_drift -> k_slot_image -> k_pair.  A run of steps
 // ends with k_finalize, which combines the last pair pass's partials.  The
 // cell list is updated from the previous one: only particles whose cell
 // changed ("movers") use atomics.  Global FP64 scalars are combined from the
@@ -69,7 +71,13 @@ __device__ inline void finalize_pair_pending(const BatchPtrs &P, const SysParams
 // NPH: area barostat of src/md_env.cu:1824-1877.  Piston:
 // src/md_env.cu:1950-1980.  FIRE: decides whether the pending drift and the
 // next force pass run.  Every slot: the cell grid of its (possibly new) box.
-__global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P) {
+enum StepMode : int32_t {
+    STEP_NORMAL = 0,    // list slots rebuild when a particle moved beyond skin / 2
+    STEP_REBUILD = 1,   // list slots rebuild their cell and neighbour lists
+    STEP_LEAN = 2,      // list slots keep their lists; no cell-list kernels run
+};
+
+__global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
     const int r = blockIdx.x;
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
@@ -155,7 +163,14 @@ __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P) {
     // outside list mode every step rebuilds the cell list
     const bool on = sp.list_mode && !sc.list_overflow && g.periodic_y && g.w >= sp.r_list && g.h >= sp.r_list;
     sc.list_on = on ? 1 : 0;
-    sc.rebuild = (!on || !sc.list_built) ? 1 : 0;
+    if (mode == STEP_LEAN) {
+        // no cell-list kernels run: a stepping slot without a usable list has
+        // to redo the window on the normal path
+        sc.rebuild = 0;
+        if (sp.kind != SK_NONE && (!on || !sc.list_built)) sc.violated = 1;
+    } else {
+        sc.rebuild = (!on || !sc.list_built || mode == STEP_REBUILD) ? 1 : 0;
+    }
 }
 
 // Half update of the Nose-Hoover variable, src/md_env.cu:1668-1673:
@@ -199,7 +214,7 @@ __device__ inline void append_movers(const BatchPtrs &P, bool mover, uint32_t pi
 // that do not move this step (idle, converged FIRE) still recompute keys so
 // the cell list is rebuilt unchanged.
 template <typename AT>
-__global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P) {
+__global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t lean) {
     const int gb = xcd_block(blockIdx.x, gridDim.x);
     const int r = P.blk2sys[gb];
     const SysParams &sp = P.sp[r];
@@ -306,7 +321,10 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P) {
                 else if (dx < -0.5 * g.Lx) dx += g.Lx;
                 if (dy > 0.5 * g.Ly) dy -= g.Ly;
                 else if (dy < -0.5 * g.Ly) dy += g.Ly;
-                if (dx * dx + dy * dy > sp.skin_half2) sc.rebuild = 1;   // same value from every writer
+                if (dx * dx + dy * dy > sp.skin_half2) {   // same value from every writer
+                    if (lean) sc.violated = 1;
+                    else sc.rebuild = 1;
+                }
             }
         }
         P.key[i] = c;
@@ -395,8 +413,7 @@ __device__ inline void finalize_drift(const BatchPtrs &P, const SysParams &sp, D
 
 // Workgroups [0, ncb): one thread per cell of the new grid (and of the
 // previous grid when it was larger): count[c] = particles of the previous list
-// of cell c that stay in c; cells beyond the new grid get zero.  The slot's
-// first cell workgroup also combines the drift reduction.  Workgroups
+// of cell c that stay in c; cells beyond the new grid get zero.  Workgroups
 // [ncb, ncb + TB): drift workgroup b's movers take an arrival index in their
 // new cell (atomics on distinct cells).  The two parts touch different arrays.
 __global__ void __launch_bounds__(kBlock) k_stay_arrive(BatchPtrs P, int ncb) {
@@ -415,7 +432,6 @@ __global__ void __launch_bounds__(kBlock) k_stay_arrive(BatchPtrs P, int ncb) {
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
     const int lc = (gb - sp.cboff) * kBlock + threadIdx.x;
-    if (gb == sp.cboff) finalize_drift(P, sp, sc);   // the slot's first cell workgroup
     // a slot that keeps its lists keeps count = cell occupancy (set by
     // k_cell_sort), so the scan reproduces its cell starts
     if (!sc.rebuild) return;
@@ -514,9 +530,10 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
     const int gb = xcd_block(blockIdx.x, gridDim.x);
     const int r = P.blk2sys[gb];
     const SysParams &sp = P.sp[r];
+    DevScalars &sc = P.sc[r];
+    if (gb == sp.boff) finalize_drift(P, sp, sc);   // the slot's first workgroup: K_half, xi_mid
     const int li = (gb - sp.boff) * kBlock + threadIdx.x;
     if (li >= sp.n) return;
-    const DevScalars &sc = P.sc[r];
     const Grid g = sc.g;
     const bool reb = sc.rebuild;
     const uint32_t k = P.cs[sp.coff] + static_cast<uint32_t>(li);
@@ -818,65 +835,83 @@ __device__ __forceinline__ void pair_cell_f32(const char *__restrict__ imgb, con
 // offset, so the displacement is formed exactly as in pair_cell_f32.  Pair
 // forces are summed in FP32 over runs of eight entries, each run entering the
 // compensated accumulator once (COMP32); ACC64 adds every pair in FP64.
+// Entries are taken four at a time so their loads are in flight together;
+// slots past the end of the list stand for the particle itself (excluded by
+// r^2 > 0).
 template <int ACC, bool OBS, bool UNI, int LPP>
 __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, float2 pi, int ci, const Grid &g,
                                               const PairTable &pt, const PairF32 &pc, ForceSum<ACC> &F,
                                               ObsSum &ob, bool want_obs) {
+    constexpr int U = 4;
+    constexpr uint32_t kSelf = 5u << 28;   // offset code of the particle's own cell
     const char *imgb = static_cast<const char *>(P.img);
     const uint32_t n = P.ncount[s];
     const size_t stride = static_cast<size_t>(P.N);
     float fxc = 0.0f, fyc = 0.0f;
     uint32_t run = 0;
-    for (uint32_t e = (LPP > 1 ? threadIdx.x % LPP : 0u); e < n; e += LPP) {
-        const uint32_t v = P.nlist[e * stride + s];
-        const uint32_t k = v & 0x0FFFFFFFu;
-        const float ox = static_cast<float>(static_cast<int>((v >> 28) & 3u) - 1);
-        const float oy = static_cast<float>(static_cast<int>(v >> 30) - 1);
-        const float2 pj = *reinterpret_cast<const float2 *>(imgb + (k << 3));
-        const int cj = P.icls[k];
-        const float dx = __builtin_fmaf(-ox, g.wf, pi.x) - pj.x;
-        const float dy = __builtin_fmaf(-oy, g.hf, pi.y) - pj.y;
-        const float rr = __builtin_fmaf(dx, dx, dy * dy);
-        const bool same = cj == ci;
-        const float rc2 = UNI ? pc.rc2s : (same ? pc.rc2s : pc.rc2o);
-        if (rr < rc2 && rr > 0.0f) {
-            if (rr < pc.r2_guard) {
-                double fx, fy;
-                pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), pair_index(ci, cj), pt, fx, fy,
-                               ob, want_obs);
-                F.addd(fx, fy);
-            } else {
-                const float sig2 = UNI ? pc.sig2s : (same ? pc.sig2s : pc.sig2o);
-                const float e24 = same ? pc.e24s : pc.e24o;
-                const float r2i = fast_rcp(rr);
-                const float sr2 = sig2 * r2i;
-                const float sr6 = sr2 * sr2 * sr2;
-                const float sr12 = sr6 * sr6;
-                const float tmp = e24 * __builtin_fmaf(2.0f, sr12, -sr6) * r2i;
-                const float fx = tmp * dx, fy = tmp * dy;
-                if constexpr (ACC == ACC_COMP32) {
-                    fxc += fx;
-                    fyc += fy;
+    for (uint32_t e = (LPP > 1 ? threadIdx.x % LPP : 0u); e < n; e += U * LPP) {
+        uint32_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t ee = e + u * LPP;
+            v[u] = ee < n ? P.nlist[ee * stride + s] : (s | kSelf);
+        }
+        float2 pj[U];
+        int cj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t k = v[u] & 0x0FFFFFFFu;
+            pj[u] = *reinterpret_cast<const float2 *>(imgb + (k << 3));
+            cj[u] = P.icls[k];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float ox = static_cast<float>(static_cast<int>((v[u] >> 28) & 3u) - 1);
+            const float oy = static_cast<float>(static_cast<int>(v[u] >> 30) - 1);
+            const float dx = __builtin_fmaf(-ox, g.wf, pi.x) - pj[u].x;
+            const float dy = __builtin_fmaf(-oy, g.hf, pi.y) - pj[u].y;
+            const float rr = __builtin_fmaf(dx, dx, dy * dy);
+            const bool same = cj[u] == ci;
+            const float rc2 = UNI ? pc.rc2s : (same ? pc.rc2s : pc.rc2o);
+            if (rr < rc2 && rr > 0.0f) {
+                if (rr < pc.r2_guard) {
+                    double fx, fy;
+                    pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), pair_index(ci, cj[u]), pt, fx,
+                                   fy, ob, want_obs);
+                    F.addd(fx, fy);
                 } else {
-                    F.add(fx, fy);
-                }
-                if constexpr (OBS) {
-                    if (want_obs) {
-                        const float e4 = same ? pc.e4s : pc.e4o;
-                        const float u = e4 * (sr12 - sr6);
-                        const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
-                        ob.u += 0.5 * (static_cast<double>(u) - pt.vc[pair_index(ci, cj)]);
-                        ob.w += 0.5 * (ddx * dfx + ddy * dfy);
-                        ob.lam += 0.5 * (dfx * ddx - dfy * ddy);
+                    const float sig2 = UNI ? pc.sig2s : (same ? pc.sig2s : pc.sig2o);
+                    const float e24 = same ? pc.e24s : pc.e24o;
+                    const float r2i = fast_rcp(rr);
+                    const float sr2 = sig2 * r2i;
+                    const float sr6 = sr2 * sr2 * sr2;
+                    const float sr12 = sr6 * sr6;
+                    const float tmp = e24 * __builtin_fmaf(2.0f, sr12, -sr6) * r2i;
+                    const float fx = tmp * dx, fy = tmp * dy;
+                    if constexpr (ACC == ACC_COMP32) {
+                        fxc += fx;
+                        fyc += fy;
+                    } else {
+                        F.add(fx, fy);
+                    }
+                    if constexpr (OBS) {
+                        if (want_obs) {
+                            const float e4 = same ? pc.e4s : pc.e4o;
+                            const float uu = e4 * (sr12 - sr6);
+                            const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
+                            ob.u += 0.5 * (static_cast<double>(uu) - pt.vc[pair_index(ci, cj[u])]);
+                            ob.w += 0.5 * (ddx * dfx + ddy * dfy);
+                            ob.lam += 0.5 * (dfx * ddx - dfy * ddy);
+                        }
                     }
                 }
             }
-        }
-        if constexpr (ACC == ACC_COMP32) {
-            if (++run == 8u) {
-                F.add(fxc, fyc);
-                fxc = fyc = 0.0f;
-                run = 0;
+            if constexpr (ACC == ACC_COMP32) {
+                if (++run == 8u) {
+                    F.add(fxc, fyc);
+                    fxc = fyc = 0.0f;
+                    run = 0;
+                }
             }
         }
     }
@@ -1115,6 +1150,11 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P) {
     }
 }
 
+
+__global__ void k_clear_violations(BatchPtrs P) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < P.S) P.sc[r].violated = 0;
+}
 
 // --------------------------------------------------------------------------
 // per-slot finalisation of the reductions

@@ -96,6 +96,8 @@ struct DevScalars {
     int32_t rebuild;        // this step rebuilds the cell list (always outside list mode)
     int32_t list_built;     // the cell list was built from key_list (a list-mode build); nlist and x0 match it
     int32_t list_overflow;  // a particle had more than list_cap partners: list mode off
+    int32_t violated;       // a lean step ran with a particle beyond skin / 2 (or without a list)
+    int32_t pad1;
 };
 
 // Pointers of a batch, passed by value to every kernel.

@@ -121,3 +121,79 @@ def test_list_mode_transitions_keep_cell_list_consistent():
     s2.close()
     rms = np.sqrt((ref ** 2).sum(1).mean())
     assert np.sqrt((((acc - ref) ** 2).sum(1)).mean()) / rms < 5e-6
+
+
+def _acc_error(got, Lx, Ly, kw):
+    c = abi.Simulation(abi.make_params(Lx, Ly, **kw), got)
+    ref = c.compute_forces(True)
+    c.close()
+    rms = np.sqrt((ref ** 2).sum(1).mean())
+    err = np.sqrt(((got["acc"] - ref) ** 2).sum(1))
+    band = near_cutoff_mask(got, Lx, Ly, 2.5, (1.0, 1.0, 1.0))
+    return err[~band].max() / rms, np.sqrt((err ** 2).mean()) / rms
+
+
+@pytest.mark.parametrize("T,skin", [(0.8, 0.5), (2.0, 0.3)])
+def test_windows_keep_the_exact_pair_set(T, skin):
+    """Rebuild windows of 16 steps; at T=2 with skin 0.3 particles leave the
+    skin inside most windows, so the redo path runs."""
+    f, Lx, Ly = lattice_frame(12000, 0.75, seed=41, jitter=0.05, T=T, aspect=2.0)
+    kw = dict(eps_ab=0.5, dt=0.005, T_target=T, Q=10.0)
+    say(f"windows T={T} skin={skin}")
+    s = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+    s.set_tuning(1.0, 1, skin, 16)
+    assert s.get_tuning().window == 16
+    s.step(abi.NVT, 160)
+    got = s.collect()
+    s.close()
+    mx, rm = _acc_error(got, Lx, Ly, kw)
+    say(f"  max rel {mx:.2e} rms rel {rm:.2e}")
+    assert mx < 1e-4 and rm < 5e-6
+
+
+def test_windows_nve_drift_against_oracle():
+    n = 4096
+    f, Lx, Ly = abi.gen_lattice(n, n // 2, 0.8, 0.05, 0.7, 1001)
+    kw = dict(eps_ab=0.5, dt=0.005)
+    o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, **kw), f)
+    o.compute_forces()
+    Eo = [o.K() + o.U()]
+    s = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+    s.set_tuning(1.0, 1, 0.4, 16)
+    ob = s.observables()
+    E = [ob.K + ob.U]
+    for _ in range(20):
+        o.step_nve(500)
+        Eo.append(o.K() + o.U())
+        s.step(abi.NVE, 500)
+        ob = s.observables()
+        E.append(ob.K + ob.U)
+    s.close()
+    d_o = np.max(np.abs(np.array(Eo) - Eo[0])) / n
+    d = np.max(np.abs(np.array(E) - E[0])) / n
+    say(f"windowed NVE drift {d:.3e} oracle {d_o:.3e}")
+    assert d <= 2.0 * d_o
+
+
+def test_windows_same_alone_and_in_batch_with_a_redoing_neighbour():
+    cold = lattice_frame(3000, 0.75, seed=51, jitter=0.05, T=0.4, aspect=2.0)
+    hot = lattice_frame(2000, 0.75, seed=52, jitter=0.05, T=2.5)
+    kws = [dict(eps_ab=0.5, dt=0.005, T_target=0.4, Q=10.0), dict(eps_ab=0.7, dt=0.005, T_target=2.5, Q=10.0)]
+    alone = []
+    for (f, Lx, Ly), kw in zip((cold, hot), kws):
+        s = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+        s.set_tuning(1.0, 1, 0.3, 16)
+        s.step(abi.NVT, 96)
+        alone.append(s.collect())
+        s.close()
+    caps = [abi.cell_capacity(abi.make_params(Lx, Ly), 1.0) for _, Lx, Ly in (cold, hot)]
+    b = abi.Batch([3000, 2000], caps)
+    for r, ((f, Lx, Ly), kw) in enumerate(zip((cold, hot), kws)):
+        b.load(r, abi.make_params(Lx, Ly, **kw), f)
+        b.set_kind(r, abi.NVT)
+    b.set_tuning(1.0, 1, 0.3, 16)
+    b.step(96)
+    for r in range(2):
+        got = b.collect(r)
+        assert np.array_equal(got["pos"], alone[r]["pos"]) and np.array_equal(got["vel"], alone[r]["vel"]), r
+    b.close()

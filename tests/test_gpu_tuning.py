@@ -101,11 +101,11 @@ def test_autotune_reports_candidates_and_keeps_state():
     rep = s.autotune(abi.NVT, 10)
     say(f"  tried {rep.table()} best {rep.best.cell_scale} {rep.best.lanes_per_particle}")
     assert rep.n_tried == 18
-    assert all(m > 0 for _, _, _, m in rep.table())
-    assert rep.best_ms_per_step == min(m for _, _, _, m in rep.table())
+    assert all(row[-1] > 0 for row in rep.table())
+    assert rep.best_ms_per_step == min(row[-1] for row in rep.table())
     t = s.get_tuning()
     assert t.cell_scale == rep.best.cell_scale and t.lanes_per_particle == rep.best.lanes_per_particle
-    assert t.skin == rep.best.skin
+    assert t.skin == rep.best.skin and t.window == rep.best.window
     after = s.collect()
     assert np.array_equal(before["pos"], after["pos"]) and np.array_equal(before["vel"], after["vel"])
     s.step(abi.NVT, 20)
