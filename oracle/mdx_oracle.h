@@ -19,6 +19,12 @@
  *   FIRE                          src/md_energy_minimizer.cu:24-71,122-272,442-475
  *   periodic wrap                 include/md_cuda_common.hpp:20-29
  *
+ * What pins each function is listed with the tests (tests/test_oracle.py).
+ * FIRE: the fixed point of the reference's two-particle case and the
+ * max_steps / immediate-convergence semantics are pinned; the dt / alpha
+ * schedule constants are otherwise parity unpinned (no reference vector or
+ * closed form exercises them).
+ *
  * Deliberate, documented differences from the reference:
  *   - one periodic domain: every pair is counted once per ordered pair (the
  *     reference's rank_size==1 self-halo double count is not reproduced);
