@@ -967,7 +967,9 @@ double Batch::cell_side_for(const Slot &s) const {
 
 int Batch::ensure_list_arrays() {
     if (nlist) return MDX_OK;
-    if (N >= (int64_t{1} << 28)) return set_error(MDX_ERR_CAPACITY, "neighbour lists need fewer than 2^28 particles");
+    // list entries carry the partner's slot in 27 bits (mdx_kernels.hpp, list_entry)
+    if (N >= (int64_t{1} << 27)) return set_error(MDX_ERR_CAPACITY, "neighbour lists need fewer than 2^27 particles");
+    if (list_cap % 4 != 0) return set_error(MDX_ERR_INVALID, "list capacity must be a multiple of 4");
     const size_t n = static_cast<size_t>(std::max<int64_t>(N, 1));
     MDX_HIP(hipMalloc(&key_list, n * 4));
     MDX_HIP(hipMalloc(&x0, n * sizeof(double2)));

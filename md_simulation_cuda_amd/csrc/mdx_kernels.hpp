@@ -573,58 +573,126 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
     }
 }
 
+// Neighbour-list entry (list mode): (partner slot << 3), so that entry &
+// kListOff is the byte offset of the partner's FP32 image; bit 2 set when the
+// partner has the particle's own type class; the neighbour cell's offset ox in
+// bits 0-1 and oy in bits 30-31 (two's complement, -1..1).  Partner slots are
+// below 2^27.
+constexpr uint32_t kListOff = 0x3FFFFFF8u;
+constexpr uint32_t kListSame = 4u;
+// list rows a lane stages in LDS per pass of the list build
+constexpr int kListStage = 32;
+
+__device__ inline uint32_t list_entry(uint32_t k, int ox, int oy, bool same) {
+    return (k << 3) | (same ? kListSame : 0u) | (static_cast<uint32_t>(ox) & 3u) |
+           ((static_cast<uint32_t>(oy) & 3u) << 30);
+}
+__device__ inline float list_ox(uint32_t v) { return static_cast<float>(static_cast<int32_t>(v << 30) >> 30); }
+__device__ inline float list_oy(uint32_t v) { return static_cast<float>(static_cast<int32_t>(v) >> 30); }
+// Padding entry of slot s: the particle itself one cell to the side, at
+// distance w >= r_list > r_c, so it is never inside the cutoff.
+__device__ inline uint32_t list_pad(uint32_t s) { return list_entry(s, 1, 0, true); }
+// ncount[s] = rows | kListGuard when some pair of the wave's lists may come
+// closer than the FP64 guard distance before the next rebuild
+constexpr uint32_t kListGuard = 0x80000000u;
+
 // Neighbour list of every cell-ordered slot of the slots that rebuild in list
 // mode: the partners within r_list in the 3x3 neighbour cells, in the pair
-// kernel's order (rows -1, 0, +1; cells -1, 0, +1; ascending id).  An entry is
-// the partner's slot index with the neighbour cell's offset code
-// (ox + 1) | (oy + 1) << 2 in the top four bits.
+// kernel's order (rows -1, 0, +1; cells -1, 0, +1; ascending id).  Entry e of
+// slot s is nlist[e * N + s].  Every lane of a wave (64 consecutive slots,
+// the pair kernel's wave) gets the same length: the wave's longest list
+// rounded up to a multiple of 4, the rows past a lane's own list holding
+// list_pad.  A lane stages its entries in its own column of LDS and the wave
+// writes whole rows, so the stores are coalesced; lists longer than
+// kListStage take one more scan per kListStage rows.  Between rebuilds no
+// particle moves more than skin / 2 from its build position, so a pair comes
+// closer than the FP64 guard distance only if it is closer than
+// guard + skin now; the wave's length carries kListGuard when one is.
 __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
     const int gb = xcd_block(blockIdx.x, gridDim.x);
     const int r = P.blk2sys[gb];
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
     if (!sc.list_on || !sc.rebuild || sp.n <= 0) return;
-    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
-    if (li >= sp.n) return;
+    __shared__ uint32_t stage[kListStage][kBlock];
+    const int tid = static_cast<int>(threadIdx.x);
+    const int li = (gb - sp.boff) * kBlock + tid;
+    const bool active = li < sp.n;
     const Grid g = sc.g;
-    const uint32_t s = P.cs[sp.coff] + static_cast<uint32_t>(li);
-    const uint32_t lc = P.scell[s];
-    const int cx = static_cast<int>(lc % static_cast<uint32_t>(g.ncx));
-    const int cy = static_cast<int>(lc / static_cast<uint32_t>(g.ncx));
     const float2 *img = static_cast<const float2 *>(P.img);
-    const float2 pi = img[s];
     const float rl2 = static_cast<float>(sp.r_list * sp.r_list);
-    const uint32_t cap = static_cast<uint32_t>(sp.list_cap);
+    const uint32_t cap = static_cast<uint32_t>(sp.list_cap);   // a multiple of 4
     const size_t stride = static_cast<size_t>(P.N);
-    uint32_t cnt = 0;
-    bool over = false;
-    for (int oy = -1; oy <= 1; ++oy) {
-        int yy = cy + oy;
-        if (yy < 0) yy += g.ncy;
-        else if (yy >= g.ncy) yy -= g.ncy;
-        const int rowbase = sp.coff + yy * g.ncx;
-        const float piy = pi.y - static_cast<float>(oy) * g.hf;
-        for (int ox = -1; ox <= 1; ++ox) {
-            int xx = cx + ox;
-            if (xx < 0) xx += g.ncx;
-            else if (xx >= g.ncx) xx -= g.ncx;
-            const float pix = pi.x - static_cast<float>(ox) * g.wf;
-            const uint32_t code = static_cast<uint32_t>((ox + 1) | ((oy + 1) << 2)) << 28;
-            const uint32_t k0 = P.cs[rowbase + xx], k1 = P.cs[rowbase + xx + 1];
-            for (uint32_t k = k0; k < k1; ++k) {
-                if (k == s) continue;
-                const float2 pj = img[k];
-                const float dx = pix - pj.x, dy = piy - pj.y;
-                if (__builtin_fmaf(dx, dx, dy * dy) < rl2) {
-                    if (cnt < cap) P.nlist[cnt * stride + s] = k | code;
-                    else over = true;
-                    ++cnt;
+    uint32_t s = 0;
+    int cx = 0, cy = 0, ci = 0;
+    float2 pi = make_float2(0.0f, 0.0f);
+    if (active) {
+        s = P.cs[sp.coff] + static_cast<uint32_t>(li);
+        const uint32_t lc = P.scell[s];
+        cx = static_cast<int>(lc % static_cast<uint32_t>(g.ncx));
+        cy = static_cast<int>(lc / static_cast<uint32_t>(g.ncx));
+        pi = img[s];
+        ci = P.icls[s];
+    }
+    // pairs closer than this may reach the guard distance before a rebuild
+    // (with a margin for the FP32 difference between this and the pair form)
+    const float rgd = sqrtf(sp.pt.r2_guard) + 2.0f * sqrtf(static_cast<float>(sp.skin_half2)) + 1.0e-3f;
+    const float rg2 = rgd * rgd;
+    bool near = false;
+    uint32_t rows = 0, cnt = 0;
+    for (uint32_t base = 0;; base += kListStage) {
+        cnt = 0;
+        if (active) {
+            for (int oy = -1; oy <= 1; ++oy) {
+                int yy = cy + oy;
+                if (yy < 0) yy += g.ncy;
+                else if (yy >= g.ncy) yy -= g.ncy;
+                const int rowbase = sp.coff + yy * g.ncx;
+                const float piy = pi.y - static_cast<float>(oy) * g.hf;
+                for (int ox = -1; ox <= 1; ++ox) {
+                    int xx = cx + ox;
+                    if (xx < 0) xx += g.ncx;
+                    else if (xx >= g.ncx) xx -= g.ncx;
+                    const float pix = pi.x - static_cast<float>(ox) * g.wf;
+                    const uint32_t k0 = P.cs[rowbase + xx], k1 = P.cs[rowbase + xx + 1];
+                    for (uint32_t k = k0; k < k1; ++k) {
+                        if (k == s) continue;
+                        const float2 pj = img[k];
+                        const float dx = pix - pj.x, dy = piy - pj.y;
+                        const float rr = __builtin_fmaf(dx, dx, dy * dy);
+                        if (rr < rl2) {
+                            near = near || rr < rg2;
+                            if (cnt - base < static_cast<uint32_t>(kListStage))
+                                stage[cnt - base][tid] = list_entry(k, ox, oy, P.icls[k] == ci);
+                            ++cnt;
+                        }
+                    }
                 }
             }
         }
+        if (base == 0) {   // the wave's row count
+            uint32_t m = active ? cnt : 0u;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const uint32_t t = static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), o, 64));
+                m = t > m ? t : m;
+            }
+            m = (m + 3u) & ~3u;
+            rows = m < cap ? m : cap;
+        }
+        const uint32_t end = rows < base + kListStage ? rows : base + kListStage;
+        if (active) {
+            const uint32_t pad = list_pad(s);
+            for (uint32_t e = base; e < end; ++e)
+                P.nlist[static_cast<size_t>(e) * stride + s] = e < cnt ? stage[e - base][tid] : pad;
+        }
+        if (base + kListStage >= rows) break;
     }
-    P.ncount[s] = cnt < cap ? cnt : cap;
-    if (over) sc.list_overflow = 1;   // this step's pair pass uses the cells instead
+    const bool guard = __any(near);
+    if (active) {
+        P.ncount[s] = rows | (guard ? kListGuard : 0u);
+        if (cnt > cap) sc.list_overflow = 1;   // this step's pair pass uses the cells instead
+    }
     if (li == 0) sc.list_built = 1;
 }
 
@@ -830,92 +898,176 @@ __device__ __forceinline__ void pair_cell_f32(const char *__restrict__ imgb, con
     if constexpr (ACC == ACC_COMP32) F.add(fxc, fyc);
 }
 
+// One neighbour-list entry of the particle at (pix, piy) (its image) whose
+// partner image is q: the FP32 pair of pair_cell_f32 with the displacement
+// formed from the entry's cell offset.  The force is computed unconditionally
+// and kept only when the pair is inside the cutoff, so no branch is taken per
+// entry.  GUARD: pairs closer than the FP64 guard are left out here; `slow`
+// collects, as a bit per lane, whether one of them is also farther than 0
+// (0 < r^2 < r2_guard, compared as IEEE bit patterns, which order like the
+// values for r^2 >= 0), and the caller evaluates those in FP64.
+template <int ACC, bool OBS, bool UNI, bool GUARD>
+__device__ __forceinline__ void list_pair_f32(uint32_t v, float2 q, float pix, float piy, int ci, const Grid &g,
+                                              const PairTable &pt, const PairF32 &pc, float &fxc, float &fyc,
+                                              ForceSum<ACC> &F, ObsSum &ob, bool want_obs, bool &slow) {
+    const float dx = __builtin_fmaf(-list_ox(v), g.wf, pix) - q.x;
+    const float dy = __builtin_fmaf(-list_oy(v), g.hf, piy) - q.y;
+    const float rr = __builtin_fmaf(dx, dx, dy * dy);
+    const bool same = (v & kListSame) != 0u;
+    const float rc2 = UNI ? pc.rc2s : (same ? pc.rc2s : pc.rc2o);
+    const float sig2 = UNI ? pc.sig2s : (same ? pc.sig2s : pc.sig2o);
+    const float e24 = same ? pc.e24s : pc.e24o;
+    const float r2i = fast_rcp(rr);
+    const float sr2 = sig2 * r2i;
+    const float sr6 = sr2 * sr2 * sr2;
+    const float sr12 = sr6 * sr6;
+    const float t = e24 * __builtin_fmaf(2.0f, sr12, -sr6) * r2i;
+    const bool in = GUARD ? (rr < rc2 && rr >= pc.r2_guard) : rr < rc2;
+    const float tmp = in ? t : 0.0f;
+    const float fx = tmp * dx, fy = tmp * dy;
+    if constexpr (ACC == ACC_COMP32) {
+        fxc += fx;
+        fyc += fy;
+    } else {
+        F.add(fx, fy);
+    }
+    if constexpr (GUARD)
+        slow = slow || (__float_as_uint(rr) - 1u < __float_as_uint(pc.r2_guard) - 1u);
+    if constexpr (OBS) {
+        if (want_obs && in) {
+            const float e4 = same ? pc.e4s : pc.e4o;
+            const float uu = e4 * (sr12 - sr6);
+            const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
+            ob.u += 0.5 * (static_cast<double>(uu) - pt.vc[same ? ci : 2]);
+            ob.w += 0.5 * (ddx * dfx + ddy * dfy);
+            ob.lam += 0.5 * (dfx * ddx - dfy * ddy);
+        }
+    }
+}
+
+// The FP64 evaluation of the guarded pairs among U entries (0 < r^2 <
+// r2_guard), in entry order.
+template <int U, bool OBS>
+__device__ __forceinline__ void list_pairs_guarded(const uint32_t (&v)[U], const float2 (&q)[U], float pix,
+                                                   float piy, int ci, const Grid &g, const PairTable &pt,
+                                                   const PairF32 &pc, double &gx, double &gy, ObsSum &ob,
+                                                   bool want_obs) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const float dx = __builtin_fmaf(-list_ox(v[u]), g.wf, pix) - q[u].x;
+        const float dy = __builtin_fmaf(-list_oy(v[u]), g.hf, piy) - q[u].y;
+        const float rr = __builtin_fmaf(dx, dx, dy * dy);
+        if (rr < pc.r2_guard && rr > 0.0f) {
+            double fx, fy;
+            pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), (v[u] & kListSame) ? ci : 2, pt, fx, fy,
+                           ob, want_obs);
+            gx += fx;
+            gy += fy;
+        }
+    }
+}
+
+// Rows [0, n) of slot s's list with one lane per particle; n is the same for
+// the whole wave, so the loop is uniform.  Each pass takes four rows; the
+// next pass's rows are loaded before this pass's pairs are evaluated (the last
+// pass reloads its own rows).  Rows are addressed from a scalar base plus the
+// lane's 32-bit byte offset.
+template <int ACC, bool OBS, bool UNI, bool GUARD>
+__device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, uint32_t n, float2 pi, int ci,
+                                              const Grid &g, const PairTable &pt, const PairF32 &pc,
+                                              ForceSum<ACC> &F, ObsSum &ob, bool want_obs, float &fxc, float &fyc,
+                                              double &gx, double &gy) {
+    constexpr int U = 4;
+    const char *imgb = static_cast<const char *>(P.img);
+    const char *lb = reinterpret_cast<const char *>(P.nlist);
+    const size_t rowb = static_cast<size_t>(P.N) * 4u;   // bytes per row
+    const uint32_t soff = s * 4u;
+    uint32_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const uint32_t *>(lb + u * rowb + soff);
+    for (uint32_t e = 0; e < n; e += U) {
+        float2 q[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) q[u] = *reinterpret_cast<const float2 *>(imgb + (v[u] & kListOff));
+        const uint32_t en = e + U < n ? e + U : e;
+        const char *row = lb + static_cast<size_t>(en) * rowb;
+        uint32_t vn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) vn[u] = *reinterpret_cast<const uint32_t *>(row + u * rowb + soff);
+        bool slow = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            list_pair_f32<ACC, OBS, UNI, GUARD>(v[u], q[u], pi.x, pi.y, ci, g, pt, pc, fxc, fyc, F, ob, want_obs,
+                                                slow);
+        if constexpr (GUARD) {
+            if (__any(slow)) list_pairs_guarded<U, OBS>(v, q, pi.x, pi.y, ci, g, pt, pc, gx, gy, ob, want_obs);
+        }
+        if constexpr (ACC == ACC_COMP32) {
+            if (e & U) {   // every eight rows
+                F.add(fxc, fyc);
+                fxc = fyc = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = vn[u];
+    }
+}
+
 // FP32 pairs of cell-ordered slot s from its neighbour list (list mode):
 // entries in the cell order of the build, each carrying its neighbour cell's
 // offset, so the displacement is formed exactly as in pair_cell_f32.  Pair
 // forces are summed in FP32 over runs of eight entries, each run entering the
-// compensated accumulator once (COMP32); ACC64 adds every pair in FP64.
-// Entries are taken four at a time so their loads are in flight together;
-// slots past the end of the list stand for the particle itself (excluded by
-// r^2 > 0).
+// compensated accumulator once (COMP32); ACC64 adds every pair in FP64.  With
+// LPP lanes per particle, lane q takes rows q, q + LPP, ...; rows past the
+// list stand for list_pad.
 template <int ACC, bool OBS, bool UNI, int LPP>
 __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, float2 pi, int ci, const Grid &g,
                                               const PairTable &pt, const PairF32 &pc, ForceSum<ACC> &F,
                                               ObsSum &ob, bool want_obs) {
     constexpr int U = 4;
-    constexpr uint32_t kSelf = 5u << 28;   // offset code of the particle's own cell
-    const char *imgb = static_cast<const char *>(P.img);
-    const uint32_t n = P.ncount[s];
-    const size_t stride = static_cast<size_t>(P.N);
     float fxc = 0.0f, fyc = 0.0f;
-    uint32_t run = 0;
-    for (uint32_t e = (LPP > 1 ? threadIdx.x % LPP : 0u); e < n; e += U * LPP) {
-        uint32_t v[U];
+    double gx = 0.0, gy = 0.0;
+    if constexpr (LPP == 1) {
+        const uint32_t nr = __builtin_amdgcn_readfirstlane(P.ncount[s]);
+        const uint32_t n = nr & ~kListGuard;
+        if (nr & kListGuard)
+            list_rows_f32<ACC, OBS, UNI, true>(P, s, n, pi, ci, g, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy);
+        else
+            list_rows_f32<ACC, OBS, UNI, false>(P, s, n, pi, ci, g, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy);
+    } else {
+        const char *imgb = static_cast<const char *>(P.img);
+        const size_t stride = static_cast<size_t>(P.N);
+        const uint32_t n = P.ncount[s] & ~kListGuard;
+        const uint32_t pad = list_pad(s);
+        uint32_t run = 0;
+        for (uint32_t e = threadIdx.x % LPP; e < n; e += U * LPP) {
+            uint32_t v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t ee = e + u * LPP;
-            v[u] = ee < n ? P.nlist[ee * stride + s] : (s | kSelf);
-        }
-        float2 pj[U];
-        int cj[U];
+            for (int u = 0; u < U; ++u) {
+                const uint32_t ee = e + u * LPP;
+                v[u] = ee < n ? P.nlist[ee * stride + s] : pad;
+            }
+            float2 q[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t k = v[u] & 0x0FFFFFFFu;
-            pj[u] = *reinterpret_cast<const float2 *>(imgb + (k << 3));
-            cj[u] = P.icls[k];
-        }
+            for (int u = 0; u < U; ++u) q[u] = *reinterpret_cast<const float2 *>(imgb + (v[u] & kListOff));
+            bool slow = false;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const float ox = static_cast<float>(static_cast<int>((v[u] >> 28) & 3u) - 1);
-            const float oy = static_cast<float>(static_cast<int>(v[u] >> 30) - 1);
-            const float dx = __builtin_fmaf(-ox, g.wf, pi.x) - pj[u].x;
-            const float dy = __builtin_fmaf(-oy, g.hf, pi.y) - pj[u].y;
-            const float rr = __builtin_fmaf(dx, dx, dy * dy);
-            const bool same = cj[u] == ci;
-            const float rc2 = UNI ? pc.rc2s : (same ? pc.rc2s : pc.rc2o);
-            if (rr < rc2 && rr > 0.0f) {
-                if (rr < pc.r2_guard) {
-                    double fx, fy;
-                    pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), pair_index(ci, cj[u]), pt, fx,
-                                   fy, ob, want_obs);
-                    F.addd(fx, fy);
-                } else {
-                    const float sig2 = UNI ? pc.sig2s : (same ? pc.sig2s : pc.sig2o);
-                    const float e24 = same ? pc.e24s : pc.e24o;
-                    const float r2i = fast_rcp(rr);
-                    const float sr2 = sig2 * r2i;
-                    const float sr6 = sr2 * sr2 * sr2;
-                    const float sr12 = sr6 * sr6;
-                    const float tmp = e24 * __builtin_fmaf(2.0f, sr12, -sr6) * r2i;
-                    const float fx = tmp * dx, fy = tmp * dy;
-                    if constexpr (ACC == ACC_COMP32) {
-                        fxc += fx;
-                        fyc += fy;
-                    } else {
-                        F.add(fx, fy);
-                    }
-                    if constexpr (OBS) {
-                        if (want_obs) {
-                            const float e4 = same ? pc.e4s : pc.e4o;
-                            const float uu = e4 * (sr12 - sr6);
-                            const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
-                            ob.u += 0.5 * (static_cast<double>(uu) - pt.vc[pair_index(ci, cj[u])]);
-                            ob.w += 0.5 * (ddx * dfx + ddy * dfy);
-                            ob.lam += 0.5 * (dfx * ddx - dfy * ddy);
-                        }
+            for (int u = 0; u < U; ++u) {
+                list_pair_f32<ACC, OBS, UNI, true>(v[u], q[u], pi.x, pi.y, ci, g, pt, pc, fxc, fyc, F, ob, want_obs,
+                                                   slow);
+                if constexpr (ACC == ACC_COMP32) {
+                    if (++run == 8u) {
+                        F.add(fxc, fyc);
+                        fxc = fyc = 0.0f;
+                        run = 0;
                     }
                 }
             }
-            if constexpr (ACC == ACC_COMP32) {
-                if (++run == 8u) {
-                    F.add(fxc, fyc);
-                    fxc = fyc = 0.0f;
-                    run = 0;
-                }
-            }
+            if (slow) list_pairs_guarded<U, OBS>(v, q, pi.x, pi.y, ci, g, pt, pc, gx, gy, ob, want_obs);
         }
     }
     if constexpr (ACC == ACC_COMP32) F.add(fxc, fyc);
+    F.addd(gx, gy);
 }
 
 // FP64 pairs of particle pi (wrapped FP64 position) with the candidates
