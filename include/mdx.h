@@ -209,12 +209,17 @@ int mdx_timing_read(mdx_sim *sim, mdx_timing *out);
  * of at least cutoff + skin) at the step where some particle has moved more
  * than skin / 2 since the last build; the pairs evaluated are exactly those
  * within the cutoff, as without the list.  skin = 0: cell list every step.
- * window: with window W > 1 and every stepping slot in list mode, steps run in
- * windows of W: the first rebuilds the lists, the other W - 1 run only the
- * drift, image and pair kernels; a slot in which some particle moved beyond
- * skin / 2 inside the window is put back to the window's start and redone
- * with on-demand rebuilds.  Rebuild points depend only on W and the slot's own
- * particles.
+ * window: with window W > 1 and every stepping slot in list mode (boxes of at
+ * least 5 cells per side), steps run in windows of W: the first (head) step
+ * rebuilds a slot's lists when some particle, at 1.5 x its current speed,
+ * could move beyond skin / 2 before the window ends; the other W - 1 steps run
+ * only the box, drift, image and pair kernels.  A slot in which a particle
+ * moves beyond skin / 2 anyway (or that has no usable list) evaluates the
+ * rest of the window's lean steps over the 5 x 5 cells of its last cell list,
+ * which holds every pair within the cutoff while no particle has moved more
+ * than (2 w - cutoff) / 2 since that build; beyond that the step returns
+ * MDX_ERR_STATE.  The run does not synchronise with the host until its end.
+ * Rebuild points depend only on W and the slot's own particles.
  * Changing the tuning recomputes the accelerations; FP32 results then follow
  * the new summation order (same tolerance, bit-identical to a fresh run with
  * the same tuning). */
@@ -239,7 +244,7 @@ int mdx_set_tuning(mdx_sim *sim, const mdx_tuning *t);
 int mdx_get_tuning(mdx_sim *sim, mdx_tuning *out);
 /* Autotuner: on a copy of the system, times `steps` steps of ensemble `kind`
  * for every candidate (without lists: cell_scale 1.0, 1.1 x lanes 1, 2, 4;
- * with lists: skin 0.3, 0.5, 0.8 x lanes 1, 2 x window 0, 16; FP64 form: cell
+ * with lists: skin 0.3, 0.5, 0.8 x lanes 1, 2 x window 0, 4, 8; FP64 form: cell
  * list, one lane), then applies the fastest to the handle.  The handle's
  * state is not advanced. */
 int mdx_autotune(mdx_sim *sim, int32_t kind, int32_t steps, mdx_tune_report *out);

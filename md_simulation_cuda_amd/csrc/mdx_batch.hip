@@ -293,12 +293,14 @@ static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &
 int Batch::launch_step_kernels(bool obs, int mode) {
     const int S = static_cast<int>(slots.size());
     const bool lean = mode == STEP_LEAN;
+    // a window's head step keeps lists only if they last its window - 1 lean steps
+    const int ahead = mode == STEP_HEAD ? std::max(window - 1, 0) : 0;
     k_box<<<S, kBlock, 0, stream>>>(ptrs, mode);
     if (TB > 0) {
         if (prec == MDX_PREC_COMP32)
-            k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs, lean ? 1 : 0);
+            k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs, lean ? 1 : 0, ahead);
         else
-            k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs, lean ? 1 : 0);
+            k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs, lean ? 1 : 0, ahead);
     }
     if (!lean) {   // cell list (and, in list mode, neighbour lists) of the slots that rebuild
         if (TCB + TB > 0) k_stay_arrive<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
@@ -393,12 +395,25 @@ int Batch::replay(bool obs, int mode, int count_) {
 
 // Windows apply when the batch is tuned for them and every stepping slot is a
 // list-mode slot (NVE / NVT in an FP32 form).
-bool Batch::windows_apply() const {
+bool Batch::windows_apply() {
     if (window < 2 || !nlist || prec == MDX_PREC_FP64) return false;
     bool any = false;
     for (size_t r = 0; r < slots.size(); ++r) {
-        if (!slots[r].loaded || h_sp[r].kind == SK_NONE) continue;
+        Slot &sl = slots[r];
+        if (!sl.loaded || h_sp[r].kind == SK_NONE) continue;
         if (!h_sp[r].list_mode || h_sp[r].middle_wrap) return false;
+        // the lean-step fallback needs 5 distinct cells per side; list-mode
+        // kinds (NVE, NVT) keep the box, which is read back once after an NPH
+        // or piston run may have changed it
+        if (sl.box_moved) {
+            DevScalars h;
+            if (read_scalars(static_cast<int>(r), h) != MDX_OK) return false;
+            sl.Lx_now = h.g.Lx;
+            sl.Ly_now = h.g.Ly;
+            sl.box_moved = false;
+        }
+        const Grid g = make_grid(sl.Lx_now, sl.Ly_now, h_sp[r].cell_side, sl.rc_max, 1);
+        if (g.ncx < 5 || g.ncy < 5) return false;
         any = true;
     }
     return any;
@@ -429,73 +444,34 @@ int Batch::run_passes(int count_, bool obs) {
     return MDX_OK;
 }
 
-// Rebuild windows (list mode): snapshot, one rebuilding step, window - 1 lean
-// steps, then the slots whose lean steps were not covered by their lists
-// (a particle beyond skin / 2, or no usable list) go back to the snapshot and
-// redo the window on the normal path, alone.
+// Rebuild windows (list mode): each window is a head step (STEP_HEAD: the
+// slot rebuilds its lists unless they last the window's lean steps) and
+// window - 1 lean steps (no cell-list kernels; a slot with a particle beyond
+// skin / 2 anyway, or without a usable list, takes the 5x5 pass over its last
+// cell list, which is exact while every particle stays within (2 w - rc_max)
+// / 2 of that build).  No host synchronisation inside the run; at its end the
+// slots are checked for a lean step that pass could not cover.
 int Batch::run_windows(int count_) {
     const int S = static_cast<int>(slots.size());
-    const size_t n = static_cast<size_t>(N);
-    const size_t asz = prec == MDX_PREC_COMP32 ? sizeof(float2) : sizeof(double2);
-    if (!x_bak) {
-        MDX_HIP(hipMalloc(&x_bak, n * sizeof(double2)));
-        MDX_HIP(hipMalloc(&v_bak, n * sizeof(double2)));
-        MDX_HIP(hipMalloc(&a_bak, n * sizeof(double2)));
-        MDX_HIP(hipMalloc(&sc_bak, static_cast<size_t>(S) * sizeof(DevScalars)));
-    }
-    std::vector<DevScalars> cur(S), bak(S);
-    std::vector<SysParams> keep = h_sp;
     for (int left = count_; left > 0;) {
         const int w = std::min(window, left);
-        MDX_HIP(hipMemcpyAsync(x_bak, x, n * sizeof(double2), hipMemcpyDeviceToDevice, stream));
-        MDX_HIP(hipMemcpyAsync(v_bak, v, n * sizeof(double2), hipMemcpyDeviceToDevice, stream));
-        MDX_HIP(hipMemcpyAsync(a_bak, a, n * asz, hipMemcpyDeviceToDevice, stream));
-        MDX_HIP(hipMemcpyAsync(sc_bak, d_sc, S * sizeof(DevScalars), hipMemcpyDeviceToDevice, stream));
-        MDX_TRY(replay(false, STEP_REBUILD, 1));
+        MDX_TRY(replay(false, STEP_HEAD, 1));
         MDX_TRY(replay(false, STEP_LEAN, w - 1));
-        k_finalize<<<S, kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
-        MDX_HIP(hipGetLastError());
-        MDX_HIP(hipMemcpyAsync(cur.data(), d_sc, S * sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
-        MDX_HIP(hipStreamSynchronize(stream));
-        std::vector<int> redo;
-        for (int r = 0; r < S; ++r)
-            if (slots[r].loaded && keep[r].kind != SK_NONE && cur[r].violated) redo.push_back(r);
-        if (!redo.empty()) {
-            MDX_HIP(hipMemcpyAsync(bak.data(), sc_bak, S * sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
-            MDX_HIP(hipStreamSynchronize(stream));
-            for (int r = 0; r < S; ++r) set_idle(h_sp[r], slots[r]);
-            for (int r : redo) {
-                const size_t off = static_cast<size_t>(h_sp[r].off), cnt = static_cast<size_t>(h_sp[r].n);
-                MDX_HIP(hipMemcpyAsync(x + off, x_bak + off, cnt * sizeof(double2), hipMemcpyDeviceToDevice,
-                                       stream));
-                MDX_HIP(hipMemcpyAsync(v + off, v_bak + off, cnt * sizeof(double2), hipMemcpyDeviceToDevice,
-                                       stream));
-                MDX_HIP(hipMemcpyAsync(static_cast<char *>(a) + off * asz, static_cast<char *>(a_bak) + off * asz,
-                                       cnt * asz, hipMemcpyDeviceToDevice, stream));
-                // the physics of the window start, the cell-list state of now
-                DevScalars h = bak[r];
-                h.g = cur[r].g;
-                h.g_old = cur[r].g_old;
-                h.list_valid = cur[r].list_valid;
-                h.list_built = cur[r].list_built;
-                h.list_overflow = cur[r].list_overflow;
-                h.pair_pending = 0;
-                h.violated = 0;
-                MDX_TRY(write_scalar_block(r, h));
-                h_sp[r] = keep[r];
-            }
-            MDX_TRY(upload_all_sp());
-            MDX_TRY(replay(false, STEP_NORMAL, w));
-            k_finalize<<<S, kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
-            MDX_HIP(hipGetLastError());
-            h_sp = keep;
-            MDX_TRY(upload_all_sp());
-        }
-        if (std::any_of(cur.begin(), cur.end(), [](const DevScalars &q) { return q.violated != 0; })) {
-            k_clear_violations<<<ceil_div(S, kBlock), kBlock, 0, stream>>>(ptrs);
-            MDX_HIP(hipGetLastError());
-        }
         left -= w;
+    }
+    k_finalize<<<S, kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
+    MDX_HIP(hipGetLastError());
+    std::vector<DevScalars> cur(S);
+    MDX_HIP(hipMemcpyAsync(cur.data(), d_sc, S * sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+    MDX_HIP(hipStreamSynchronize(stream));
+    bool bad = false;
+    for (int r = 0; r < S; ++r) bad = bad || (slots[r].loaded && cur[r].violated != 0);
+    if (bad) {
+        k_clear_violations<<<ceil_div(S, kBlock), kBlock, 0, stream>>>(ptrs);
+        MDX_HIP(hipGetLastError());
+        return set_error(MDX_ERR_STATE,
+                         "rebuild window: a particle moved beyond the reach of the lean-step fallback "
+                         "(or the grid has fewer than 5 cells per side); use a shorter window");
     }
     return MDX_OK;
 }
@@ -522,6 +498,9 @@ int Batch::load(int r, const mdx_params &p, const mdx_particle64 *frame, int64_t
     s.t = 0.0;
     s.Lx0 = p.Lx;
     s.Ly0 = p.Ly;
+    s.Lx_now = p.Lx;
+    s.Ly_now = p.Ly;
+    s.box_moved = false;
     s.cells_valid = false;
     s.cells_py = 1;
     s.K_valid = s.W_valid = false;
@@ -608,6 +587,7 @@ int Batch::set_kind(int r, int kind, double p_ext, uint32_t flags) {
     slots[r].kind = kind;
     slots[r].p_ext = p_ext;
     slots[r].flags = flags;
+    if (kind == SK_NPH || kind == SK_PISTON) slots[r].box_moved = true;
     return MDX_OK;
 }
 
@@ -1014,7 +994,9 @@ int Batch::set_tuning(const mdx_tuning &t) {
     }
     cell_scale = t.cell_scale;
     skin = new_skin;
-    window = new_skin > 0.0 ? t.window : 0;
+    const int new_window = new_skin > 0.0 ? t.window : 0;
+    if (new_window != window) invalidate_graphs();   // the head step's look-ahead is captured
+    window = new_window;
     std::vector<int> per, pis;
     for (size_t r = 0; r < slots.size(); ++r) {
         Slot &s = slots[r];
@@ -1075,7 +1057,7 @@ int Batch::autotune(int steps, mdx_tune_report *rep, int kind_override) {
     if (prec != MDX_PREC_FP64)
         for (double sk : {0.3, 0.5, 0.8})
             for (int l : {1, 2})
-                for (int w : {0, 16}) cands.push_back(mdx_tuning{1.0, sk, l, w});
+                for (int w : {0, 4, 8}) cands.push_back(mdx_tuning{1.0, sk, l, w});
     double best = 1e300;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MDX_HIP(hipEventCreate(&e0));

@@ -28,6 +28,10 @@ struct Slot {
     std::vector<double> theta;
     double t = 0.0;
     double Lx0 = 0.0, Ly0 = 0.0;
+    // box as the host last knew it; box_moved once an NPH or piston step may
+    // have changed it on the device
+    double Lx_now = 0.0, Ly_now = 0.0;
+    bool box_moved = false;
     // validity of device-side quantities for the current positions
     bool cells_valid = false;
     int cells_py = 1;
@@ -65,7 +69,7 @@ struct Batch {
     int32_t *d_blk2sys = nullptr, *d_cblk2sys = nullptr, *d_want = nullptr;
     BatchPtrs ptrs{};
     // captured step graphs, by [obs]
-    hipGraphExec_t graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};   // [obs][StepMode]
+    hipGraphExec_t graph[2][4] = {};   // [obs][StepMode]
     bool use_graph = true;
     bool graph_uni = true;   // pair-kernel variant the captured graphs hold
     int lpp = 1;             // lanes per particle of the FP32 pair kernel (1, 2, 4)
@@ -130,7 +134,7 @@ struct Batch {
     int run_passes(int count, bool obs);
     int replay(bool obs, int mode, int count);
     int run_windows(int count);
-    bool windows_apply() const;
+    bool windows_apply();
     int pre_step();
     int rebuild_cells(const std::vector<int> &which, int periodic_y);
     int kinetic(const std::vector<int> &which);

@@ -75,6 +75,8 @@ enum StepMode : int32_t {
     STEP_NORMAL = 0,    // list slots rebuild when a particle moved beyond skin / 2
     STEP_REBUILD = 1,   // list slots rebuild their cell and neighbour lists
     STEP_LEAN = 2,      // list slots keep their lists; no cell-list kernels run
+    STEP_HEAD = 3,      // first step of a window: STEP_NORMAL, with lists that
+                        // must also hold through the window's lean steps
 };
 
 __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
@@ -164,12 +166,14 @@ __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
     const bool on = sp.list_mode && !sc.list_overflow && g.periodic_y && g.w >= sp.r_list && g.h >= sp.r_list;
     sc.list_on = on ? 1 : 0;
     if (mode == STEP_LEAN) {
-        // no cell-list kernels run: a stepping slot without a usable list has
-        // to redo the window on the normal path
+        // no cell-list kernels run: a stepping slot without a usable list takes
+        // the 5x5 pass over its last cell list, which needs 5 distinct cells
+        // per row and column
         sc.rebuild = 0;
-        if (sp.kind != SK_NONE && (!on || !sc.list_built)) sc.violated = 1;
+        sc.fallback = (sp.kind != SK_NONE && (!on || !sc.list_built)) ? 1 : 0;
     } else {
         sc.rebuild = (!on || !sc.list_built || mode == STEP_REBUILD) ? 1 : 0;
+        sc.fallback = 0;
     }
 }
 
@@ -213,8 +217,12 @@ __device__ inline void append_movers(const BatchPtrs &P, bool mover, uint32_t pi
 // piston) and src/md_energy_minimizer.cu:32-71 (FIRE zero/mix/drift).  Slots
 // that do not move this step (idle, converged FIRE) still recompute keys so
 // the cell list is rebuilt unchanged.
+// lean: a lean step (no cell-list kernels); ahead: a window's head step, whose
+// lists must also hold through `ahead` lean steps: the slot rebuilds now when
+// some particle, moving 1.5 x its current speed, would pass skin / 2 within
+// them (a particle that passes it anyway makes its lean step fall back).
 template <typename AT>
-__global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t lean) {
+__global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t lean, int32_t ahead) {
     const int gb = xcd_block(blockIdx.x, gridDim.x);
     const int r = P.blk2sys[gb];
     const SysParams &sp = P.sp[r];
@@ -234,6 +242,7 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t lean) {
         const int i = sp.off + li;
         double2 p = P.x[i];
         const double dt = sp.dt;
+        double speed2 = 0.0;   // |v| after this step's kick
         if (moves) {
             double2 q = P.v[i];
             const double2 acc = load_acc<AT>(P.a, i);
@@ -300,7 +309,8 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t lean) {
             }
             P.v[i] = q;
             P.x[i] = p;
-            if (reduce) red = 0.5 * sp.pt.m[P.cls[i]] * (q.x * q.x + q.y * q.y);
+            speed2 = q.x * q.x + q.y * q.y;
+            if (reduce) red = 0.5 * sp.pt.m[P.cls[i]] * speed2;
         } else if (sp.wrap_keys) {
             p.x = pbc_wrap(p.x, g.Lx);
             if (g.periodic_y) p.y = pbc_wrap(p.y, g.Ly);
@@ -311,20 +321,29 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t lean) {
         // movers are relative to the keys the current cell list was built
         // from: key_list after a list-mode build, else the previous step's key
         mover = !sc.list_valid || (sc.list_built ? P.key_list[i] : P.key[i]) != c;
-        if (sc.list_on && sc.list_built) {
-            // a particle beyond skin / 2 of its build position makes this step
-            // rebuild the lists
-            {
-                const double2 p0 = P.x0[i];
-                double dx = p.x - p0.x, dy = p.y - p0.y;
-                if (dx > 0.5 * g.Lx) dx -= g.Lx;
-                else if (dx < -0.5 * g.Lx) dx += g.Lx;
-                if (dy > 0.5 * g.Ly) dy -= g.Ly;
-                else if (dy < -0.5 * g.Ly) dy += g.Ly;
-                if (dx * dx + dy * dy > sp.skin_half2) {   // same value from every writer
-                    if (lean) sc.violated = 1;
-                    else sc.rebuild = 1;
+        // displacement since the last cell-list build of a list-mode slot (x0
+        // is written at every such build)
+        if (lean ? (sp.list_mode != 0 && sc.list_valid) : (sc.list_on && sc.list_built)) {
+            const double2 p0 = P.x0[i];
+            double dx = p.x - p0.x, dy = p.y - p0.y;
+            if (dx > 0.5 * g.Lx) dx -= g.Lx;
+            else if (dx < -0.5 * g.Lx) dx += g.Lx;
+            if (dy > 0.5 * g.Ly) dy -= g.Ly;
+            else if (dy < -0.5 * g.Ly) dy += g.Ly;
+            const double d2 = dx * dx + dy * dy;
+            if (lean) {
+                // beyond skin / 2 the list may miss pairs: the 5x5 pass; beyond
+                // (2 min(w, h) - rc_max) / 2 even that may (same value from every writer)
+                if (d2 > sp.skin_half2) sc.fallback = 1;
+                const double reach = 0.5 * (2.0 * fmin(g.w, g.h) - sp.rc_max);
+                if (d2 > reach * reach) sc.violated = 1;
+            } else {
+                double r2 = d2;
+                if (ahead > 0) {
+                    const double rr = sqrt(d2) + 1.5 * ahead * sp.dt * sqrt(speed2);
+                    r2 = rr * rr;
                 }
+                if (r2 > sp.skin_half2) sc.rebuild = 1;   // same value from every writer
             }
         }
         P.key[i] = c;
@@ -566,7 +585,7 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
         P.icls[k] = P.cls[id];
         P.scell[k] = lc;
         P.sorted_old[k] = id;
-        if (sc.list_on) {
+        if (sp.list_mode) {
             P.x0[id] = p;
             P.key_list[id] = P.key[id];
         }
@@ -614,7 +633,7 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
     if (!sc.list_on || !sc.rebuild || sp.n <= 0) return;
-    __shared__ uint32_t stage[kListStage][kBlock];
+    __shared__ uint32_t stage[kListStage + 1][kBlock];   // + a spare row for rejected candidates
     const int tid = static_cast<int>(threadIdx.x);
     const int li = (gb - sp.boff) * kBlock + tid;
     const bool active = li < sp.n;
@@ -649,23 +668,52 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
                 else if (yy >= g.ncy) yy -= g.ncy;
                 const int rowbase = sp.coff + yy * g.ncx;
                 const float piy = pi.y - static_cast<float>(oy) * g.hf;
-                for (int ox = -1; ox <= 1; ++ox) {
-                    int xx = cx + ox;
-                    if (xx < 0) xx += g.ncx;
-                    else if (xx >= g.ncx) xx -= g.ncx;
-                    const float pix = pi.x - static_cast<float>(ox) * g.wf;
-                    const uint32_t k0 = P.cs[rowbase + xx], k1 = P.cs[rowbase + xx + 1];
-                    for (uint32_t k = k0; k < k1; ++k) {
-                        if (k == s) continue;
-                        const float2 pj = img[k];
-                        const float dx = pix - pj.x, dy = piy - pj.y;
+                // the row's three cells (periodic in x) as one run of candidates:
+                // cell -1 [a0, a0 + na), cell 0 [b0, ...), cell +1 [c0, ...)
+                const int xl = cx == 0 ? g.ncx - 1 : cx - 1;
+                const int xr = cx == g.ncx - 1 ? 0 : cx + 1;
+                const uint32_t a0 = P.cs[rowbase + xl], a1 = P.cs[rowbase + xl + 1];
+                const uint32_t b0 = P.cs[rowbase + cx], b1 = P.cs[rowbase + cx + 1];
+                const uint32_t c0 = P.cs[rowbase + xr], c1 = P.cs[rowbase + xr + 1];
+                const uint32_t na = a1 - a0, nab = na + (b1 - b0), nt = nab + (c1 - c0);
+                // candidate t of the run is slot t + off(t): off = a0, b0 - na or
+                // c0 - nab by the cell it falls in (selects, no branches)
+                const uint32_t off1 = a0, off2 = b0 - na, off3 = c0 - nab;
+                // four candidates per pass, their loads in flight together
+                for (uint32_t t = 0; t < nt; t += 4) {
+                    uint32_t k[4];
+                    int ox[4];
+                    bool ok[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t tt = t + static_cast<uint32_t>(u);
+                        const bool g1 = tt >= na, g2 = tt >= nab;
+                        const uint32_t kk = tt + (g2 ? off3 : (g1 ? off2 : off1));
+                        ok[u] = tt < nt && kk != s;
+                        k[u] = tt < nt ? kk : s;   // past the run: the particle itself, skipped
+                        ox[u] = static_cast<int>(g1) + static_cast<int>(g2) - 1;
+                    }
+                    float2 pj[4];
+                    int cj[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        pj[u] = img[k[u]];
+                        cj[u] = P.icls[k[u]];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float pix = pi.x - static_cast<float>(ox[u]) * g.wf;
+                        const float dx = pix - pj[u].x, dy = piy - pj[u].y;
                         const float rr = __builtin_fmaf(dx, dx, dy * dy);
-                        if (rr < rl2) {
-                            near = near || rr < rg2;
-                            if (cnt - base < static_cast<uint32_t>(kListStage))
-                                stage[cnt - base][tid] = list_entry(k, ox, oy, P.icls[k] == ci);
-                            ++cnt;
-                        }
+                        const bool acc = ok[u] && rr < rl2;
+                        near = near || (acc && rr < rg2);
+                        // unconditional store: a rejected candidate, or an entry
+                        // outside this pass's rows, goes to the spare row
+                        const uint32_t row = acc && cnt - base < static_cast<uint32_t>(kListStage)
+                                                 ? cnt - base
+                                                 : static_cast<uint32_t>(kListStage);
+                        stage[row][tid] = list_entry(k[u], ox[u], oy, cj[u] == ci);
+                        cnt += acc ? 1u : 0u;
                     }
                 }
             }
@@ -1173,7 +1221,28 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P) {
         }
         bool by_list = false;
         if constexpr (!kFp64) {
-            if (sc.list_on && !sc.list_overflow) {
+            if (sc.fallback) {   // lean step the list does not cover: 5x5 cells
+                // (5 distinct cells per row and column, periodic; else the
+                // forces are not exact and the run reports it)
+                if ((g.ncx < 5 || g.ncy < 5 || !g.periodic_y) && li == 0) sc.violated = 1;
+                const char *imgb = static_cast<const char *>(P.img);
+                for (int oy = -2; oy <= 2; ++oy) {
+                    int yy = cy + oy;
+                    if (yy < 0) yy += g.ncy;
+                    else if (yy >= g.ncy) yy -= g.ncy;
+                    const int rowbase = sp.coff + yy * g.ncx;
+                    const float piy = pi.y - static_cast<float>(oy) * g.hf;
+                    for (int ox = -2; ox <= 2; ++ox) {
+                        int xx = cx + ox;
+                        if (xx < 0) xx += g.ncx;
+                        else if (xx >= g.ncx) xx -= g.ncx;
+                        const uint32_t k0 = P.cs[rowbase + xx], k1 = P.cs[rowbase + xx + 1];
+                        pair_cell_f32<ACC, OBS, UNI, LPP>(imgb, P.icls, k0, k1, pi.x - static_cast<float>(ox) * g.wf,
+                                                          piy, ci, g, pt, pc, F, ob, want_obs);
+                    }
+                }
+                by_list = true;
+            } else if (sc.list_on && !sc.list_overflow) {
                 pair_list_f32<ACC, OBS, UNI, LPP>(P, s, pi, ci, g, pt, pc, F, ob, want_obs);
                 by_list = true;
             }

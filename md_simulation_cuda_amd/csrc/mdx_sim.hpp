@@ -96,8 +96,12 @@ struct DevScalars {
     int32_t rebuild;        // this step rebuilds the cell list (always outside list mode)
     int32_t list_built;     // the cell list was built from key_list (a list-mode build); nlist and x0 match it
     int32_t list_overflow;  // a particle had more than list_cap partners: list mode off
-    int32_t violated;       // a lean step ran with a particle beyond skin / 2 (or without a list)
-    int32_t pad1;
+    // a lean step with a particle beyond skin / 2 (or without a usable list):
+    // its pair pass takes the 5x5 cells of the last cell list instead
+    int32_t fallback;
+    // a lean step with a particle beyond the reach of that 5x5 pass
+    // ((2 w - rc_max) / 2), or a grid too small for it: its forces are not exact
+    int32_t violated;
 };
 
 // Pointers of a batch, passed by value to every kernel.

@@ -133,16 +133,16 @@ def _acc_error(got, Lx, Ly, kw):
     return err[~band].max() / rms, np.sqrt((err ** 2).mean()) / rms
 
 
-@pytest.mark.parametrize("T,skin", [(0.8, 0.5), (2.0, 0.3)])
-def test_windows_keep_the_exact_pair_set(T, skin):
-    """Rebuild windows of 16 steps; at T=2 with skin 0.3 particles leave the
-    skin inside most windows, so the redo path runs."""
+@pytest.mark.parametrize("T,skin,window", [(0.8, 0.5, 8), (2.0, 0.3, 16), (2.0, 0.3, 32)])
+def test_windows_keep_the_exact_pair_set(T, skin, window):
+    """Rebuild windows; at T=2 with skin 0.3 particles leave the skin inside
+    most 16- and 32-step windows, so the lean steps' 5x5 fallback runs."""
     f, Lx, Ly = lattice_frame(12000, 0.75, seed=41, jitter=0.05, T=T, aspect=2.0)
     kw = dict(eps_ab=0.5, dt=0.005, T_target=T, Q=10.0)
-    say(f"windows T={T} skin={skin}")
+    say(f"windows T={T} skin={skin} window={window}")
     s = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
-    s.set_tuning(1.0, 1, skin, 16)
-    assert s.get_tuning().window == 16
+    s.set_tuning(1.0, 1, skin, window)
+    assert s.get_tuning().window == window
     s.step(abi.NVT, 160)
     got = s.collect()
     s.close()
@@ -175,7 +175,7 @@ def test_windows_nve_drift_against_oracle():
     assert d <= 2.0 * d_o
 
 
-def test_windows_same_alone_and_in_batch_with_a_redoing_neighbour():
+def test_windows_same_alone_and_in_batch_with_a_falling_back_neighbour():
     cold = lattice_frame(3000, 0.75, seed=51, jitter=0.05, T=0.4, aspect=2.0)
     hot = lattice_frame(2000, 0.75, seed=52, jitter=0.05, T=2.5)
     kws = [dict(eps_ab=0.5, dt=0.005, T_target=0.4, Q=10.0), dict(eps_ab=0.7, dt=0.005, T_target=2.5, Q=10.0)]
