@@ -296,11 +296,14 @@ int Batch::launch_step_kernels(bool obs, int mode) {
     // a window's head step keeps lists only if they last its window - 1 lean steps
     const int ahead = mode == STEP_HEAD ? std::max(window - 1, 0) : 0;
     k_box<<<S, kBlock, 0, stream>>>(ptrs, mode);
-    if (TB > 0) {
+    if (TB > 0 && lean) {   // drift and image in one pass (windows run in the FP32 forms only)
+        if (prec == MDX_PREC_COMP32) k_lean<float2><<<TB, kBlock, 0, stream>>>(ptrs);
+        else k_lean<double2><<<TB, kBlock, 0, stream>>>(ptrs);
+    } else if (TB > 0) {
         if (prec == MDX_PREC_COMP32)
-            k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs, lean ? 1 : 0, ahead);
+            k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs, 0, ahead);
         else
-            k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs, lean ? 1 : 0, ahead);
+            k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs, 0, ahead);
     }
     if (!lean) {   // cell list (and, in list mode, neighbour lists) of the slots that rebuild
         if (TCB + TB > 0) k_stay_arrive<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
@@ -312,12 +315,12 @@ int Batch::launch_step_kernels(bool obs, int mode) {
         if (TCB + TB > 0) k_fill<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
         if (TCB > 0) k_cell_sort<<<TCB, kBlock, 0, stream>>>(ptrs);
     }
-    if (TB > 0) {
+    if (TB > 0 && !lean) {
         if (prec == MDX_PREC_FP64)
             k_slot_image<double2><<<TB, kBlock, 0, stream>>>(ptrs);
         else
             k_slot_image<float2><<<TB, kBlock, 0, stream>>>(ptrs);
-        if (!lean && nlist && prec != MDX_PREC_FP64) k_list_build<<<TB, kBlock, 0, stream>>>(ptrs);
+        if (nlist && prec != MDX_PREC_FP64) k_list_build<<<TB, kBlock, 0, stream>>>(ptrs);
     }
     if (TB > 0) {
         const bool timed = timing;
@@ -1057,7 +1060,7 @@ int Batch::autotune(int steps, mdx_tune_report *rep, int kind_override) {
     if (prec != MDX_PREC_FP64)
         for (double sk : {0.3, 0.5, 0.8})
             for (int l : {1, 2})
-                for (int w : {0, 4, 8}) cands.push_back(mdx_tuning{1.0, sk, l, w});
+                for (int w : {0, 8, 16}) cands.push_back(mdx_tuning{1.0, sk, l, w});
     double best = 1e300;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MDX_HIP(hipEventCreate(&e0));

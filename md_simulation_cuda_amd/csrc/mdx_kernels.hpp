@@ -79,6 +79,8 @@ enum StepMode : int32_t {
                         // must also hold through the window's lean steps
 };
 
+__device__ inline double nh_half_update(double xi, double K, const SysParams &sp);
+
 __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
     const int r = blockIdx.x;
     const SysParams &sp = P.sp[r];
@@ -171,6 +173,16 @@ __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
         // per row and column
         sc.rebuild = 0;
         sc.fallback = (sp.kind != SK_NONE && (!on || !sc.list_built)) ? 1 : 0;
+        if (sp.kind == SK_NVT) {
+            // the lean step's first kick is v' = c v + (dt/2) a with
+            // c = 1 - dt xi1 / 2, so K(v') = c^2 K + (c dt / 2) sum m v.a
+            // + (dt^2 / 4) sum m |a|^2 / 2, from the last pair pass's sums
+            const double xi1 = nh_half_update(sc.xi, sc.K, sp);
+            const double c = 1.0 - 0.5 * sp.dt * xi1;
+            const double Kh = c * c * sc.K + 0.5 * c * sp.dt * sc.S_va + 0.25 * sp.dt * sp.dt * sc.S_aa;
+            sc.K_half = Kh;
+            sc.xi_mid = nh_half_update(xi1, Kh, sp);
+        }
     } else {
         sc.rebuild = (!on || !sc.list_built || mode == STEP_REBUILD) ? 1 : 0;
         sc.fallback = 0;
@@ -592,6 +604,76 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
     }
 }
 
+// Lean step (list mode, NVE / NVT / idle slots of a rebuild window): the
+// first kick, drift and wrap of k_drift and the image of k_slot_image in one
+// pass over the cell-ordered slots of the last cell list.  No keys or movers
+// (no cell-list kernels run this step) and no drift reduction: k_box has set
+// xi_mid from the last pair pass's sums.  A particle beyond skin / 2 of its
+// build position makes the step's pair pass fall back to the 5x5 cells;
+// beyond (2 min(w, h) - rc_max) / 2 the step is flagged as not exact.
+template <typename AT>
+__global__ void __launch_bounds__(kBlock) k_lean(BatchPtrs P) {
+    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int r = P.blk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    DevScalars &sc = P.sc[r];
+    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
+    if (li >= sp.n) return;
+    const Grid g = sc.g;
+    const uint32_t k = P.cs[sp.coff] + static_cast<uint32_t>(li);
+    const uint32_t id = P.sorted[k];
+    const uint32_t lc = P.scell[k];
+    double2 p = P.x[id];
+    const int kind = sp.kind;
+    if (kind == SK_NVE || kind == SK_NVT) {
+        const double dt = sp.dt;
+        double2 q = P.v[id];
+        const double2 acc = load_acc<AT>(P.a, id);
+        if (kind == SK_NVE) {   // src/md_cuda_common.cu:381-394
+            q.x += 0.5 * dt * acc.x;
+            q.y += 0.5 * dt * acc.y;
+            p.x += dt * q.x;
+            p.y += dt * q.y;
+        } else {                // src/md_cuda_common.cu:454-478
+            const double xi1 = nh_half_update(sc.xi, sc.K, sp);
+            double vx = q.x, vy = q.y;
+            vx += 0.5 * dt * (acc.x - xi1 * vx);
+            vy += 0.5 * dt * (acc.y - xi1 * vy);
+            q.x = vx;
+            q.y = vy;
+            p.x += dt * vx;
+            p.y += dt * vy;
+        }
+        p.x = pbc_wrap(p.x, g.Lx);
+        p.y = pbc_wrap(p.y, g.Ly);
+        P.v[id] = q;
+        P.x[id] = p;
+        if (sc.list_valid) {
+            const double2 p0 = P.x0[id];
+            double dx = p.x - p0.x, dy = p.y - p0.y;
+            if (dx > 0.5 * g.Lx) dx -= g.Lx;
+            else if (dx < -0.5 * g.Lx) dx += g.Lx;
+            if (dy > 0.5 * g.Ly) dy -= g.Ly;
+            else if (dy < -0.5 * g.Ly) dy += g.Ly;
+            const double d2 = dx * dx + dy * dy;
+            if (d2 > sp.skin_half2) sc.fallback = 1;
+            const double reach = 0.5 * (2.0 * fmin(g.w, g.h) - sp.rc_max);
+            if (d2 > reach * reach) sc.violated = 1;
+        }
+    }
+    // image relative to the centre of the particle's cell in the last cell
+    // list, minimum image (the particle may have crossed the box edge since)
+    const uint32_t cy = lc / static_cast<uint32_t>(g.ncx);
+    const uint32_t cx = lc - cy * static_cast<uint32_t>(g.ncx);
+    double dx = p.x - (static_cast<double>(cx) + 0.5) * g.w;
+    double dy = p.y - (static_cast<double>(cy) + 0.5) * g.h;
+    if (dx > 0.5 * g.Lx) dx -= g.Lx;
+    else if (dx < -0.5 * g.Lx) dx += g.Lx;
+    if (dy > 0.5 * g.Ly) dy -= g.Ly;
+    else if (dy < -0.5 * g.Ly) dy += g.Ly;
+    static_cast<float2 *>(P.img)[k] = make_float2(static_cast<float>(dx), static_cast<float>(dy));
+}
+
 // Neighbour-list entry (list mode): (partner slot << 3), so that entry &
 // kListOff is the byte offset of the partner's FP32 image; bit 2 set when the
 // partner has the particle's own type class; the neighbour cell's offset ox in
@@ -815,6 +897,8 @@ __device__ inline void finalize_pair(int pmode, const double (&out)[3], const Sy
         break;
     case PM_NVT:
         sc.K = out[0];
+        sc.S_va = out[1];
+        sc.S_aa = out[2];
         sc.xi = sc.xi_mid;
         break;
     case PM_NPH: {   // src/md_env.cu:1896-1917
@@ -1345,6 +1429,8 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P) {
             q.y = vy;
             P.v[id] = q;
             r0 = 0.5 * pt.m[ci] * (vx * vx + vy * vy);
+            r1 = pt.m[ci] * (vx * axd + vy * ayd);
+            r2v = 0.5 * pt.m[ci] * (axd * axd + ayd * ayd);
         } else if (pmode == PM_FIRE) {
             const double fdt = sc.f_dt;
             double2 q = P.v[id];

@@ -82,6 +82,9 @@ struct DevScalars {
     double xi_mid;       // xi of the step in flight (after both half updates)
     double K;            // kinetic energy of the current velocities
     double K_half;       // kinetic energy after the first kick
+    // sums of the last Nose-Hoover pair pass: sum m v.a and sum m |a|^2 / 2 at
+    // the stored accelerations, from which a lean step gets K_half in closed form
+    double S_va, S_aa;
     double W, U, lam, wallE;
     double p_A, gamma, nph_lin, nph_vsc, P_old;
     double v_p, F_p, last_P;
