@@ -112,7 +112,7 @@ int mdx_create(const mdx_params *params, const mdx_particle64 *frame, int64_t n,
     if (!params || !frame || !out) return set_error(MDX_ERR_INVALID, "null argument");
     *out = nullptr;
     if (n <= 0) return set_error(MDX_ERR_INVALID, "n must be positive");
-    if (n > (int64_t{1} << 30)) return set_error(MDX_ERR_CAPACITY, "n above 2^30");
+    if (n > (int64_t{1} << 29) - 256) return set_error(MDX_ERR_CAPACITY, "n above 2^29 - 256");
     if (!(params->Lx > 0.0) || !(params->Ly > 0.0) || !std::isfinite(params->Lx) || !std::isfinite(params->Ly))
         return set_error(MDX_ERR_BOX, "box sides must be positive and finite");
     if (!(params->cutoff > 0.0) || !std::isfinite(params->cutoff))

@@ -165,8 +165,9 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
         coff += ccap[r];
         boff += sp.nblk;
         cboff += sp.ncblk;
-        if (off > INT32_MAX - kBlock || coff > INT32_MAX / 2)
-            return set_error(MDX_ERR_CAPACITY, "batch above 2^31 particles or 2^30 cells");
+        // FP32 images are addressed with 32-bit byte offsets (slot << 3)
+        if (off > (int64_t{1} << 29) - kBlock || coff > INT32_MAX / 2)
+            return set_error(MDX_ERR_CAPACITY, "batch above 2^29 particles or 2^30 cells");
     }
     N = off;
     C = static_cast<int32_t>(coff);
