@@ -244,7 +244,8 @@ int mdx_set_tuning(mdx_sim *sim, const mdx_tuning *t);
 int mdx_get_tuning(mdx_sim *sim, mdx_tuning *out);
 /* Autotuner: on a copy of the system, times `steps` steps of ensemble `kind`
  * for every candidate (without lists: cell_scale 1.0, 1.1 x lanes 1, 2, 4;
- * with lists: skin 0.3, 0.5, 0.8 x lanes 1, 2 x window 0, 8, 16; FP64 form: cell
+ * with lists: skin 0.3, 0.5, 0.65, 0.8 x window 0, 8, 12, 16, 20 with one lane, and
+ * skin 0.5, 0.8 with two lanes and on-demand rebuilds; FP64 form: cell
  * list, one lane), then applies the fastest to the handle.  The handle's
  * state is not advanced. */
 int mdx_autotune(mdx_sim *sim, int32_t kind, int32_t steps, mdx_tune_report *out);

@@ -1058,10 +1058,11 @@ int Batch::autotune(int steps, mdx_tune_report *rep, int kind_override) {
     std::vector<mdx_tuning> cands;
     for (double cs : {1.0, 1.1})
         for (int l : {1, 2, 4}) cands.push_back(mdx_tuning{cs, 0.0, l, 0});
-    if (prec != MDX_PREC_FP64)
-        for (double sk : {0.3, 0.5, 0.8})
-            for (int l : {1, 2})
-                for (int w : {0, 8, 16}) cands.push_back(mdx_tuning{1.0, sk, l, w});
+    if (prec != MDX_PREC_FP64) {
+        for (double sk : {0.3, 0.5, 0.65, 0.8})
+            for (int w : {0, 8, 12, 16, 20}) cands.push_back(mdx_tuning{1.0, sk, 1, w});
+        for (double sk : {0.5, 0.8}) cands.push_back(mdx_tuning{1.0, sk, 2, 0});
+    }
     double best = 1e300;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MDX_HIP(hipEventCreate(&e0));
