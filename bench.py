@@ -132,27 +132,28 @@ def run_gpu(frame, Lx, Ly, kw, prec, steps, warmup, tune_steps, fixed_tuning=Non
 
 
 def pair_traffic(tune):
-    """HBM bytes per pair-kernel launch at this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same kernel variant
-    (scaled by the copy-kernel calibration), or None when no summary matches."""
-    skin = (tune or {}).get("skin", 0.0)
-    lanes = (tune or {}).get("lanes_per_particle", 1)
-    name = "nvt_1m_list_pair_traffic.md" if skin > 0 else "nvt_1m_cell_path_pair_traffic.md"
-    path = ROOT / "profiles" / name
-    if not path.exists():
-        return None
-    vals = {}
-    for line in path.read_text().splitlines():
-        if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "skin", "lanes_per_particle"):
-            k, v = line.split(":", 1)
-            vals[k] = float(v)
-    if "traffic_bytes_per_launch" not in vals:
-        return None
-    if skin > 0 and (vals.get("skin") != skin or vals.get("lanes_per_particle") != lanes):
-        return None
-    if skin == 0 and lanes != 1:
-        return None
-    return vals["traffic_bytes_per_launch"]
+    """HBM bytes per pair-kernel launch at this workload, from a committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/*_pair_traffic.md,
+    scaled by the copy-kernel calibration) of the same kernel variant and
+    tuning, or None when no summary matches."""
+    skin = float((tune or {}).get("skin", 0.0))
+    lanes = int((tune or {}).get("lanes_per_particle", 1))
+    window = int((tune or {}).get("window", 0) or 0)
+    for path in sorted((ROOT / "profiles").glob("nvt_1m_*_pair_traffic.md")):
+        vals = {}
+        for line in path.read_text().splitlines():
+            if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "skin", "lanes_per_particle",
+                                                        "window"):
+                k, v = line.split(":", 1)
+                vals[k] = float(v)
+        if "traffic_bytes_per_launch" not in vals:
+            continue
+        if vals.get("skin", 0.0) != skin or int(vals.get("lanes_per_particle", 1)) != lanes:
+            continue
+        if skin > 0 and int(vals.get("window", 0)) != window:
+            continue
+        return vals["traffic_bytes_per_launch"]
+    return None
 
 
 def accuracy(frame, Lx, Ly, kw):

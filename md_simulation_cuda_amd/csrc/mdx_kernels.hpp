@@ -1056,17 +1056,17 @@ __device__ __forceinline__ void list_pair_f32(uint32_t v, float2 q, float pix, f
     const float t = e24 * __builtin_fmaf(2.0f, sr12, -sr6) * r2i;
     const bool in = GUARD ? (rr < rc2 && rr >= pc.r2_guard) : rr < rc2;
     const float tmp = in ? t : 0.0f;
-    const float fx = tmp * dx, fy = tmp * dy;
-    if constexpr (ACC == ACC_COMP32) {
-        fxc += fx;
-        fyc += fy;
+    if constexpr (ACC == ACC_COMP32) {   // the run's FP32 sum, one rounding per term
+        fxc = __builtin_fmaf(tmp, dx, fxc);
+        fyc = __builtin_fmaf(tmp, dy, fyc);
     } else {
-        F.add(fx, fy);
+        F.add(tmp * dx, tmp * dy);
     }
     if constexpr (GUARD)
         slow = slow || (__float_as_uint(rr) - 1u < __float_as_uint(pc.r2_guard) - 1u);
     if constexpr (OBS) {
         if (want_obs && in) {
+            const float fx = tmp * dx, fy = tmp * dy;
             const float e4 = same ? pc.e4s : pc.e4o;
             const float uu = e4 * (sr12 - sr6);
             const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
