@@ -106,7 +106,7 @@ Batch::~Batch() {
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
                      d_sp, d_sc, d_blk2sys, d_cblk2sys, d_want, sorted_old, cs_old, movers, mover_off,
-                     n_movers, arrivals, key_list, x0, nlist, ncount, x_bak, v_bak, a_bak, sc_bak};
+                     n_movers, arrivals, key_list, x0, img0, nlist, ncount, x_bak, v_bak, a_bak, sc_bak};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
     for (auto *vec : {&force_ev, &step_ev})
@@ -957,11 +957,13 @@ int Batch::ensure_list_arrays() {
     const size_t n = static_cast<size_t>(std::max<int64_t>(N, 1));
     MDX_HIP(hipMalloc(&key_list, n * 4));
     MDX_HIP(hipMalloc(&x0, n * sizeof(double2)));
+    MDX_HIP(hipMalloc(&img0, n * sizeof(float2)));
     MDX_HIP(hipMalloc(&ncount, n * 4));
     MDX_HIP(hipMalloc(&nlist, n * static_cast<size_t>(list_cap) * 4));
     MDX_HIP(hipMemset(ncount, 0, n * 4));
     ptrs.key_list = key_list;
     ptrs.x0 = x0;
+    ptrs.img0 = img0;
     ptrs.ncount = ncount;
     ptrs.nlist = nlist;
     invalidate_graphs();   // the step sequence gains the list build

@@ -82,6 +82,7 @@ struct Batch {
     DevScalars *sc_bak = nullptr;
     uint32_t *key_list = nullptr, *nlist = nullptr, *ncount = nullptr;
     double2 *x0 = nullptr;
+    float2 *img0 = nullptr;
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> force_ev, step_ev;

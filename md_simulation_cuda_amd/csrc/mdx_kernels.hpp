@@ -589,7 +589,9 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
             if (dy > 0.5 * g.Ly) dy -= g.Ly;
             else if (dy < -0.5 * g.Ly) dy += g.Ly;
         }
-        img[k] = make_float2(static_cast<float>(dx), static_cast<float>(dy));
+        const float2 im = make_float2(static_cast<float>(dx), static_cast<float>(dy));
+        img[k] = im;
+        if (reb && sp.list_mode) P.img0[k] = im;   // build-time image, for the lean steps' check
     } else {
         img[k] = p;
     }
@@ -648,18 +650,6 @@ __global__ void __launch_bounds__(kBlock) k_lean(BatchPtrs P) {
         p.y = pbc_wrap(p.y, g.Ly);
         P.v[id] = q;
         P.x[id] = p;
-        if (sc.list_valid) {
-            const double2 p0 = P.x0[id];
-            double dx = p.x - p0.x, dy = p.y - p0.y;
-            if (dx > 0.5 * g.Lx) dx -= g.Lx;
-            else if (dx < -0.5 * g.Lx) dx += g.Lx;
-            if (dy > 0.5 * g.Ly) dy -= g.Ly;
-            else if (dy < -0.5 * g.Ly) dy += g.Ly;
-            const double d2 = dx * dx + dy * dy;
-            if (d2 > sp.skin_half2) sc.fallback = 1;
-            const double reach = 0.5 * (2.0 * fmin(g.w, g.h) - sp.rc_max);
-            if (d2 > reach * reach) sc.violated = 1;
-        }
     }
     // image relative to the centre of the particle's cell in the last cell
     // list, minimum image (the particle may have crossed the box edge since)
@@ -671,7 +661,19 @@ __global__ void __launch_bounds__(kBlock) k_lean(BatchPtrs P) {
     else if (dx < -0.5 * g.Lx) dx += g.Lx;
     if (dy > 0.5 * g.Ly) dy -= g.Ly;
     else if (dy < -0.5 * g.Ly) dy += g.Ly;
-    static_cast<float2 *>(P.img)[k] = make_float2(static_cast<float>(dx), static_cast<float>(dy));
+    const float2 im = make_float2(static_cast<float>(dx), static_cast<float>(dy));
+    static_cast<float2 *>(P.img)[k] = im;
+    if ((kind == SK_NVE || kind == SK_NVT) && sc.list_valid) {
+        // displacement since the build: the two images share the build cell's
+        // centre (FP32; the thresholds are lowered by 1e-5 relative so that a
+        // rounding near them errs towards the fallback)
+        const float2 i0 = P.img0[k];
+        const float ex = im.x - i0.x, ey = im.y - i0.y;
+        const float d2 = __builtin_fmaf(ex, ex, ey * ey);
+        if (d2 > static_cast<float>(sp.skin_half2) * (1.0f - 1.0e-5f)) sc.fallback = 1;
+        const float reach = 0.5f * static_cast<float>(2.0 * fmin(g.w, g.h) - sp.rc_max);
+        if (d2 > reach * reach * (1.0f - 1.0e-5f)) sc.violated = 1;
+    }
 }
 
 // Neighbour-list entry (list mode): (partner slot << 3), so that entry &

@@ -127,6 +127,7 @@ struct BatchPtrs {
     // (partner slot | cell offset code << 28) and its length ncount[s]
     uint32_t *key_list;
     double2 *x0;
+    float2 *img0;     // per cell-ordered slot: its FP32 image at the last list-mode build
     uint32_t *nlist, *ncount;
     const SysParams *sp;
     DevScalars *sc;
