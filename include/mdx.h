@@ -306,10 +306,13 @@ int mdx_batch_autotune(mdx_batch *b, int32_t steps, mdx_tune_report *out);
  * admitted slots within hbm_budget_bytes, arrival order without head-of-line
  * blocking), advances every running job by up to `quantum` steps in the same
  * launches, records K/U/P of every running MD job, and retires finished jobs
- * (final frame copied to the host, slot freed).  MD jobs run in a batch of
- * md_precision, FIRE jobs in an ACC64 batch; both batches' launches overlap on
- * the device.  A job's result is the same as the same system run alone through
- * mdx_create + mdx_step (or mdx_fire with fp64_switch = 0). */
+ * (final frame copied to the host, slot freed).  NVE and NVT jobs run in a
+ * batch of md_precision with neighbour lists (skin list_skin, rebuilt on
+ * demand), NPH and piston jobs in a batch of md_precision with the cell list
+ * every step, FIRE jobs in an ACC64 batch; the batches' launches overlap on the
+ * device.  A job's result is the same as the same system run alone through
+ * mdx_create + mdx_step (NVE / NVT with mdx_set_tuning {1.0, list_skin, 1, 0}
+ * in the FP32 forms) or mdx_fire with fp64_switch = 0. */
 #define MDX_FIRE 4            /* mdx_job.kind for a FIRE minimisation */
 #define MDX_JOB_QUEUED 0
 #define MDX_JOB_RUNNING 1
@@ -325,8 +328,12 @@ typedef struct mdx_server_params {
     int32_t fire_slots_per_class; /* FIRE slots per size class (2)                       */
     int32_t quantum;              /* most steps per tick (100)                           */
     int32_t record_energies;      /* 1: K/U/P of every running MD job every tick         */
-    int32_t md_precision;         /* mdx_precision of the MD batch (COMP32)              */
+    int32_t md_precision;         /* mdx_precision of the MD batches (COMP32)            */
     int32_t device;               /* -1: current                                         */
+    int32_t list_slots_per_class; /* NVE / NVT slots per size class, in a batch with
+                                     neighbour lists (6; 0: they share the MD slots)     */
+    int32_t pad0;
+    double list_skin;             /* neighbour-list skin of those slots (0.5)           */
 } mdx_server_params;
 
 typedef struct mdx_job {

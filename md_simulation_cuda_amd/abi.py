@@ -118,7 +118,8 @@ class ServerParams(C.Structure):
         ("min_class_log2", C.c_int32), ("max_class_log2", C.c_int32),
         ("md_slots_per_class", C.c_int32), ("fire_slots_per_class", C.c_int32),
         ("quantum", C.c_int32), ("record_energies", C.c_int32), ("md_precision", C.c_int32),
-        ("device", C.c_int32),
+        ("device", C.c_int32), ("list_slots_per_class", C.c_int32), ("pad0", C.c_int32),
+        ("list_skin", C.c_double),
     ]
 
 

@@ -162,7 +162,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
         for (int b = 0; b < sp.nblk; ++b) b2s.push_back(r);
         for (int b = 0; b < sp.ncblk; ++b) cb2s.push_back(r);
         off += n_cap[r];
-        coff += ccap[r];
+        coff += ccap[r] + 1;   // + the slot's padding cell (mdx_sim.hpp)
         boff += sp.nblk;
         cboff += sp.ncblk;
         // FP32 images are addressed with 32-bit byte offsets (slot << 3)
@@ -364,6 +364,10 @@ int Batch::launch_step_kernels(bool obs, int mode) {
 
 // count passes of the step sequence; replayed from a captured hipGraph unless
 // kernel timing is on.
+bool Batch::list_kind(int kind) const {
+    return nlist && skin > 0.0 && prec != MDX_PREC_FP64 && (kind == SK_NVE || kind == SK_NVT);
+}
+
 bool Batch::uniform_sigma() const {
     for (const Slot &s : slots)
         if (s.loaded && !(s.p.sigma_aa == s.p.sigma_bb && s.p.sigma_aa == s.p.sigma_ab)) return false;
@@ -624,14 +628,22 @@ int Batch::walls(const std::vector<int> &which) {
 
 // One pass at the current positions: slots in `which` rebuild their cell list
 // with the given y boundary and run pair mode pmode; the others idle.
+// A pass at the current positions for the slots in `which`.  Energy passes
+// (PM_OBS) of NVE / NVT slots in list mode use the neighbour lists and rebuild
+// them, as a step does, only when a particle has moved beyond skin / 2 since
+// the last build: so when energies are sampled does not move the rebuild
+// points of the trajectory (the cell list is canonical).  Other passes rebuild
+// the cell list at the current positions.
 static int one_pass(Batch &b, const std::vector<int> &which, int periodic_y, int pmode) {
     for (size_t r = 0; r < b.slots.size(); ++r) set_idle(b.h_sp[r], b.slots[r]);
     for (int r : which) {
         SysParams &sp = b.h_sp[r];
+        const Slot &sl = b.slots[r];
         sp.kind = SK_KEYS;
         sp.periodic_y = periodic_y;
         sp.pmode = pmode;
         sp.wrap_keys = 1;
+        sp.list_mode = (pmode == PM_OBS && b.list_kind(sl.kind)) ? 1 : 0;
     }
     MDX_TRY(b.upload_all_sp());
     const bool obs = pmode == PM_OBS;
@@ -736,7 +748,7 @@ int Batch::step(int nsteps) {
         case SK_FIRE: sp.pmode = PM_FIRE; break;
         default: return set_error(MDX_ERR_INVALID, "unknown ensemble");
         }
-        sp.list_mode = (nlist && prec != MDX_PREC_FP64 && (s.kind == SK_NVE || s.kind == SK_NVT)) ? 1 : 0;
+        sp.list_mode = list_kind(s.kind) ? 1 : 0;
         sp.list_cap = list_cap;
         sp.skin_half2 = 0.25 * skin * skin;
         sp.r_list = s.rc_max + skin;
@@ -1013,6 +1025,9 @@ int Batch::set_tuning(const mdx_tuning &t) {
         DevScalars h;   // a list that overflowed gets another chance under the new tuning
         MDX_TRY(read_scalars(static_cast<int>(r), h));
         h.list_overflow = 0;
+        // list_built stays: it names the keys the cell list was built from
+        // (the movers' reference), and the force pass below rebuilds the cell
+        // list, which clears it, so the next list-mode step builds new lists
         MDX_TRY(write_scalar_block(static_cast<int>(r), h));
         (s.piston_active ? pis : per).push_back(static_cast<int>(r));
     }

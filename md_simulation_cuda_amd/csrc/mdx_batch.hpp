@@ -143,6 +143,10 @@ struct Batch {
     int obs_pass(int r, int periodic_y);
     void invalidate_graphs();
     bool uniform_sigma() const;   // every loaded slot has one sigma for all type pairs
+    // a slot of this ensemble kind steps in list mode: a positive skin, list
+    // arrays, an FP32 form and NVE or NVT (skin 0 is the cell list every step,
+    // also after an earlier tuning allocated the lists)
+    bool list_kind(int kind) const;
     double cell_side_for(const Slot &s) const;   // cell side of a slot under the batch's tuning
     int ensure_list_arrays();
     int write_scalar_block(int r, const DevScalars &h);

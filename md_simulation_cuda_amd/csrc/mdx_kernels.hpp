@@ -85,9 +85,28 @@ __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
     const int r = blockIdx.x;
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
-    if (sp.n <= 0) return;
+    // the padding cell anchors the slot's cell starts at its first particle
+    if (threadIdx.x == 0) P.count[sp.coff + sp.ccap] = static_cast<uint32_t>(sp.n > 0 ? sp.n_cap - sp.n : sp.n_cap);
+    if (sp.n <= 0) {   // unloaded: the cell-list kernels clear every cell of the slot
+        if (threadIdx.x == 0) {
+            sc.rebuild = 1;
+            sc.list_valid = 0;
+            sc.list_built = 0;
+            sc.list_on = 0;
+            sc.fallback = 0;
+        }
+        return;
+    }
     if (sc.pair_pending && sp.nblk > 0) finalize_pair_pending(P, sp, sc);
     if (threadIdx.x != 0) return;
+    if (sp.kind == SK_NONE && sc.list_valid) {
+        // an idle slot keeps its cell list (and neighbour lists) as they are:
+        // its cells' counts hold their occupancy, so the scan keeps its cell
+        // starts, and its next step continues from the same lists
+        sc.rebuild = 0;
+        sc.fallback = 0;
+        return;
+    }
     double Lx = sc.g.Lx, Ly = sc.g.Ly;
     if (sp.kind == SK_NPH) {
         const double area_old = Lx * Ly;
@@ -226,9 +245,10 @@ __device__ inline void append_movers(const BatchPtrs &P, bool mover, uint32_t pi
 
 // Kick / drift / wrap of one particle by its slot's ensemble, then its cell
 // key.  Restates src/md_cuda_common.cu:381-496 (VV, NH, NPH scale+drift,
-// piston) and src/md_energy_minimizer.cu:32-71 (FIRE zero/mix/drift).  Slots
-// that do not move this step (idle, converged FIRE) still recompute keys so
-// the cell list is rebuilt unchanged.
+// piston) and src/md_energy_minimizer.cu:32-71 (FIRE zero/mix/drift).  Idle
+// slots that keep their cell list skip the kernel; other slots that do not
+// move this step (idle without a cell list, converged FIRE) still recompute
+// keys so the cell list is rebuilt unchanged.
 // lean: a lean step (no cell-list kernels); ahead: a window's head step, whose
 // lists must also hold through `ahead` lean steps: the slot rebuilds now when
 // some particle, moving 1.5 x its current speed, would pass skin / 2 within
@@ -240,6 +260,7 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t lean, int
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
     const int kind = sp.kind;
+    if (kind == SK_NONE && !sc.rebuild) return;   // idle slot keeping its cell list (whole workgroup)
     const int li = (gb - sp.boff) * kBlock + threadIdx.x;
     const bool reduce = (kind == SK_NVT || kind == SK_NPH);
     const Grid g = sc.g;

@@ -2,16 +2,18 @@
 // slots.  No counterpart in the reference, whose drivers (run/*.cpp) run one
 // system per process from start to end.
 //
-// Two batches ("pools") hold fixed slots in power-of-two size classes: MD jobs
-// (NVE, NVT, NPH, piston) in a batch of the server's MD precision, FIRE jobs in
-// an FP64-accelerating batch (FIRE needs accelerations beyond FP32 range at
-// overlapping starts).  Each tick:
+// Three batches ("pools") hold fixed slots in power-of-two size classes: NVE
+// and NVT jobs in a batch of the server's MD precision with neighbour lists
+// (rebuilt on demand, so a job's rebuild points depend only on its own
+// particles), NPH and piston jobs in a batch of that precision with the cell
+// list every step, FIRE jobs in an FP64-accelerating batch (FIRE needs
+// accelerations beyond FP32 range at overlapping starts).  Each tick:
 //   1. admits queued jobs in arrival order into the smallest free slot that
 //      fits them, while the bytes of the admitted slots stay within the HBM
 //      budget (a job that does not fit now does not block later ones);
 //   2. advances every running job by the same number of steps q (at most the
 //      quantum and at most any MD job's remaining steps); the FIRE pool's
-//      launches are queued first so both pools overlap on the device;
+//      launches are queued first so the pools overlap on the device;
 //   3. records K, U, P of every running MD job (one kinetic and one energy
 //      pass per pool) and retires finished jobs: final observables, frame
 //      copied to the host, slot freed.
@@ -39,15 +41,18 @@ using mdx::set_error;
 
 namespace {
 
-constexpr int kPoolMD = 0, kPoolFire = 1;
+constexpr int kPoolMD = 0, kPoolFire = 1, kPoolList = 2, kPools = 3;
 
 // Device bytes one slot of capacity n particles and c cells occupies (the
-// batch's per-particle and per-cell arrays at the given precision).
-double slot_bytes(int64_t n, int32_t c, int prec) {
+// batch's per-particle and per-cell arrays at the given precision, and the
+// neighbour-list arrays when it has them).
+double slot_bytes(int64_t n, int32_t c, int prec, bool lists) {
     const double a = prec == MDX_PREC_COMP32 ? 8.0 : 16.0;
     const double img = prec == MDX_PREC_FP64 ? 16.0 : 8.0;
     // x, v, a, image, class x2, key, sorted, cell of slot, sorted_old, movers
-    const double per_particle = 16.0 + 16.0 + a + img + 2.0 + 4.0 * 5.0;
+    double per_particle = 16.0 + 16.0 + a + img + 2.0 + 4.0 * 5.0;
+    // 64 list entries, list length, key and position and image at the build
+    if (lists) per_particle += 64.0 * 4.0 + 4.0 + 4.0 + 16.0 + 8.0;
     // count, cell_start, cell_start_old, mover offsets
     const double per_cell = 4.0 * 4.0;
     return per_particle * static_cast<double>(n) + per_cell * static_cast<double>(c);
@@ -79,7 +84,8 @@ struct Pool {
 
 struct mdx_server {
     mdx_server_params p{};
-    Pool pools[2];
+    Pool pools[kPools];
+    bool lists = false;   // the NVE / NVT pool has neighbour lists
     std::map<int64_t, Job> jobs;
     std::deque<int64_t> queue;
     int64_t next_id = 1;
@@ -95,9 +101,9 @@ struct mdx_server {
     int init(const mdx_server_params &sp) {
         p = sp;
         t0 = std::chrono::steady_clock::now();
-        const int counts[2] = {p.md_slots_per_class, p.fire_slots_per_class};
-        const int precs[2] = {p.md_precision, MDX_PREC_ACC64};
-        for (int q = 0; q < 2; ++q) {
+        const int counts[kPools] = {p.md_slots_per_class, p.fire_slots_per_class, p.list_slots_per_class};
+        const int precs[kPools] = {p.md_precision, MDX_PREC_ACC64, p.md_precision};
+        for (int q = 0; q < kPools; ++q) {
             Pool &pl = pools[q];
             pl.prec = precs[q];
             for (int k = p.min_class_log2; k <= p.max_class_log2; ++k) {
@@ -118,11 +124,22 @@ struct mdx_server {
             }
             pl.job.assign(pl.cap.size(), -1);
             if (pl.cap.empty()) continue;
-            const int rc = pl.b.init(static_cast<int>(pl.cap.size()), pl.cap.data(), pl.ccap.data(), pl.prec,
-                                     p.device);
+            int rc = pl.b.init(static_cast<int>(pl.cap.size()), pl.cap.data(), pl.ccap.data(), pl.prec, p.device);
             if (rc != MDX_OK) return rc;
+            if (q == kPoolList && p.list_skin > 0.0 && pl.prec != MDX_PREC_FP64) {
+                rc = pl.b.set_tuning(mdx_tuning{1.0, p.list_skin, 1, 0});
+                if (rc != MDX_OK) return rc;
+                lists = true;
+            }
         }
         return MDX_OK;
+    }
+
+    // the pool a job of this kind runs in
+    int pool_of(int32_t kind) const {
+        if (kind == MDX_FIRE) return kPoolFire;
+        if ((kind == MDX_NVE || kind == MDX_NVT) && !pools[kPoolList].cap.empty()) return kPoolList;
+        return kPoolMD;
     }
 
     int submit(const mdx_job &j, const mdx_particle64 *frame, int64_t n, int64_t *id) {
@@ -131,7 +148,7 @@ struct mdx_server {
         const bool fire = j.kind == MDX_FIRE;
         if (!fire && (j.kind < MDX_NVE || j.kind > MDX_PISTON))
             return set_error(MDX_ERR_INVALID, "server: unknown job kind");
-        const Pool &pl = pools[fire ? kPoolFire : kPoolMD];
+        const Pool &pl = pools[pool_of(j.kind)];
         if (pl.cap.empty() || n > pl.cap.back())
             return set_error(MDX_ERR_CAPACITY, "server: job larger than the largest slot");
         Job job;
@@ -160,7 +177,7 @@ struct mdx_server {
     int admit() {
         for (auto it = queue.begin(); it != queue.end();) {
             Job &j = jobs.at(*it);
-            const int pool = j.spec.kind == MDX_FIRE ? kPoolFire : kPoolMD;
+            const int pool = pool_of(j.spec.kind);
             Pool &pl = pools[pool];
             const mdx_params &jp = j.spec.params;
             const int need = mdx::mdx_cell_capacity(jp, std::max(1.0, jp.barostat_height_max_ratio));
@@ -170,7 +187,7 @@ struct mdx_server {
                 ++it;
                 continue;
             }
-            const double bytes = slot_bytes(pl.cap[s], pl.ccap[s], pl.prec);
+            const double bytes = slot_bytes(pl.cap[s], pl.ccap[s], pl.prec, pool == kPoolList && lists);
             if (reserved + bytes > p.hbm_budget_bytes) {
                 ++it;
                 continue;
@@ -235,8 +252,8 @@ struct mdx_server {
             }
         if (any) {
             ++ticks;
-            // FIRE pool first (it does not wait on the host), then the MD pool
-            for (int pi : {kPoolFire, kPoolMD}) {
+            // FIRE pool first (it does not wait on the host), then the MD pools
+            for (int pi : {kPoolFire, kPoolMD, kPoolList}) {
                 Pool &pl = pools[pi];
                 if (std::none_of(pl.job.begin(), pl.job.end(), [](int64_t x) { return x >= 0; })) continue;
                 rc = pl.b.step(static_cast<int>(q));
@@ -250,32 +267,34 @@ struct mdx_server {
                         atom_steps += static_cast<double>(q) * static_cast<double>(j.n);
                     }
             // MD jobs: energies of every running job, then retire the finished
-            Pool &md = pools[kPoolMD];
-            std::vector<int> running;
-            for (size_t s = 0; s < md.job.size(); ++s) {
-                if (md.job[s] < 0) continue;
-                const Job &j = jobs.at(md.job[s]);
-                if (p.record_energies || j.steps_done >= j.spec.nsteps) running.push_back(static_cast<int>(s));
-            }
-            std::vector<int> plain, piston;
-            for (int s : running) (md.b.slots[s].piston_active ? piston : plain).push_back(s);
-            std::vector<mdx_obs> ob(plain.size());
-            rc = md.b.observables_many(plain, MDX_OBS_K | MDX_OBS_U | MDX_OBS_P | MDX_OBS_VIRIAL, ob.data());
-            if (rc != MDX_OK) return rc;
-            for (size_t q2 = 0; q2 < plain.size(); ++q2) jobs.at(md.job[plain[q2]]).last = ob[q2];
-            for (int s : piston) {
-                rc = md.b.observables(s, MDX_OBS_K | MDX_OBS_U | MDX_OBS_P | MDX_OBS_VIRIAL,
-                                      &jobs.at(md.job[s]).last);
+            for (int pi : {kPoolMD, kPoolList}) {
+                Pool &md = pools[pi];
+                std::vector<int> running;
+                for (size_t s = 0; s < md.job.size(); ++s) {
+                    if (md.job[s] < 0) continue;
+                    const Job &j = jobs.at(md.job[s]);
+                    if (p.record_energies || j.steps_done >= j.spec.nsteps) running.push_back(static_cast<int>(s));
+                }
+                std::vector<int> plain, piston;
+                for (int s : running) (md.b.slots[s].piston_active ? piston : plain).push_back(s);
+                std::vector<mdx_obs> ob(plain.size());
+                rc = md.b.observables_many(plain, MDX_OBS_K | MDX_OBS_U | MDX_OBS_P | MDX_OBS_VIRIAL, ob.data());
                 if (rc != MDX_OK) return rc;
-            }
-            for (int s : running) {
-                Job &j = jobs.at(md.job[s]);
-                if (p.record_energies) j.records.push_back({j.steps_done, j.last.K, j.last.U, j.last.P});
-                if (j.steps_done >= j.spec.nsteps) {
-                    rc = retire(j);
+                for (size_t q2 = 0; q2 < plain.size(); ++q2) jobs.at(md.job[plain[q2]]).last = ob[q2];
+                for (int s : piston) {
+                    rc = md.b.observables(s, MDX_OBS_K | MDX_OBS_U | MDX_OBS_P | MDX_OBS_VIRIAL,
+                                          &jobs.at(md.job[s]).last);
                     if (rc != MDX_OK) return rc;
-                    if (nd < cap_done && done_ids) done_ids[nd] = j.id;
-                    ++nd;
+                }
+                for (int s : running) {
+                    Job &j = jobs.at(md.job[s]);
+                    if (p.record_energies) j.records.push_back({j.steps_done, j.last.K, j.last.U, j.last.P});
+                    if (j.steps_done >= j.spec.nsteps) {
+                        rc = retire(j);
+                        if (rc != MDX_OK) return rc;
+                        if (nd < cap_done && done_ids) done_ids[nd] = j.id;
+                        ++nd;
+                    }
                 }
             }
             // FIRE jobs: converged or out of iterations
@@ -323,6 +342,8 @@ int mdx_server_default_params(mdx_server_params *out) {
     out->min_density = 0.3;
     out->max_box_growth = 1.4;
     out->device = -1;
+    out->list_slots_per_class = 6;
+    out->list_skin = 0.5;
     return MDX_OK;
 }
 
@@ -332,8 +353,10 @@ int mdx_server_create(const mdx_server_params *params, mdx_server **out) {
     const mdx_server_params &p = *params;
     if (p.min_class_log2 < 0 || p.max_class_log2 > 28 || p.min_class_log2 > p.max_class_log2)
         return set_error(MDX_ERR_INVALID, "server: bad size classes");
-    if (p.md_slots_per_class < 0 || p.fire_slots_per_class < 0 || p.md_slots_per_class + p.fire_slots_per_class == 0)
+    if (p.md_slots_per_class < 0 || p.fire_slots_per_class < 0 || p.list_slots_per_class < 0 ||
+        p.md_slots_per_class + p.fire_slots_per_class + p.list_slots_per_class == 0)
         return set_error(MDX_ERR_INVALID, "server: no slots");
+    if (!(p.list_skin >= 0.0) || !(p.list_skin <= 4.0)) return set_error(MDX_ERR_INVALID, "server: list_skin in [0, 4]");
     if (!(p.min_density > 0.0) || !(p.max_box_growth >= 1.0) || !(p.hbm_budget_bytes > 0.0))
         return set_error(MDX_ERR_INVALID, "server: bad density, growth or budget");
     if (p.md_precision < MDX_PREC_COMP32 || p.md_precision > MDX_PREC_FP64)

@@ -2,11 +2,16 @@
 // (device-written) of a batch of independent systems.
 //
 // A batch owns S slots.  Slot r owns particles [off_r, off_r + n_cap_r) of the
-// concatenated particle arrays, cells [coff_r, coff_r + ccap_r) of the
+// concatenated particle arrays, cells [coff_r, coff_r + ccap_r] of the
 // concatenated cell arrays, per-particle workgroups [boff_r, boff_r + nblk_r)
 // and per-cell workgroups [cboff_r, cboff_r + ncblk_r).  Every kernel covers
 // all slots in one launch; a workgroup belongs to exactly one slot, so each
-// slot's reductions depend only on its own particles.
+// slot's reductions depend only on its own particles.  Cell coff_r + ccap_r is
+// the slot's padding cell: its count is n_cap_r - n_r, so the cell starts of
+// the batch-wide scan put slot r's cell-ordered particles at [off_r, off_r +
+// n_r) whatever the other slots hold, and a slot that keeps its cell list
+// between steps (list mode, idle) keeps valid cell starts while other slots
+// are loaded or unloaded.
 #pragma once
 
 #include <hip/hip_runtime.h>

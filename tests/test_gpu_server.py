@@ -44,9 +44,14 @@ def make(spec):
     return f, Lx, Ly, kw
 
 
+LIST_SKIN = 0.5   # the server's default for its NVE / NVT slots
+
+
 def alone(spec, f, Lx, Ly, kw):
     kind, n, rho, T, eab, nsteps, pext, seed = spec
     s = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), f)
+    if kind in (abi.NVE, abi.NVT):
+        s.set_tuning(1.0, 1, LIST_SKIN)   # as the server's NVE / NVT slots
     if kind == abi.FIRE:
         res, _, _ = s.fire(FP, 1e-6, nsteps)
         out = s.collect(), res
@@ -58,7 +63,8 @@ def alone(spec, f, Lx, Ly, kw):
 
 
 def run_server(specs, frames, budget=None, stagger=True):
-    kw = dict(min_class_log2=9, max_class_log2=12, md_slots_per_class=2, fire_slots_per_class=1, quantum=32)
+    kw = dict(min_class_log2=9, max_class_log2=12, md_slots_per_class=2, fire_slots_per_class=1, quantum=32,
+              list_slots_per_class=2, list_skin=LIST_SKIN)
     if budget is not None:
         kw["hbm_budget_bytes"] = budget
     srv = abi.Server(**kw)
@@ -115,8 +121,9 @@ def test_server_results_equal_alone_runs():
 def test_server_admission_respects_budget():
     specs = job_specs()[:6]
     frames = [make(s) for s in specs]
-    # room for about two of the largest slots at a time
-    budget = 2.5 * 4096 * 120.0
+    # room for about two of the largest slots at a time (a neighbour-list slot
+    # holds about 400 bytes per particle)
+    budget = 2.5 * 4096 * 400.0
     say("server budget %.0f bytes" % budget)
     srv, ids = run_server(specs, frames, budget=budget, stagger=False)
     st = srv.stats()
