@@ -106,7 +106,7 @@ Batch::~Batch() {
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
                      d_sp, d_sc, d_blk2sys, d_cblk2sys, d_want, sorted_old, cs_old, movers, mover_off,
-                     n_movers, arrivals, key_list, x0, img0, nlist, ncount, x_bak, v_bak, a_bak, sc_bak};
+                     n_movers, arrivals, key_list, x0, img0, nlist, ncount};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
     for (auto *vec : {&force_ev, &step_ev})

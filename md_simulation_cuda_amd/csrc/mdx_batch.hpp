@@ -77,9 +77,6 @@ struct Batch {
     double skin = 0.0;       // neighbour-list skin (0: cell list every step)
     int list_cap = 64;       // neighbour-list entries per particle
     int window = 0;          // list mode: steps per rebuild window
-    double2 *x_bak = nullptr, *v_bak = nullptr;   // window start state
-    void *a_bak = nullptr;
-    DevScalars *sc_bak = nullptr;
     uint32_t *key_list = nullptr, *nlist = nullptr, *ncount = nullptr;
     double2 *x0 = nullptr;
     float2 *img0 = nullptr;

@@ -129,7 +129,8 @@ struct BatchPtrs {
     uint32_t *arrivals;              // per cell: movers that arrived this step (zero at rest)
     // neighbour-list mode: key and position of each particle at the last
     // build; per cell-ordered slot s, entry e of its list at nlist[e * N + s]
-    // (partner slot | cell offset code << 28) and its length ncount[s]
+    // (list_entry in mdx_kernels.hpp: partner slot, neighbour-cell offset,
+    // same-class bit) and its row count ncount[s]
     uint32_t *key_list;
     double2 *x0;
     float2 *img0;     // per cell-ordered slot: its FP32 image at the last list-mode build
