@@ -213,12 +213,14 @@ int mdx_timing_read(mdx_sim *sim, mdx_timing *out);
  * least 5 cells per side), steps run in windows of W: the first (head) step
  * rebuilds a slot's lists when some particle, at 1.5 x its current speed,
  * could move beyond skin / 2 before the window ends; the other W - 1 steps run
- * only the box, drift, image and pair kernels.  A slot in which a particle
- * moves beyond skin / 2 anyway (or that has no usable list) evaluates the
- * rest of the window's lean steps over the 5 x 5 cells of its last cell list,
+ * three kernels (box, a fused kick/drift/image pass, pair).  A lean step in
+ * which a slot has a particle beyond skin / 2 anyway (or no usable list)
+ * evaluates that slot's forces over the 5 x 5 cells of its last cell list,
  * which holds every pair within the cutoff while no particle has moved more
- * than (2 w - cutoff) / 2 since that build; beyond that the step returns
- * MDX_ERR_STATE.  The run does not synchronise with the host until its end.
+ * than (2 w - cutoff) / 2 since that build.  If a particle goes beyond that,
+ * the run still completes its steps and then returns MDX_ERR_STATE ("use a
+ * shorter window"): such a step's forces may have missed pairs.  The run does
+ * not synchronise with the host until its end.
  * Rebuild points depend only on W and the slot's own particles.
  * Changing the tuning recomputes the accelerations; FP32 results then follow
  * the new summation order (same tolerance, bit-identical to a fresh run with
