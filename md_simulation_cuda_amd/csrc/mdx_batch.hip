@@ -64,6 +64,11 @@ PairTable make_pair_table(const mdx_params &p, double &rc_max) {
         pt.sig2f[k] = sf * sf;
         pt.eps24f[k] = 24.0f * ef;
         pt.eps4f[k] = 4.0f * ef;
+        const double q6 = pt.sig2[k] * pt.sig2[k] * pt.sig2[k];   // sigma^6
+        pt.fa[k] = static_cast<float>(2.0 * pt.eps24[k] * q6 * q6);
+        pt.fb[k] = static_cast<float>(pt.eps24[k] * q6);
+        pt.ua[k] = static_cast<float>(pt.eps4[k] * q6 * q6);
+        pt.ub[k] = static_cast<float>(pt.eps4[k] * q6);
     }
     pt.m[0] = p.mass_a;
     pt.m[1] = p.mass_b;

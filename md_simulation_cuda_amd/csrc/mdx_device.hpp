@@ -24,6 +24,10 @@ struct PairTable {
     double vc[3];          // eps * 4 (c^-12 - c^-6): energy shift
     double inv_m[3];       // 1 / mass by class (class 0 -> mass_a, 1,2 -> mass_b)
     double m[3];
+    // FP32 neighbour-list pair in powers of 1/r^2: F/r = r^-2 r^-6 (fa r^-6 -
+    // fb) with fa = 48 eps sigma^12, fb = 24 eps sigma^6; u = r^-6 (ua r^-6 -
+    // ub) with ua = 4 eps sigma^12, ub = 4 eps sigma^6 (rounded from FP64)
+    float fa[3], fb[3], ua[3], ub[3];
     float r2_guard;        // FP32 pairs closer than this are evaluated in FP64
     float pad;
 };

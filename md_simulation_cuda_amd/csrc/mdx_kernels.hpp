@@ -699,20 +699,33 @@ __global__ void __launch_bounds__(kBlock) k_lean(BatchPtrs P) {
 
 // Neighbour-list entry (list mode): (partner slot << 3), so that entry &
 // kListOff is the byte offset of the partner's FP32 image; bit 2 set when the
-// partner has the particle's own type class; the neighbour cell's offset ox in
-// bits 0-1 and oy in bits 30-31 (two's complement, -1..1).  Partner slots are
-// below 2^27.
+// partner has the particle's own type class; the neighbour cell's offset
+// (ox, oy), -1..1 each, as the code c = (ox + 1) + 3 (oy + 1) in 0..8, whose
+// bits 0-1 sit in entry bits 30-31 and bits 2-3 in entry bits 0-1: rotated
+// left by 13 the entry holds c in bits 11-14, the byte offset of row c of a
+// [9][kBlock] table of float2 (list_shift_off).  Partner slots are below 2^27.
 constexpr uint32_t kListOff = 0x3FFFFFF8u;
 constexpr uint32_t kListSame = 4u;
 // list rows a lane stages in LDS per pass of the list build
 constexpr int kListStage = 32;
 
 __device__ inline uint32_t list_entry(uint32_t k, int ox, int oy, bool same) {
-    return (k << 3) | (same ? kListSame : 0u) | (static_cast<uint32_t>(ox) & 3u) |
-           ((static_cast<uint32_t>(oy) & 3u) << 30);
+    const uint32_t c = static_cast<uint32_t>((ox + 1) + 3 * (oy + 1));
+    return (k << 3) | (same ? kListSame : 0u) | (c >> 2) | ((c & 3u) << 30);
 }
-__device__ inline float list_ox(uint32_t v) { return static_cast<float>(static_cast<int32_t>(v << 30) >> 30); }
-__device__ inline float list_oy(uint32_t v) { return static_cast<float>(static_cast<int32_t>(v) >> 30); }
+__device__ inline uint32_t list_code(uint32_t v) { return (v >> 30) | ((v & 3u) << 2); }
+// byte offset of the entry's row in a [9][kBlock] float2 table
+__device__ inline uint32_t list_shift_off(uint32_t v) { return __builtin_amdgcn_alignbit(v, v, 19) & 0x7800u; }
+static_assert(kBlock * sizeof(float2) == 0x800u, "list_shift_off assumes 2048-byte table rows");
+// the offset's components from the code: oy + 1 = c / 3 (as (11 c) >> 5 for
+// c <= 8), ox + 1 = c - 3 (oy + 1)
+__device__ inline float list_oy(uint32_t v) {
+    return static_cast<float>(static_cast<int>((list_code(v) * 11u) >> 5) - 1);
+}
+__device__ inline float list_ox(uint32_t v) {
+    const uint32_t c = list_code(v);
+    return static_cast<float>(static_cast<int>(c - 3u * ((c * 11u) >> 5)) - 1);
+}
 // Padding entry of slot s: the particle itself one cell to the side, at
 // distance w >= r_list > r_c, so it is never inside the cutoff.
 __device__ inline uint32_t list_pad(uint32_t s) { return list_entry(s, 1, 0, true); }
@@ -987,6 +1000,7 @@ __device__ inline void finalize_pair(int pmode, const double (&out)[3], const Sy
 // the particle's own class, "o" for the AB combination.
 struct PairF32 {
     float rc2s, rc2o, sig2s, sig2o, e24s, e24o, e4s, e4o, r2_guard;
+    float fas, fao, fbs, fbo, uas, uao, ubs, ubo;   // PairTable fa, fb, ua, ub (list path)
 };
 
 // FP32 pairs of particle (pix, piy) -- its cell-relative image already moved
@@ -1053,30 +1067,29 @@ __device__ __forceinline__ void pair_cell_f32(const char *__restrict__ imgb, con
     if constexpr (ACC == ACC_COMP32) F.add(fxc, fyc);
 }
 
-// One neighbour-list entry of the particle at (pix, piy) (its image) whose
-// partner image is q: the FP32 pair of pair_cell_f32 with the displacement
-// formed from the entry's cell offset.  The force is computed unconditionally
-// and kept only when the pair is inside the cutoff, so no branch is taken per
-// entry.  GUARD: pairs closer than the FP64 guard are left out here; `slow`
-// collects, as a bit per lane, whether one of them is also farther than 0
-// (0 < r^2 < r2_guard, compared as IEEE bit patterns, which order like the
-// values for r^2 >= 0), and the caller evaluates those in FP64.
+// One neighbour-list entry of a particle whose image, moved by minus the
+// entry's neighbour-cell offset, is (psx, psy), with the partner image q: the
+// FP32 pair of pair_cell_f32 (the same displacement), its force evaluated in
+// powers of 1/r^2 as F/r = r^-2 r^-6 (fa r^-6 - fb).  The force is computed
+// unconditionally and kept only when the pair is inside the cutoff, so no
+// branch is taken per entry.  GUARD: pairs closer than the FP64 guard are left
+// out here; `slow` collects, as a bit per lane, whether one of them is also
+// farther than 0 (0 < r^2 < r2_guard, compared as IEEE bit patterns, which
+// order like the values for r^2 >= 0), and the caller evaluates those in FP64.
 template <int ACC, bool OBS, bool UNI, bool GUARD>
-__device__ __forceinline__ void list_pair_f32(uint32_t v, float2 q, float pix, float piy, int ci, const Grid &g,
+__device__ __forceinline__ void list_pair_f32(uint32_t v, float2 q, float psx, float psy, int ci,
                                               const PairTable &pt, const PairF32 &pc, float &fxc, float &fyc,
                                               ForceSum<ACC> &F, ObsSum &ob, bool want_obs, bool &slow) {
-    const float dx = __builtin_fmaf(-list_ox(v), g.wf, pix) - q.x;
-    const float dy = __builtin_fmaf(-list_oy(v), g.hf, piy) - q.y;
+    const float dx = psx - q.x;
+    const float dy = psy - q.y;
     const float rr = __builtin_fmaf(dx, dx, dy * dy);
     const bool same = (v & kListSame) != 0u;
     const float rc2 = UNI ? pc.rc2s : (same ? pc.rc2s : pc.rc2o);
-    const float sig2 = UNI ? pc.sig2s : (same ? pc.sig2s : pc.sig2o);
-    const float e24 = same ? pc.e24s : pc.e24o;
+    const float fa = same ? pc.fas : pc.fao;
+    const float fb = same ? pc.fbs : pc.fbo;
     const float r2i = fast_rcp(rr);
-    const float sr2 = sig2 * r2i;
-    const float sr6 = sr2 * sr2 * sr2;
-    const float sr12 = sr6 * sr6;
-    const float t = e24 * __builtin_fmaf(2.0f, sr12, -sr6) * r2i;
+    const float r6i = r2i * r2i * r2i;
+    const float t = __builtin_fmaf(fa, r6i, -fb) * r6i * r2i;
     const bool in = GUARD ? (rr < rc2 && rr >= pc.r2_guard) : rr < rc2;
     const float tmp = in ? t : 0.0f;
     if constexpr (ACC == ACC_COMP32) {   // the run's FP32 sum, one rounding per term
@@ -1090,8 +1103,7 @@ __device__ __forceinline__ void list_pair_f32(uint32_t v, float2 q, float pix, f
     if constexpr (OBS) {
         if (want_obs && in) {
             const float fx = tmp * dx, fy = tmp * dy;
-            const float e4 = same ? pc.e4s : pc.e4o;
-            const float uu = e4 * (sr12 - sr6);
+            const float uu = __builtin_fmaf(same ? pc.uas : pc.uao, r6i, -(same ? pc.ubs : pc.ubo)) * r6i;
             const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
             ob.u += 0.5 * (static_cast<double>(uu) - pt.vc[same ? ci : 2]);
             ob.w += 0.5 * (ddx * dfx + ddy * dfy);
@@ -1101,16 +1113,17 @@ __device__ __forceinline__ void list_pair_f32(uint32_t v, float2 q, float pix, f
 }
 
 // The FP64 evaluation of the guarded pairs among U entries (0 < r^2 <
-// r2_guard), in entry order.
+// r2_guard), in entry order; ps[u] is the particle's image moved by minus
+// entry u's cell offset.
 template <int U, bool OBS>
-__device__ __forceinline__ void list_pairs_guarded(const uint32_t (&v)[U], const float2 (&q)[U], float pix,
-                                                   float piy, int ci, const Grid &g, const PairTable &pt,
+__device__ __forceinline__ void list_pairs_guarded(const uint32_t (&v)[U], const float2 (&q)[U],
+                                                   const float2 (&ps)[U], int ci, const PairTable &pt,
                                                    const PairF32 &pc, double &gx, double &gy, ObsSum &ob,
                                                    bool want_obs) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const float dx = __builtin_fmaf(-list_ox(v[u]), g.wf, pix) - q[u].x;
-        const float dy = __builtin_fmaf(-list_oy(v[u]), g.hf, piy) - q[u].y;
+        const float dx = ps[u].x - q[u].x;
+        const float dy = ps[u].y - q[u].y;
         const float rr = __builtin_fmaf(dx, dx, dy * dy);
         if (rr < pc.r2_guard && rr > 0.0f) {
             double fx, fy;
@@ -1126,24 +1139,30 @@ __device__ __forceinline__ void list_pairs_guarded(const uint32_t (&v)[U], const
 // the whole wave, so the loop is uniform.  Each pass takes four rows; the
 // next pass's rows are loaded before this pass's pairs are evaluated (the last
 // pass reloads its own rows).  Rows are addressed from a scalar base plus the
-// lane's 32-bit byte offset.
+// lane's 32-bit byte offset.  shb: the workgroup's [9][kBlock] table of the
+// particles' images moved by minus each neighbour-cell offset, read at the
+// entry's code (list_shift_off).
 template <int ACC, bool OBS, bool UNI, bool GUARD>
-__device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, uint32_t n, float2 pi, int ci,
-                                              const Grid &g, const PairTable &pt, const PairF32 &pc,
-                                              ForceSum<ACC> &F, ObsSum &ob, bool want_obs, float &fxc, float &fyc,
-                                              double &gx, double &gy) {
+__device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, uint32_t n, const char *shb, int ci,
+                                              const PairTable &pt, const PairF32 &pc, ForceSum<ACC> &F,
+                                              ObsSum &ob, bool want_obs, float &fxc, float &fyc, double &gx,
+                                              double &gy) {
     constexpr int U = 4;
     const char *imgb = static_cast<const char *>(P.img);
     const char *lb = reinterpret_cast<const char *>(P.nlist);
     const size_t rowb = static_cast<size_t>(P.N) * 4u;   // bytes per row
     const uint32_t soff = s * 4u;
+    const uint32_t lane8 = threadIdx.x * 8u;
     uint32_t v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const uint32_t *>(lb + u * rowb + soff);
     for (uint32_t e = 0; e < n; e += U) {
-        float2 q[U];
+        float2 q[U], ps[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) q[u] = *reinterpret_cast<const float2 *>(imgb + (v[u] & kListOff));
+        for (int u = 0; u < U; ++u) {
+            q[u] = *reinterpret_cast<const float2 *>(imgb + (v[u] & kListOff));
+            ps[u] = *reinterpret_cast<const float2 *>(shb + (list_shift_off(v[u]) | lane8));
+        }
         const uint32_t en = e + U < n ? e + U : e;
         const char *row = lb + static_cast<size_t>(en) * rowb;
         uint32_t vn[U];
@@ -1152,10 +1171,10 @@ __device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, ui
         bool slow = false;
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            list_pair_f32<ACC, OBS, UNI, GUARD>(v[u], q[u], pi.x, pi.y, ci, g, pt, pc, fxc, fyc, F, ob, want_obs,
+            list_pair_f32<ACC, OBS, UNI, GUARD>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob, want_obs,
                                                 slow);
         if constexpr (GUARD) {
-            if (__any(slow)) list_pairs_guarded<U, OBS>(v, q, pi.x, pi.y, ci, g, pt, pc, gx, gy, ob, want_obs);
+            if (__any(slow)) list_pairs_guarded<U, OBS>(v, q, ps, ci, pt, pc, gx, gy, ob, want_obs);
         }
         if constexpr (ACC == ACC_COMP32) {
             if (e & U) {   // every eight rows
@@ -1183,12 +1202,23 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, fl
     float fxc = 0.0f, fyc = 0.0f;
     double gx = 0.0, gy = 0.0;
     if constexpr (LPP == 1) {
+        // the nine shifted images of each particle, one table row per cell
+        // code (list_entry), the lane's own column
+        __shared__ float2 shift[9][kBlock];
+#pragma unroll
+        for (int oy = -1; oy <= 1; ++oy)
+#pragma unroll
+            for (int ox = -1; ox <= 1; ++ox)
+                shift[(ox + 1) + 3 * (oy + 1)][threadIdx.x] =
+                    make_float2(__builtin_fmaf(-static_cast<float>(ox), g.wf, pi.x),
+                                __builtin_fmaf(-static_cast<float>(oy), g.hf, pi.y));
+        const char *shb = reinterpret_cast<const char *>(&shift[0][0]);
         const uint32_t nr = __builtin_amdgcn_readfirstlane(P.ncount[s]);
         const uint32_t n = nr & ~kListGuard;
         if (nr & kListGuard)
-            list_rows_f32<ACC, OBS, UNI, true>(P, s, n, pi, ci, g, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy);
+            list_rows_f32<ACC, OBS, UNI, true>(P, s, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy);
         else
-            list_rows_f32<ACC, OBS, UNI, false>(P, s, n, pi, ci, g, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy);
+            list_rows_f32<ACC, OBS, UNI, false>(P, s, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy);
     } else {
         const char *imgb = static_cast<const char *>(P.img);
         const size_t stride = static_cast<size_t>(P.N);
@@ -1202,14 +1232,18 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, fl
                 const uint32_t ee = e + u * LPP;
                 v[u] = ee < n ? P.nlist[ee * stride + s] : pad;
             }
-            float2 q[U];
+            float2 q[U], ps[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) q[u] = *reinterpret_cast<const float2 *>(imgb + (v[u] & kListOff));
+            for (int u = 0; u < U; ++u) {
+                q[u] = *reinterpret_cast<const float2 *>(imgb + (v[u] & kListOff));
+                ps[u] = make_float2(__builtin_fmaf(-list_ox(v[u]), g.wf, pi.x),
+                                    __builtin_fmaf(-list_oy(v[u]), g.hf, pi.y));
+            }
             bool slow = false;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                list_pair_f32<ACC, OBS, UNI, true>(v[u], q[u], pi.x, pi.y, ci, g, pt, pc, fxc, fyc, F, ob, want_obs,
-                                                   slow);
+                list_pair_f32<ACC, OBS, UNI, true>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
+                                                   want_obs, slow);
                 if constexpr (ACC == ACC_COMP32) {
                     if (++run == 8u) {
                         F.add(fxc, fyc);
@@ -1218,7 +1252,7 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, fl
                     }
                 }
             }
-            if (slow) list_pairs_guarded<U, OBS>(v, q, pi.x, pi.y, ci, g, pt, pc, gx, gy, ob, want_obs);
+            if (slow) list_pairs_guarded<U, OBS>(v, q, ps, ci, pt, pc, gx, gy, ob, want_obs);
         }
     }
     if constexpr (ACC == ACC_COMP32) F.add(fxc, fyc);
@@ -1325,6 +1359,14 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P) {
             pc.e4s = pt.eps4f[ci];
             pc.e4o = pt.eps4f[2];
             pc.r2_guard = pt.r2_guard;
+            pc.fas = pt.fa[ci];
+            pc.fao = pt.fa[2];
+            pc.fbs = pt.fb[ci];
+            pc.fbo = pt.fb[2];
+            pc.uas = pt.ua[ci];
+            pc.uao = pt.ua[2];
+            pc.ubs = pt.ub[ci];
+            pc.ubo = pt.ub[2];
         }
         bool by_list = false;
         if constexpr (!kFp64) {
