@@ -110,7 +110,7 @@ Batch::~Batch() {
     if (stream) (void)hipStreamSynchronize(stream);
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
-                     d_sp, d_sc, d_blk2sys, d_cblk2sys, d_want, sorted_old, cs_old, movers, mover_off,
+                     d_sp, d_sc, d_blk2sys, d_cblk2sys, d_lb2sys, d_want, sorted_old, cs_old, movers, mover_off,
                      n_movers, arrivals, key_list, x0, img0, nlist, ncount};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
@@ -124,11 +124,12 @@ Batch::~Batch() {
 
 void Batch::invalidate_graphs() {
     for (auto &row : graph)
-        for (auto &g : row)
-            if (g) {
-                (void)hipGraphExecDestroy(g);
-                g = nullptr;
-            }
+        for (auto &pair : row)
+            for (auto &g : pair)
+                if (g) {
+                    (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
 }
 
 int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int precision, int device_ordinal) {
@@ -145,9 +146,9 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     slots.assign(nslots, Slot{});
     h_sp.assign(nslots, SysParams{});
     h_want.assign(nslots, 0);
-    std::vector<int32_t> b2s, cb2s;
+    std::vector<int32_t> b2s, cb2s, lb2s;
     int64_t off = 0, coff = 0;
-    int boff = 0, cboff = 0;
+    int boff = 0, cboff = 0, lboff = 0;
     for (int r = 0; r < nslots; ++r) {
         if (n_cap[r] < 0 || ccap[r] < 0) return set_error(MDX_ERR_INVALID, "negative capacity");
         Slot &s = slots[r];
@@ -162,14 +163,18 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
         sp.nblk = ceil_div(n_cap[r], kBlock);
         sp.cboff = cboff;
         sp.ncblk = ceil_div(ccap[r], kBlock);
+        sp.lboff = lboff;
+        sp.nlblk = ceil_div(n_cap[r], kLeanSpan);
         sp.periodic_y = 1;
         sp.wrap_keys = 1;
         for (int b = 0; b < sp.nblk; ++b) b2s.push_back(r);
         for (int b = 0; b < sp.ncblk; ++b) cb2s.push_back(r);
+        for (int b = 0; b < sp.nlblk; ++b) lb2s.push_back(r);
         off += n_cap[r];
         coff += ccap[r] + 1;   // + the slot's padding cell (mdx_sim.hpp)
         boff += sp.nblk;
         cboff += sp.ncblk;
+        lboff += sp.nlblk;
         // FP32 images are addressed with 32-bit byte offsets (slot << 3)
         if (off > (int64_t{1} << 29) - kBlock || coff > INT32_MAX / 2)
             return set_error(MDX_ERR_CAPACITY, "batch above 2^29 particles or 2^30 cells");
@@ -178,6 +183,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     C = static_cast<int32_t>(coff);
     TB = boff;
     TCB = cboff;
+    TLB = lboff;
     const size_t n16 = static_cast<size_t>(std::max<int64_t>(N, 1)) * 16;
     const size_t n4 = static_cast<size_t>(std::max<int64_t>(N, 1)) * 4;
     const size_t n1 = static_cast<size_t>(std::max<int64_t>(N, 1));
@@ -209,6 +215,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     MDX_HIP(hipMalloc(&d_sc, nslots * sizeof(DevScalars)));
     MDX_HIP(hipMalloc(&d_blk2sys, std::max<size_t>(b2s.size(), 1) * 4));
     MDX_HIP(hipMalloc(&d_cblk2sys, std::max<size_t>(cb2s.size(), 1) * 4));
+    MDX_HIP(hipMalloc(&d_lb2sys, std::max<size_t>(lb2s.size(), 1) * 4));
     MDX_HIP(hipMalloc(&d_want, nslots * 4));
     MDX_HIP(hipMemset(count, 0, (static_cast<size_t>(C) + 1) * 4));
     MDX_HIP(hipMemset(d_sc, 0, nslots * sizeof(DevScalars)));
@@ -217,6 +224,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     MDX_HIP(hipMemset(a, 0, n16));
     if (!b2s.empty()) MDX_HIP(hipMemcpy(d_blk2sys, b2s.data(), b2s.size() * 4, hipMemcpyHostToDevice));
     if (!cb2s.empty()) MDX_HIP(hipMemcpy(d_cblk2sys, cb2s.data(), cb2s.size() * 4, hipMemcpyHostToDevice));
+    if (!lb2s.empty()) MDX_HIP(hipMemcpy(d_lb2sys, lb2s.data(), lb2s.size() * 4, hipMemcpyHostToDevice));
     MDX_HIP(hipMemcpy(d_sp, h_sp.data(), nslots * sizeof(SysParams), hipMemcpyHostToDevice));
     ptrs.x = x;
     ptrs.v = v;
@@ -240,6 +248,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     ptrs.sc = d_sc;
     ptrs.blk2sys = d_blk2sys;
     ptrs.cblk2sys = d_cblk2sys;
+    ptrs.lb2sys = d_lb2sys;
     ptrs.TB = TB;
     ptrs.S = nslots;
     ptrs.C = C;
@@ -287,29 +296,32 @@ void set_idle(SysParams &sp, const Slot &s) {
 
 // FP32 pair kernel with lpp lanes per particle (1, 2 or 4).
 template <int ACC, bool OBS, bool UNI>
-static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &p) {
+static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &p, int parity, int lean) {
     switch (lpp) {
-    case 2: k_pair<float2, ACC, OBS, UNI, 2><<<grid, kBlock * 2, 0, st>>>(p); break;
-    case 4: k_pair<float2, ACC, OBS, UNI, 4><<<grid, kBlock * 4, 0, st>>>(p); break;
-    default: k_pair<float2, ACC, OBS, UNI, 1><<<grid, kBlock, 0, st>>>(p); break;
+    case 2: k_pair<float2, ACC, OBS, UNI, 2><<<grid, kBlock * 2, 0, st>>>(p, parity, lean); break;
+    case 4: k_pair<float2, ACC, OBS, UNI, 4><<<grid, kBlock * 4, 0, st>>>(p, parity, lean); break;
+    default: k_pair<float2, ACC, OBS, UNI, 1><<<grid, kBlock, 0, st>>>(p, parity, lean); break;
     }
 }
 
-// Launch one step of every slot (their kind/pmode in d_sp).
-int Batch::launch_step_kernels(bool obs, int mode) {
+// Launch one step of every slot (their kind/pmode in d_sp).  A lean step
+// (STEP_LEAN, step parity `parity`) is k_lean and the pair pass; every other
+// step starts with k_box and runs the cell-list kernels.
+int Batch::launch_step_kernels(bool obs, int mode, int parity) {
     const int S = static_cast<int>(slots.size());
     const bool lean = mode == STEP_LEAN;
+    if (!lean) parity = 0;
     // a window's head step keeps lists only if they last its window - 1 lean steps
     const int ahead = mode == STEP_HEAD ? std::max(window - 1, 0) : 0;
-    k_box<<<S, kBlock, 0, stream>>>(ptrs, mode);
-    if (TB > 0 && lean) {   // drift and image in one pass (windows run in the FP32 forms only)
-        if (prec == MDX_PREC_COMP32) k_lean<float2><<<TB, kBlock, 0, stream>>>(ptrs);
-        else k_lean<double2><<<TB, kBlock, 0, stream>>>(ptrs);
-    } else if (TB > 0) {
+    if (!lean) k_box<<<S, kBlock, 0, stream>>>(ptrs, mode);
+    if (TLB > 0 && lean) {   // windows run in the FP32 forms only
+        if (prec == MDX_PREC_COMP32) k_lean<float2><<<TLB, kLeanThreads, 0, stream>>>(ptrs, parity);
+        else k_lean<double2><<<TLB, kLeanThreads, 0, stream>>>(ptrs, parity);
+    } else if (TB > 0 && !lean) {
         if (prec == MDX_PREC_COMP32)
-            k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs, 0, ahead);
+            k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs, ahead);
         else
-            k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs, 0, ahead);
+            k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs, ahead);
     }
     if (!lean) {   // cell list (and, in list mode, neighbour lists) of the slots that rebuild
         if (TCB + TB > 0) k_stay_arrive<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
@@ -337,25 +349,25 @@ int Batch::launch_step_kernels(bool obs, int mode) {
         switch (prec) {
         case MDX_PREC_COMP32:
             if (obs) {
-                if (uni) launch_pair_f32<ACC_COMP32, true, true>(lpp, TB, stream, ptrs);
-                else launch_pair_f32<ACC_COMP32, true, false>(lpp, TB, stream, ptrs);
+                if (uni) launch_pair_f32<ACC_COMP32, true, true>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
+                else launch_pair_f32<ACC_COMP32, true, false>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
             } else {
-                if (uni) launch_pair_f32<ACC_COMP32, false, true>(lpp, TB, stream, ptrs);
-                else launch_pair_f32<ACC_COMP32, false, false>(lpp, TB, stream, ptrs);
+                if (uni) launch_pair_f32<ACC_COMP32, false, true>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
+                else launch_pair_f32<ACC_COMP32, false, false>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
             }
             break;
         case MDX_PREC_ACC64:
             if (obs) {
-                if (uni) launch_pair_f32<ACC_FP64, true, true>(lpp, TB, stream, ptrs);
-                else launch_pair_f32<ACC_FP64, true, false>(lpp, TB, stream, ptrs);
+                if (uni) launch_pair_f32<ACC_FP64, true, true>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
+                else launch_pair_f32<ACC_FP64, true, false>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
             } else {
-                if (uni) launch_pair_f32<ACC_FP64, false, true>(lpp, TB, stream, ptrs);
-                else launch_pair_f32<ACC_FP64, false, false>(lpp, TB, stream, ptrs);
+                if (uni) launch_pair_f32<ACC_FP64, false, true>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
+                else launch_pair_f32<ACC_FP64, false, false>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
             }
             break;
         default:   // the FP64 form keeps one lane per particle (the oracle's order)
-            if (obs) k_pair<double2, ACC_FP64, true, false, 1><<<TB, kBlock, 0, stream>>>(ptrs);
-            else k_pair<double2, ACC_FP64, false, false, 1><<<TB, kBlock, 0, stream>>>(ptrs);
+            if (obs) k_pair<double2, ACC_FP64, true, false, 1><<<TB, kBlock, 0, stream>>>(ptrs, 0, 0);
+            else k_pair<double2, ACC_FP64, false, false, 1><<<TB, kBlock, 0, stream>>>(ptrs, 0, 0);
             break;
         }
         if (e1) {
@@ -381,14 +393,15 @@ bool Batch::uniform_sigma() const {
 
 // count steps of one mode, replayed from a captured hipGraph unless kernel
 // timing is on.
-int Batch::replay(bool obs, int mode, int count_) {
+int Batch::replay(bool obs, int mode, int count_, int parity) {
     if (count_ <= 0) return MDX_OK;
+    if (mode != STEP_LEAN) parity = 0;
     if (use_graph && !timing) {
-        hipGraphExec_t &ge = graph[obs ? 1 : 0][mode];
+        hipGraphExec_t &ge = graph[obs ? 1 : 0][mode][parity];
         if (!ge) {
             hipGraph_t g = nullptr;
             MDX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            const int rc = launch_step_kernels(obs, mode);
+            const int rc = launch_step_kernels(obs, mode, parity);
             hipError_t e = hipStreamEndCapture(stream, &g);
             if (rc != MDX_OK) {
                 if (g) (void)hipGraphDestroy(g);
@@ -401,7 +414,7 @@ int Batch::replay(bool obs, int mode, int count_) {
         }
         for (int k = 0; k < count_; ++k) MDX_HIP(hipGraphLaunch(ge, stream));
     } else {
-        for (int k = 0; k < count_; ++k) MDX_TRY(launch_step_kernels(obs, mode));
+        for (int k = 0; k < count_; ++k) MDX_TRY(launch_step_kernels(obs, mode, parity));
     }
     return MDX_OK;
 }
@@ -468,8 +481,8 @@ int Batch::run_windows(int count_) {
     const int S = static_cast<int>(slots.size());
     for (int left = count_; left > 0;) {
         const int w = std::min(window, left);
-        MDX_TRY(replay(false, STEP_HEAD, 1));
-        MDX_TRY(replay(false, STEP_LEAN, w - 1));
+        MDX_TRY(replay(false, STEP_HEAD, 1));   // step parity 0
+        for (int j = 1; j < w; ++j) MDX_TRY(replay(false, STEP_LEAN, 1, j & 1));
         left -= w;
     }
     k_finalize<<<S, kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);

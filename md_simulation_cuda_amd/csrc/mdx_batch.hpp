@@ -52,6 +52,7 @@ struct Batch {
     int32_t C = 0;     // total cell capacity
     int32_t TB = 0;    // per-particle workgroups
     int32_t TCB = 0;   // per-cell workgroups
+    int32_t TLB = 0;   // lean-step workgroups (kLeanSpan slots each)
     std::vector<SysParams> h_sp;
     std::vector<int32_t> h_want;
     // device memory
@@ -66,10 +67,10 @@ struct Batch {
     double *partials = nullptr;
     SysParams *d_sp = nullptr;
     DevScalars *d_sc = nullptr;
-    int32_t *d_blk2sys = nullptr, *d_cblk2sys = nullptr, *d_want = nullptr;
+    int32_t *d_blk2sys = nullptr, *d_cblk2sys = nullptr, *d_lb2sys = nullptr, *d_want = nullptr;
     BatchPtrs ptrs{};
     // captured step graphs, by [obs]
-    hipGraphExec_t graph[2][4] = {};   // [obs][StepMode]
+    hipGraphExec_t graph[2][4][2] = {};   // [obs][StepMode][step parity (lean steps)]
     bool use_graph = true;
     bool graph_uni = true;   // pair-kernel variant the captured graphs hold
     int lpp = 1;             // lanes per particle of the FP32 pair kernel (1, 2, 4)
@@ -128,9 +129,9 @@ struct Batch {
     // internals
     int upload_sp(int r);
     int upload_all_sp();
-    int launch_step_kernels(bool obs, int mode);
+    int launch_step_kernels(bool obs, int mode, int parity);
     int run_passes(int count, bool obs);
-    int replay(bool obs, int mode, int count);
+    int replay(bool obs, int mode, int count, int parity = 0);
     int run_windows(int count);
     bool windows_apply();
     int pre_step();

@@ -146,7 +146,9 @@ __device__ void block_partials(double (&v)[NV], unsigned maxmask, double *partia
 }
 
 // Second part: one workgroup combines nb partials (partials[k * stride + base
-// + i], i < nb) in a fixed order.  Every thread returns the totals.
+// + i], i < nb) in a fixed order.  Every thread returns the totals.  The first
+// kBlock threads combine, so a larger workgroup gives the same order (and the
+// same totals).
 template <int NV>
 __device__ void slot_reduce(unsigned maxmask, const double *partials, int stride, int base, int nb,
                             double (&out)[NV]) {
@@ -159,7 +161,7 @@ __device__ void slot_reduce(unsigned maxmask, const double *partials, int stride
     double a[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) a[k] = ((maxmask >> k) & 1u) ? -1.0e308 : 0.0;
-    int i = threadIdx.x;
+    int i = threadIdx.x < kBlock ? static_cast<int>(threadIdx.x) : nb;
     for (; i + (kU - 1) * kBlock < nb; i += kU * kBlock) {
         double q[NV][kU];
 #pragma unroll
@@ -185,7 +187,7 @@ __device__ void slot_reduce(unsigned maxmask, const double *partials, int stride
     for (int k = 0; k < NV; ++k) {
         const bool mx = (maxmask >> k) & 1u;
         const double r = mx ? wave_max(a[k]) : wave_sum(a[k]);
-        if (lane == 0) sh[k][wid] = r;
+        if (lane == 0 && wid < kWaves) sh[k][wid] = r;
     }
     __syncthreads();
 #pragma unroll

@@ -74,7 +74,7 @@ __device__ inline void finalize_pair_pending(const BatchPtrs &P, const SysParams
 enum StepMode : int32_t {
     STEP_NORMAL = 0,    // list slots rebuild when a particle moved beyond skin / 2
     STEP_REBUILD = 1,   // list slots rebuild their cell and neighbour lists
-    STEP_LEAN = 2,      // list slots keep their lists; no cell-list kernels run
+    STEP_LEAN = 2,      // a rebuild window's lean step: k_lean and k_pair only
     STEP_HEAD = 3,      // first step of a window: STEP_NORMAL, with lists that
                         // must also hold through the window's lean steps
 };
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
             sc.list_valid = 0;
             sc.list_built = 0;
             sc.list_on = 0;
-            sc.fallback = 0;
+            sc.fallback[0] = sc.fallback[1] = 0;
         }
         return;
     }
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
         // its cells' counts hold their occupancy, so the scan keeps its cell
         // starts, and its next step continues from the same lists
         sc.rebuild = 0;
-        sc.fallback = 0;
+        sc.fallback[0] = sc.fallback[1] = 0;
         return;
     }
     double Lx = sc.g.Lx, Ly = sc.g.Ly;
@@ -186,26 +186,8 @@ __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
     // outside list mode every step rebuilds the cell list
     const bool on = sp.list_mode && !sc.list_overflow && g.periodic_y && g.w >= sp.r_list && g.h >= sp.r_list;
     sc.list_on = on ? 1 : 0;
-    if (mode == STEP_LEAN) {
-        // no cell-list kernels run: a stepping slot without a usable list takes
-        // the 5x5 pass over its last cell list, which needs 5 distinct cells
-        // per row and column
-        sc.rebuild = 0;
-        sc.fallback = (sp.kind != SK_NONE && (!on || !sc.list_built)) ? 1 : 0;
-        if (sp.kind == SK_NVT) {
-            // the lean step's first kick is v' = c v + (dt/2) a with
-            // c = 1 - dt xi1 / 2, so K(v') = c^2 K + (c dt / 2) sum m v.a
-            // + (dt^2 / 4) sum m |a|^2 / 2, from the last pair pass's sums
-            const double xi1 = nh_half_update(sc.xi, sc.K, sp);
-            const double c = 1.0 - 0.5 * sp.dt * xi1;
-            const double Kh = c * c * sc.K + 0.5 * c * sp.dt * sc.S_va + 0.25 * sp.dt * sp.dt * sc.S_aa;
-            sc.K_half = Kh;
-            sc.xi_mid = nh_half_update(xi1, Kh, sp);
-        }
-    } else {
-        sc.rebuild = (!on || !sc.list_built || mode == STEP_REBUILD) ? 1 : 0;
-        sc.fallback = 0;
-    }
+    sc.rebuild = (!on || !sc.list_built || mode == STEP_REBUILD) ? 1 : 0;
+    sc.fallback[0] = sc.fallback[1] = 0;   // (lean steps: k_lean of the step before clears its own)
 }
 
 // Half update of the Nose-Hoover variable, src/md_env.cu:1668-1673:
@@ -249,12 +231,12 @@ __device__ inline void append_movers(const BatchPtrs &P, bool mover, uint32_t pi
 // slots that keep their cell list skip the kernel; other slots that do not
 // move this step (idle without a cell list, converged FIRE) still recompute
 // keys so the cell list is rebuilt unchanged.
-// lean: a lean step (no cell-list kernels); ahead: a window's head step, whose
-// lists must also hold through `ahead` lean steps: the slot rebuilds now when
-// some particle, moving 1.5 x its current speed, would pass skin / 2 within
-// them (a particle that passes it anyway makes its lean step fall back).
+// ahead: a window's head step, whose lists must also hold through `ahead`
+// lean steps: the slot rebuilds now when some particle, moving 1.5 x its
+// current speed, would pass skin / 2 within them (a particle that passes it
+// anyway makes its lean step fall back).
 template <typename AT>
-__global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t lean, int32_t ahead) {
+__global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t ahead) {
     const int gb = xcd_block(blockIdx.x, gridDim.x);
     const int r = P.blk2sys[gb];
     const SysParams &sp = P.sp[r];
@@ -356,28 +338,19 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t lean, int
         mover = !sc.list_valid || (sc.list_built ? P.key_list[i] : P.key[i]) != c;
         // displacement since the last cell-list build of a list-mode slot (x0
         // is written at every such build)
-        if (lean ? (sp.list_mode != 0 && sc.list_valid) : (sc.list_on && sc.list_built)) {
+        if (sc.list_on && sc.list_built) {
             const double2 p0 = P.x0[i];
             double dx = p.x - p0.x, dy = p.y - p0.y;
             if (dx > 0.5 * g.Lx) dx -= g.Lx;
             else if (dx < -0.5 * g.Lx) dx += g.Lx;
             if (dy > 0.5 * g.Ly) dy -= g.Ly;
             else if (dy < -0.5 * g.Ly) dy += g.Ly;
-            const double d2 = dx * dx + dy * dy;
-            if (lean) {
-                // beyond skin / 2 the list may miss pairs: the 5x5 pass; beyond
-                // (2 min(w, h) - rc_max) / 2 even that may (same value from every writer)
-                if (d2 > sp.skin_half2) sc.fallback = 1;
-                const double reach = 0.5 * (2.0 * fmin(g.w, g.h) - sp.rc_max);
-                if (d2 > reach * reach) sc.violated = 1;
-            } else {
-                double r2 = d2;
-                if (ahead > 0) {
-                    const double rr = sqrt(d2) + 1.5 * ahead * sp.dt * sqrt(speed2);
-                    r2 = rr * rr;
-                }
-                if (r2 > sp.skin_half2) sc.rebuild = 1;   // same value from every writer
+            double r2 = dx * dx + dy * dy;
+            if (ahead > 0) {
+                const double rr = sqrt(r2) + 1.5 * ahead * sp.dt * sqrt(speed2);
+                r2 = rr * rr;
             }
+            if (r2 > sp.skin_half2) sc.rebuild = 1;   // same value from every writer
         }
         P.key[i] = c;
         pid = static_cast<uint32_t>(i);
@@ -459,7 +432,7 @@ __device__ inline void finalize_drift(const BatchPtrs &P, const SysParams &sp, D
     slot_reduce<1>(0u, P.partials, P.TB, sp.boff, sp.nblk, out);
     if (threadIdx.x == 0) {
         sc.K_half = out[0];
-        if (sp.kind == SK_NVT) sc.xi_mid = nh_half_update(nh_half_update(sc.xi, sc.K, sp), out[0], sp);
+        if (sp.kind == SK_NVT) sc.xi_mid[0] = nh_half_update(nh_half_update(sc.xi, sc.K, sp), out[0], sp);
     }
 }
 
@@ -627,74 +600,143 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
     }
 }
 
-// Lean step (list mode, NVE / NVT / idle slots of a rebuild window): the
-// first kick, drift and wrap of k_drift and the image of k_slot_image in one
-// pass over the cell-ordered slots of the last cell list.  No keys or movers
-// (no cell-list kernels run this step) and no drift reduction: k_box has set
-// xi_mid from the last pair pass's sums.  A particle beyond skin / 2 of its
-// build position makes the step's pair pass fall back to the 5x5 cells;
-// beyond (2 min(w, h) - rc_max) / 2 the step is flagged as not exact.
+// Lean step of a rebuild window (list mode: NVE / NVT slots, idle slots),
+// with step parity `parity`: the first kick, drift and wrap of k_drift and
+// the image of k_slot_image in one pass over the cell-ordered slots of the
+// last cell list, and the scalar work a step starts with, so that a lean step
+// is this kernel and the pair pass.  No keys or movers (no cell-list kernels
+// run this step).
+//
+// Each workgroup (kLeanThreads threads, kLeanSpan slots) of a Nose-Hoover
+// slot first combines the previous pair pass's partials itself, in
+// slot_reduce's fixed order, so every workgroup of the slot gets the same K,
+// sum m v.a and sum m |a|^2 / 2 with no synchronisation between workgroups.
+// From them: xi1 = the first half update of xi (the kick's friction), and,
+// since the kick is v' = c v + (dt/2) a with c = 1 - dt xi1 / 2, K(v') = c^2 K
+// + (c dt / 2) sum m v.a + (dt^2 / 4) sum m |a|^2 / 2 and the mid-step xi for
+// the pair pass's second kick.  The slot's first workgroup stores them:
+// xi_mid[parity] (the other entry, the previous step's, is what every
+// workgroup reads), the pass's sums as finalize_pair would, and it clears
+// fallback[parity ^ 1] for the next lean step.
+//
+// A particle beyond skin / 2 of its build position sets fallback[parity]: the
+// step's pair pass takes the 5x5 cells of the last cell list; beyond (2 min(w,
+// h) - rc_max) / 2 the step is flagged as not exact.
 template <typename AT>
-__global__ void __launch_bounds__(kBlock) k_lean(BatchPtrs P) {
+__global__ void __launch_bounds__(kLeanThreads) k_lean(BatchPtrs P, int32_t parity) {
     const int gb = xcd_block(blockIdx.x, gridDim.x);
-    const int r = P.blk2sys[gb];
+    const int r = P.lb2sys[gb];
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
-    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
-    if (li >= sp.n) return;
-    const Grid g = sc.g;
-    const uint32_t k = P.cs[sp.coff] + static_cast<uint32_t>(li);
-    const uint32_t id = P.sorted[k];
-    const uint32_t lc = P.scell[k];
-    double2 p = P.x[id];
     const int kind = sp.kind;
-    if (kind == SK_NVE || kind == SK_NVT) {
-        const double dt = sp.dt;
-        double2 q = P.v[id];
-        const double2 acc = load_acc<AT>(P.a, id);
-        if (kind == SK_NVE) {   // src/md_cuda_common.cu:381-394
-            q.x += 0.5 * dt * acc.x;
-            q.y += 0.5 * dt * acc.y;
-            p.x += dt * q.x;
-            p.y += dt * q.y;
-        } else {                // src/md_cuda_common.cu:454-478
-            const double xi1 = nh_half_update(sc.xi, sc.K, sp);
-            double vx = q.x, vy = q.y;
-            vx += 0.5 * dt * (acc.x - xi1 * vx);
-            vy += 0.5 * dt * (acc.y - xi1 * vy);
-            q.x = vx;
-            q.y = vy;
-            p.x += dt * vx;
-            p.y += dt * vy;
+    const bool moves = kind == SK_NVE || kind == SK_NVT;
+    const bool first = gb == sp.lboff && threadIdx.x == 0;
+    const uint32_t k0 = P.cs[sp.coff];
+    const int base = (gb - sp.lboff) * kLeanSpan + static_cast<int>(threadIdx.x);
+    // the particles' data first: their loads are in flight during the reduction
+    bool act[kLeanPer];
+    uint32_t ids[kLeanPer], lcs[kLeanPer];
+    double2 xs[kLeanPer], vs[kLeanPer], as[kLeanPer];
+#pragma unroll
+    for (int j = 0; j < kLeanPer; ++j) {
+        const int li = base + j * kLeanThreads;
+        act[j] = li < sp.n;
+        if (act[j]) {
+            const uint32_t k = k0 + static_cast<uint32_t>(li);
+            ids[j] = P.sorted[k];
+            lcs[j] = P.scell[k];
         }
-        p.x = pbc_wrap(p.x, g.Lx);
-        p.y = pbc_wrap(p.y, g.Ly);
-        P.v[id] = q;
-        P.x[id] = p;
     }
-    // image relative to the centre of the particle's cell in the last cell
-    // list, minimum image (the particle may have crossed the box edge since)
-    const uint32_t cy = lc / static_cast<uint32_t>(g.ncx);
-    const uint32_t cx = lc - cy * static_cast<uint32_t>(g.ncx);
-    double dx = p.x - (static_cast<double>(cx) + 0.5) * g.w;
-    double dy = p.y - (static_cast<double>(cy) + 0.5) * g.h;
-    if (dx > 0.5 * g.Lx) dx -= g.Lx;
-    else if (dx < -0.5 * g.Lx) dx += g.Lx;
-    if (dy > 0.5 * g.Ly) dy -= g.Ly;
-    else if (dy < -0.5 * g.Ly) dy += g.Ly;
-    const float2 im = make_float2(static_cast<float>(dx), static_cast<float>(dy));
-    static_cast<float2 *>(P.img)[k] = im;
-    if ((kind == SK_NVE || kind == SK_NVT) && sc.list_valid) {
-        // displacement since the build: the two images share the build cell's
-        // centre (FP32; the thresholds are lowered by 1e-5 relative so that a
-        // rounding near them errs towards the fallback)
-        const float2 i0 = P.img0[k];
-        const float ex = im.x - i0.x, ey = im.y - i0.y;
-        const float d2 = __builtin_fmaf(ex, ex, ey * ey);
-        if (d2 > static_cast<float>(sp.skin_half2) * (1.0f - 1.0e-5f)) sc.fallback = 1;
-        const float reach = 0.5f * static_cast<float>(2.0 * fmin(g.w, g.h) - sp.rc_max);
-        if (d2 > reach * reach * (1.0f - 1.0e-5f)) sc.violated = 1;
+#pragma unroll
+    for (int j = 0; j < kLeanPer; ++j) {
+        if (act[j]) {
+            xs[j] = P.x[ids[j]];
+            if (moves) {
+                vs[j] = P.v[ids[j]];
+                as[j] = load_acc<AT>(P.a, ids[j]);
+            }
+        }
     }
+    double xi1 = 0.0;
+    if (kind == SK_NVT && sp.nblk > 0) {   // uniform per workgroup
+        double out[3];
+        slot_reduce<3>(0u, P.partials, P.TB, sp.boff, sp.nblk, out);
+        const double xi = sc.xi_mid[parity ^ 1];
+        xi1 = nh_half_update(xi, out[0], sp);
+        if (first) {
+            const double c = 1.0 - 0.5 * sp.dt * xi1;
+            const double Kh = c * c * out[0] + 0.5 * c * sp.dt * out[1] + 0.25 * sp.dt * sp.dt * out[2];
+            sc.K = out[0];
+            sc.S_va = out[1];
+            sc.S_aa = out[2];
+            sc.xi = xi;
+            sc.K_half = Kh;
+            sc.xi_mid[parity] = nh_half_update(xi1, Kh, sp);
+            sc.pair_pending = 0;
+        }
+    }
+    if (first) sc.fallback[parity ^ 1] = 0;
+    const Grid g = sc.g;
+    const bool check = moves && sc.list_valid;
+    const float thr = static_cast<float>(sp.skin_half2) * (1.0f - 1.0e-5f);
+    const float reach = 0.5f * static_cast<float>(2.0 * fmin(g.w, g.h) - sp.rc_max);
+    const float thr_v = reach * reach * (1.0f - 1.0e-5f);
+    bool fb = false, bad = false;
+#pragma unroll
+    for (int j = 0; j < kLeanPer; ++j) {
+        if (!act[j]) continue;
+        const uint32_t k = k0 + static_cast<uint32_t>(base + j * kLeanThreads);
+        const uint32_t id = ids[j];
+        double2 p = xs[j];
+        if (moves) {
+            const double dt = sp.dt;
+            double2 q = vs[j];
+            const double2 acc = as[j];
+            if (kind == SK_NVE) {   // src/md_cuda_common.cu:381-394
+                q.x += 0.5 * dt * acc.x;
+                q.y += 0.5 * dt * acc.y;
+                p.x += dt * q.x;
+                p.y += dt * q.y;
+            } else {                // src/md_cuda_common.cu:454-478
+                double vx = q.x, vy = q.y;
+                vx += 0.5 * dt * (acc.x - xi1 * vx);
+                vy += 0.5 * dt * (acc.y - xi1 * vy);
+                q.x = vx;
+                q.y = vy;
+                p.x += dt * vx;
+                p.y += dt * vy;
+            }
+            p.x = pbc_wrap(p.x, g.Lx);
+            p.y = pbc_wrap(p.y, g.Ly);
+            P.v[id] = q;
+            P.x[id] = p;
+        }
+        // image relative to the centre of the particle's cell in the last cell
+        // list, minimum image (the particle may have crossed the box edge since)
+        const uint32_t lc = lcs[j];
+        const uint32_t cy = lc / static_cast<uint32_t>(g.ncx);
+        const uint32_t cx = lc - cy * static_cast<uint32_t>(g.ncx);
+        double dx = p.x - (static_cast<double>(cx) + 0.5) * g.w;
+        double dy = p.y - (static_cast<double>(cy) + 0.5) * g.h;
+        if (dx > 0.5 * g.Lx) dx -= g.Lx;
+        else if (dx < -0.5 * g.Lx) dx += g.Lx;
+        if (dy > 0.5 * g.Ly) dy -= g.Ly;
+        else if (dy < -0.5 * g.Ly) dy += g.Ly;
+        const float2 im = make_float2(static_cast<float>(dx), static_cast<float>(dy));
+        static_cast<float2 *>(P.img)[k] = im;
+        if (check) {
+            // displacement since the build: the two images share the build
+            // cell's centre (FP32; the thresholds are lowered by 1e-5 relative
+            // so that a rounding near them errs towards the fallback)
+            const float2 i0 = P.img0[k];
+            const float ex = im.x - i0.x, ey = im.y - i0.y;
+            const float d2 = __builtin_fmaf(ex, ex, ey * ey);
+            fb = fb || d2 > thr;
+            bad = bad || d2 > thr_v;
+        }
+    }
+    if (fb) sc.fallback[parity] = 1;   // same value from every writer
+    if (bad) sc.violated = 1;
 }
 
 // Neighbour-list entry (list mode): (partner slot << 3), so that entry &
@@ -935,7 +977,7 @@ __device__ inline void finalize_pair(int pmode, const double (&out)[3], const Sy
         sc.K = out[0];
         sc.S_va = out[1];
         sc.S_aa = out[2];
-        sc.xi = sc.xi_mid;
+        sc.xi = sc.xi_mid[sc.pair_parity];
         break;
     case PM_NPH: {   // src/md_env.cu:1896-1917
         const double W = out[1];
@@ -1320,8 +1362,12 @@ __device__ inline void combine_lanes(ForceSum<ACC> &F, ObsSum &ob) {
 // when some slot needs them); UNI = true when every slot of the batch has one
 // sigma for all type pairs.  A workgroup (kBlock * LPP threads) covers the
 // same kBlock particles whatever LPP is; LPP > 1 only for the FP32 pair forms.
+// parity: the step parity (0 outside rebuild windows); lean: a window's lean
+// step, whose pair pass uses the list unless the slot has a particle beyond
+// skin / 2 (fallback[parity]) or no usable list, and then the 5x5 cells of the
+// last cell list.
 template <typename IT, int ACC, bool OBS, bool UNI, int LPP>
-__global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P) {
+__global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t parity, int32_t lean) {
     using AT = std::conditional_t<ACC == ACC_COMP32, float2, double2>;
     constexpr bool kFp64 = std::is_same_v<IT, double2>;
     static_assert(!kFp64 || LPP == 1, "the FP64 form keeps the oracle's sequential order");
@@ -1370,7 +1416,8 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P) {
         }
         bool by_list = false;
         if constexpr (!kFp64) {
-            if (sc.fallback) {   // lean step the list does not cover: 5x5 cells
+            if (lean && (sc.fallback[parity] || !sc.list_on || !sc.list_built || sc.list_overflow)) {
+                // lean step the list does not cover: 5x5 cells
                 // (5 distinct cells per row and column, periodic; else the
                 // forces are not exact and the run reports it)
                 if ((g.ncx < 5 || g.ncy < 5 || !g.periodic_y) && li == 0) sc.violated = 1;
@@ -1485,7 +1532,7 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P) {
             P.v[id] = q;
             if (pmode == PM_NPH) r0 = 0.5 * pt.m[ci] * (q.x * q.x + q.y * q.y);
         } else if (pmode == PM_NVT) {
-            const double xi_mid = sc.xi_mid;
+            const double xi_mid = sc.xi_mid[parity];
             double2 q = P.v[id];
             double vx = q.x, vy = q.y;
             vx += 0.5 * sp.dt * (axd - xi_mid * vx);
@@ -1518,7 +1565,10 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P) {
     if (reduce) {
         double vals[3] = {r0, r1, r2v};
         block_partials<3, kWaves * LPP>(vals, pmode == PM_FIRE ? 1u : 0u, P.partials, P.TB, gb);
-        if (gb == sp.boff && threadIdx.x == 0) sc.pair_pending = 1;
+        if (gb == sp.boff && threadIdx.x == 0) {
+            sc.pair_pending = 1;
+            sc.pair_parity = parity;
+        }
     }
 }
 

@@ -23,6 +23,12 @@
 
 namespace mdx {
 
+// Lean-step workgroups (rebuild windows): kLeanThreads threads covering
+// kLeanSpan consecutive cell-ordered slots of one slot, kLeanPer per thread.
+constexpr int kLeanThreads = 1024;
+constexpr int kLeanPer = 4;
+constexpr int kLeanSpan = kLeanThreads * kLeanPer;
+
 enum StepKind : int32_t {
     SK_NONE = 0,     // slot idle this launch
     SK_NVE = 1,
@@ -54,6 +60,7 @@ struct SysParams {
     int32_t ccap;        // cell capacity
     int32_t boff, nblk;  // per-particle workgroups
     int32_t cboff, ncblk;
+    int32_t lboff, nlblk;  // lean-step workgroups (kLeanSpan cell-ordered slots each)
     int32_t kind;        // StepKind of the drift kernel for this slot
     int32_t pmode;       // PairMode of the pair kernel for this slot
     int32_t periodic_y;  // grid of the step: 1 except piston
@@ -82,9 +89,14 @@ struct DevScalars {
     Grid g;              // current cell grid (box lives in g.Lx, g.Ly)
     Grid g_old;          // grid of the previous cell list
     int32_t list_valid;  // the previous cell list (cs_old, sorted_old) holds every particle
-    int32_t pair_pending;  // the last pair pass left partials that k_box / k_finalize combine
+    int32_t pair_pending;  // the last pair pass left partials that k_box / k_finalize / k_lean combine
+    int32_t pair_parity;   // the step parity that pass ran with (0 outside rebuild windows)
+    int32_t pad_s;
     double xi;           // Nose-Hoover variable after the last completed half update
-    double xi_mid;       // xi of the step in flight (after both half updates)
+    // xi of a step in flight (after both half updates), by step parity: steps
+    // outside rebuild windows use [0]; a window's lean steps alternate, each
+    // reading the previous step's entry while writing its own
+    double xi_mid[2];
     double K;            // kinetic energy of the current velocities
     double K_half;       // kinetic energy after the first kick
     // sums of the last Nose-Hoover pair pass: sum m v.a and sum m |a|^2 / 2 at
@@ -104,9 +116,10 @@ struct DevScalars {
     int32_t rebuild;        // this step rebuilds the cell list (always outside list mode)
     int32_t list_built;     // the cell list was built from key_list (a list-mode build); nlist and x0 match it
     int32_t list_overflow;  // a particle had more than list_cap partners: list mode off
-    // a lean step with a particle beyond skin / 2 (or without a usable list):
-    // its pair pass takes the 5x5 cells of the last cell list instead
-    int32_t fallback;
+    // by step parity: a lean step with a particle beyond skin / 2 (set by
+    // k_lean, cleared by the step before); its pair pass takes the 5x5 cells of
+    // the last cell list instead of the list
+    int32_t fallback[2];
     // a lean step with a particle beyond the reach of that 5x5 pass
     // ((2 w - rc_max) / 2), or a grid too small for it: its forces are not exact
     int32_t violated;
@@ -139,6 +152,7 @@ struct BatchPtrs {
     DevScalars *sc;
     const int32_t *blk2sys;   // per-particle workgroup -> slot
     const int32_t *cblk2sys;  // per-cell workgroup -> slot
+    const int32_t *lb2sys;    // lean-step workgroup -> slot
     int32_t TB;               // total per-particle workgroups
     int32_t S;                // slots
     int32_t C;                // total cells
