@@ -21,8 +21,9 @@
 // k_scan_a/k_scan_b -> k_fill -> k_cell_sort (id order of the cells that
 // received movers) -> k_slot_image (the drift reduction; per-slot image,
 // coalesced) -> k_list_build (neighbour lists, list mode) -> k_pair (forces,
-// second kick, the ensemble's reduction partials).  A lean step (list mode,
-// no rebuild) is k_box -> k_drift -> k_slot_image -> k_pair.  A run of steps
+// second kick, the ensemble's reduction partials).  A lean step of a rebuild
+// window is k_lean (the previous pair pass's reduction, redone by each of its
+// workgroups, then kick/drift/image) -> k_pair.  A run of steps
 // ends with k_finalize, which combines the last pair pass's partials.  The
 // cell list is updated from the previous one: only particles whose cell
 // changed ("movers") use atomics.  Global FP64 scalars are combined from the
@@ -436,6 +437,9 @@ __device__ inline void finalize_drift(const BatchPtrs &P, const SysParams &sp, D
     }
 }
 
+// cell-list entries a per-cell thread loads at a time
+constexpr int kCellUnroll = 8;
+
 // Workgroups [0, ncb): one thread per cell of the new grid (and of the
 // previous grid when it was larger): count[c] = particles of the previous list
 // of cell c that stay in c; cells beyond the new grid get zero.  Workgroups
@@ -470,7 +474,14 @@ __global__ void __launch_bounds__(kBlock) k_stay_arrive(BatchPtrs P, int ncb) {
     uint32_t stay = 0;
     if (sp.n > 0 && lc < n_old && lc < n_new) {
         const uint32_t s0 = P.cs_old[c], s1 = P.cs_old[c + 1];
-        for (uint32_t k = s0; k < s1; ++k) stay += (P.key[P.sorted_old[k]] == c) ? 1u : 0u;
+        // eight entries at a time, their loads in flight together
+        for (uint32_t k = s0; k < s1; k += kCellUnroll) {
+            uint32_t id[kCellUnroll];
+#pragma unroll
+            for (int u = 0; u < kCellUnroll; ++u) id[u] = k + u < s1 ? P.sorted_old[k + u] : 0u;
+#pragma unroll
+            for (int u = 0; u < kCellUnroll; ++u) stay += (k + u < s1 && P.key[id[u]] == c) ? 1u : 0u;
+        }
     }
     P.count[c] = stay;
 }
@@ -504,11 +515,46 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchPtrs P, int ncb) {
         const uint32_t c = static_cast<uint32_t>(sp.coff + lc);
         uint32_t dst = P.cs[c];
         const uint32_t s0 = P.cs_old[c], s1 = P.cs_old[c + 1];
-        for (uint32_t k = s0; k < s1; ++k) {
-            const uint32_t id = P.sorted_old[k];
-            if (P.key[id] == c) P.sorted[dst++] = id;
+        for (uint32_t k = s0; k < s1; k += kCellUnroll) {
+            uint32_t id[kCellUnroll];
+            bool st[kCellUnroll];
+#pragma unroll
+            for (int u = 0; u < kCellUnroll; ++u) id[u] = k + u < s1 ? P.sorted_old[k + u] : 0u;
+#pragma unroll
+            for (int u = 0; u < kCellUnroll; ++u) st[u] = k + u < s1 && P.key[id[u]] == c;
+#pragma unroll
+            for (int u = 0; u < kCellUnroll; ++u)
+                if (st[u]) P.sorted[dst++] = id[u];
         }
     }
+}
+
+// Ascending sort of the n <= N ids at a (one thread): loaded together into
+// registers (padded with the largest key), a bitonic network with
+// compile-time indices, stored back.
+template <int N>
+__device__ inline void sort_cell(uint32_t *a, uint32_t n) {
+    uint32_t v[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = static_cast<uint32_t>(i) < n ? a[i] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t x = v[i], y = v[l];
+                    const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
+                    const bool up = (i & k) == 0;
+                    v[i] = up ? lo : hi;
+                    v[l] = up ? hi : lo;
+                }
+            }
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        if (static_cast<uint32_t>(i) < n) a[i] = v[i];
 }
 
 // One thread per cell: cells that received movers restore ascending-id order
@@ -534,7 +580,16 @@ __global__ void __launch_bounds__(kBlock) k_cell_sort(BatchPtrs P) {
     if (P.arrivals[c] == 0) return;
     P.arrivals[c] = 0;
     uint32_t *sorted = P.sorted;
-    for (uint32_t k = s0 + 1; k < s1; ++k) {
+    const uint32_t n = s1 - s0;
+    if (n <= 16u) {
+        sort_cell<16>(sorted + s0, n);
+        return;
+    }
+    if (n <= 32u) {
+        sort_cell<32>(sorted + s0, n);
+        return;
+    }
+    for (uint32_t k = s0 + 1; k < s1; ++k) {   // larger cells: insertion sort in place
         const uint32_t id = sorted[k];
         uint32_t j = k;
         while (j > s0) {
