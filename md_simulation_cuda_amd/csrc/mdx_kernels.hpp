@@ -892,21 +892,25 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
                 const uint32_t c0 = P.cs[rowbase + xr], c1 = P.cs[rowbase + xr + 1];
                 const uint32_t na = a1 - a0, nab = na + (b1 - b0), nt = nab + (c1 - c0);
                 // candidate t of the run is slot t + off(t): off = a0, b0 - na or
-                // c0 - nab by the cell it falls in (selects, no branches)
+                // c0 - nab by the cell it falls in; per cell also the particle's
+                // image moved by minus the cell's offset and the entry's cell
+                // code bits (selects, no branches)
                 const uint32_t off1 = a0, off2 = b0 - na, off3 = c0 - nab;
+                const float pxa = pi.x + g.wf, pxc = pi.x - g.wf;
+                const uint32_t cba = list_entry(0u, -1, oy, false), cbb = list_entry(0u, 0, oy, false),
+                               cbc = list_entry(0u, 1, oy, false);
                 // four candidates per pass, their loads in flight together
                 for (uint32_t t = 0; t < nt; t += 4) {
-                    uint32_t k[4];
-                    int ox[4];
-                    bool ok[4];
+                    uint32_t k[4], cb[4];
+                    float px[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const uint32_t tt = t + static_cast<uint32_t>(u);
                         const bool g1 = tt >= na, g2 = tt >= nab;
-                        const uint32_t kk = tt + (g2 ? off3 : (g1 ? off2 : off1));
-                        ok[u] = tt < nt && kk != s;
-                        k[u] = tt < nt ? kk : s;   // past the run: the particle itself, skipped
-                        ox[u] = static_cast<int>(g1) + static_cast<int>(g2) - 1;
+                        // past the run: the particle itself, rejected below
+                        k[u] = tt < nt ? tt + (g2 ? off3 : (g1 ? off2 : off1)) : s;
+                        px[u] = g2 ? pxc : (g1 ? pi.x : pxa);
+                        cb[u] = g2 ? cbc : (g1 ? cbb : cba);
                     }
                     float2 pj[4];
                     int cj[4];
@@ -917,17 +921,22 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        const float pix = pi.x - static_cast<float>(ox[u]) * g.wf;
-                        const float dx = pix - pj[u].x, dy = piy - pj[u].y;
+                        const float dx = px[u] - pj[u].x, dy = piy - pj[u].y;
                         const float rr = __builtin_fmaf(dx, dx, dy * dy);
-                        const bool acc = ok[u] && rr < rl2;
-                        near = near || (acc && rr < rg2);
+                        // 0 < r^2 (itself, and coincident particles, as in the
+                        // cell pass, are not partners) and r^2 < r_list^2, as
+                        // IEEE bit patterns
+                        const uint32_t rb = __float_as_uint(rr) - 1u;
+                        const bool in_run = t + static_cast<uint32_t>(u) < nt;
+                        const bool acc = in_run && rb < __float_as_uint(rl2) - 1u;
+                        near = near || (in_run && rb < __float_as_uint(rg2) - 1u);
                         // unconditional store: a rejected candidate, or an entry
                         // outside this pass's rows, goes to the spare row
-                        const uint32_t row = acc && cnt - base < static_cast<uint32_t>(kListStage)
-                                                 ? cnt - base
+                        const uint32_t rel = cnt - base;
+                        const uint32_t row = acc ? (rel < static_cast<uint32_t>(kListStage) ? rel
+                                                                                           : static_cast<uint32_t>(kListStage))
                                                  : static_cast<uint32_t>(kListStage);
-                        stage[row][tid] = list_entry(k[u], ox[u], oy, cj[u] == ci);
+                        stage[row][tid] = (k[u] << 3) | cb[u] | (cj[u] == ci ? kListSame : 0u);
                         cnt += acc ? 1u : 0u;
                     }
                 }
