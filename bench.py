@@ -197,6 +197,7 @@ def accuracy(frame, Lx, Ly, kw):
                      "force_max_rel_outside_cutoff_band": float(err[~band].max() / rms),
                      "particles_in_cutoff_band": int(band.sum()),
                      "force_rms_rel": float(np.sqrt((err ** 2).mean()) / rms),
+                     "force_rms_rel_outside_cutoff_band": float(np.sqrt((err[~band] ** 2).mean()) / rms),
                      "pe_per_particle_err": abs(o.U - o64.U) / len(frame),
                      "pressure_rel_err": abs(o.P - o64.P) / abs(o64.P)}
     n1 = 4096

@@ -297,10 +297,19 @@ void set_idle(SysParams &sp, const Slot &s) {
 // FP32 pair kernel with lpp lanes per particle (1, 2 or 4).
 template <int ACC, bool OBS, bool UNI>
 static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &p, int parity, int lean) {
+    if constexpr (!OBS) {   // lean steps take no energy sums
+        if (lean) {
+            switch (lpp) {
+            case 2: k_pair<float2, ACC, OBS, UNI, 2, true><<<grid, kBlock * 2, 0, st>>>(p, parity); return;
+            case 4: k_pair<float2, ACC, OBS, UNI, 4, true><<<grid, kBlock * 4, 0, st>>>(p, parity); return;
+            default: k_pair<float2, ACC, OBS, UNI, 1, true><<<grid, kBlock, 0, st>>>(p, parity); return;
+            }
+        }
+    }
     switch (lpp) {
-    case 2: k_pair<float2, ACC, OBS, UNI, 2><<<grid, kBlock * 2, 0, st>>>(p, parity, lean); break;
-    case 4: k_pair<float2, ACC, OBS, UNI, 4><<<grid, kBlock * 4, 0, st>>>(p, parity, lean); break;
-    default: k_pair<float2, ACC, OBS, UNI, 1><<<grid, kBlock, 0, st>>>(p, parity, lean); break;
+    case 2: k_pair<float2, ACC, OBS, UNI, 2, false><<<grid, kBlock * 2, 0, st>>>(p, 0); break;
+    case 4: k_pair<float2, ACC, OBS, UNI, 4, false><<<grid, kBlock * 4, 0, st>>>(p, 0); break;
+    default: k_pair<float2, ACC, OBS, UNI, 1, false><<<grid, kBlock, 0, st>>>(p, 0); break;
     }
 }
 
@@ -310,6 +319,7 @@ static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &
 int Batch::launch_step_kernels(bool obs, int mode, int parity) {
     const int S = static_cast<int>(slots.size());
     const bool lean = mode == STEP_LEAN;
+    if (lean && obs) return set_error(MDX_ERR_STATE, "a lean step takes no energy sums");
     if (!lean) parity = 0;
     // a window's head step keeps lists only if they last its window - 1 lean steps
     const int ahead = mode == STEP_HEAD ? std::max(window - 1, 0) : 0;
@@ -366,8 +376,8 @@ int Batch::launch_step_kernels(bool obs, int mode, int parity) {
             }
             break;
         default:   // the FP64 form keeps one lane per particle (the oracle's order)
-            if (obs) k_pair<double2, ACC_FP64, true, false, 1><<<TB, kBlock, 0, stream>>>(ptrs, 0, 0);
-            else k_pair<double2, ACC_FP64, false, false, 1><<<TB, kBlock, 0, stream>>>(ptrs, 0, 0);
+            if (obs) k_pair<double2, ACC_FP64, true, false, 1, false><<<TB, kBlock, 0, stream>>>(ptrs, 0);
+            else k_pair<double2, ACC_FP64, false, false, 1, false><<<TB, kBlock, 0, stream>>>(ptrs, 0);
             break;
         }
         if (e1) {

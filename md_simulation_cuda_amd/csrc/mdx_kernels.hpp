@@ -1426,12 +1426,15 @@ __device__ inline void combine_lanes(ForceSum<ACC> &F, ObsSum &ob) {
 // when some slot needs them); UNI = true when every slot of the batch has one
 // sigma for all type pairs.  A workgroup (kBlock * LPP threads) covers the
 // same kBlock particles whatever LPP is; LPP > 1 only for the FP32 pair forms.
-// parity: the step parity (0 outside rebuild windows); lean: a window's lean
-// step, whose pair pass uses the list unless the slot has a particle beyond
-// skin / 2 (fallback[parity]) or no usable list, and then the 5x5 cells of the
-// last cell list.
-template <typename IT, int ACC, bool OBS, bool UNI, int LPP>
-__global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t parity, int32_t lean) {
+// LEAN: a rebuild window's lean step with step parity `lean_parity`, whose
+// pair pass uses the list unless the slot has a particle beyond skin / 2
+// (fallback[parity]) or no usable list, and then the 5x5 cells of the last
+// cell list.  Other passes have parity 0.  (A compile-time LEAN keeps the
+// other passes' code free of the lean branch: as a runtime argument it cost
+// the cell pass about 13%.)
+template <typename IT, int ACC, bool OBS, bool UNI, int LPP, bool LEAN>
+__global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean_parity) {
+    const int32_t parity = LEAN ? lean_parity : 0;
     using AT = std::conditional_t<ACC == ACC_COMP32, float2, double2>;
     constexpr bool kFp64 = std::is_same_v<IT, double2>;
     static_assert(!kFp64 || LPP == 1, "the FP64 form keeps the oracle's sequential order");
@@ -1480,7 +1483,7 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t pari
         }
         bool by_list = false;
         if constexpr (!kFp64) {
-            if (lean && (sc.fallback[parity] || !sc.list_on || !sc.list_built || sc.list_overflow)) {
+            if (LEAN && (sc.fallback[parity] || !sc.list_on || !sc.list_built || sc.list_overflow)) {
                 // lean step the list does not cover: 5x5 cells
                 // (5 distinct cells per row and column, periodic; else the
                 // forces are not exact and the run reports it)
