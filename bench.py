@@ -50,11 +50,15 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-vs-FP64 error report")
-    ap.add_argument("--workload", default="nvt1m", choices=["nvt1m", "ensemble", "serve", "fire_nph", "nve4k"],
+    ap.add_argument("--workload", default="nvt1m",
+                    choices=["nvt1m", "ensemble", "serve", "fire_nph", "nve4k", "long"],
                     help="nvt1m: N=1M Nose-Hoover (default line); ensemble: 256 NVT replicas of uneven N "
                          "in one batch vs per-replica launches; serve: Poisson job stream through the "
                          "continuous-batching server; fire_nph: FIRE from a random start at N=262144 "
-                         "then NPH; nve4k: NVE N=4096 with the FP64 oracle timed beside it")
+                         "then NPH; nve4k: NVE N=4096 with the FP64 oracle timed beside it; long: the "
+                         "N=1M NVT step timed after up to --long-steps steps, with and without the "
+                         "storage reorder")
+    ap.add_argument("--long-steps", type=int, default=60000)
     ap.add_argument("--tune-steps", type=int, default=40,
                     help="steps per autotuner candidate before the timed run (0: default tuning)")
     ap.add_argument("--tuning", default=None, metavar="CELL_SCALE,LANES,SKIN[,WINDOW]",
@@ -502,6 +506,60 @@ def run_fire_nph(args):
     print(json.dumps(out), flush=True)
 
 
+def run_long(args):
+    """The N=1M NVT workload over a long run: 400 steps timed after 0, 2K,
+    10K, 30K and 60K steps (up to --long-steps), once with the storage
+    reorder at its default interval and once with it off.  As particles
+    diffuse, storage order drifts from cell order unless it is reordered;
+    the gathers through the cell list then lose coalescing."""
+    from md_simulation_cuda_amd import abi
+
+    frame, Lx, Ly, kw = make_workload(args.n)
+    n = len(frame)
+    marks = [m for m in (0, 2000, 10000, 30000, 60000) if m <= args.long_steps]
+    tuning = None
+    if args.tuning:
+        parts = args.tuning.split(",") + ["0"]
+        tuning = (float(parts[0]), int(parts[1]), float(parts[2]), int(parts[3]))
+    runs = {}
+    for label, every in (("reorder every 1000 steps", 1000), ("no reorder", 0)):
+        sim = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), frame)
+        sim.set_reorder(every)
+        if tuning:
+            sim.set_tuning(*tuning)
+        elif args.tune_steps > 0:
+            rep = sim.autotune(abi.NVT, args.tune_steps)
+            b = rep.best
+            tuning = (b.cell_scale, b.lanes_per_particle, b.skin, b.window)
+        done, rows = 0, []
+        for m in marks:
+            if m > done:
+                sim.step(abi.NVT, m - done)
+                done = m
+            sim.sync()
+            t0 = time.perf_counter()
+            sim.step(abi.NVT, 400)
+            sim.sync()
+            ms = (time.perf_counter() - t0) * 1e3 / 400
+            done += 400
+            rows.append({"after_steps": m, "ms_per_step": ms})
+            print(f"# {label}: after {m} steps {ms:.4f} ms/step", file=sys.stderr, flush=True)
+        sim.close()
+        runs[label] = rows
+    out = {
+        "metric": "ms/step over a long NVT run @N=1M",
+        "unit": "ms/step",
+        "higher_is_better": False,
+        "n_gpus": 1,
+        "dtype": "fp32 pair + fp32 two-sum accumulation, fp64 state",
+        "data": "synthetic (reference init_particles, phase-separated, seeded)",
+        "config": {"workload": "NVT Nose-Hoover, 2D binary LJ, N=%d, 2:1 box rho=0.75, T=0.6, Q=10, dt=0.005" % n,
+                   "tuning": tuning, "timed_steps_per_mark": 400},
+        "runs": runs,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_nve4k(args):
     """NVE, N=4096 square lattice (density 0.8, T=0.7, eps_AB 0.5, dt 0.005,
     seed 1001), 1000 steps, with the FP64 oracle timed on the host beside it."""
@@ -569,6 +627,9 @@ def main():
         return
     if args.workload == "nve4k":
         run_nve4k(args)
+        return
+    if args.workload == "long":
+        run_long(args)
         return
     from md_simulation_cuda_amd import abi
 

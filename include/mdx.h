@@ -174,6 +174,12 @@ int mdx_compute_forces(mdx_sim *sim, int32_t periodic_y, double *acc_out);
 int mdx_get_n(const mdx_sim *sim, int64_t *n);
 /* Replay each step from a captured hipGraph (default on). */
 int mdx_set_graph(mdx_sim *sim, int32_t enable);
+/* (new) Steps between two reorders of the particle storage into cell order,
+ * which keeps the kernels' particle accesses coalesced as particles diffuse
+ * away from their load order (default 1000; 0: never).  Results do not
+ * depend on it: the cell order sorts by original index and every reduction
+ * runs in cell order. */
+int mdx_set_reorder(mdx_sim *sim, int32_t steps);
 int mdx_set_precision(mdx_sim *sim, int32_t precision);
 int mdx_set_dt(mdx_sim *sim, double dt);
 int mdx_sync(mdx_sim *sim);
@@ -282,6 +288,7 @@ int mdx_batch_fire(mdx_batch *b, int32_t slot, const mdx_fire_params *fp, double
                    double *energy_trace, int32_t cap);
 int mdx_batch_sync(mdx_batch *b);
 int mdx_batch_set_graph(mdx_batch *b, int32_t enable);
+int mdx_batch_set_reorder(mdx_batch *b, int32_t steps);   /* as mdx_set_reorder */
 int mdx_batch_timing_enable(mdx_batch *b, int32_t enable);
 int mdx_batch_timing_read(mdx_batch *b, mdx_timing *out);
 /* FIRE as a slot kind: after mdx_batch_fire_begin the slot takes one FIRE

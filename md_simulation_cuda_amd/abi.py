@@ -184,6 +184,7 @@ _SIGNATURES = {
     "mdx_set_precision": (C.c_int, [_P, C.c_int32]),
     "mdx_set_dt": (C.c_int, [_P, C.c_double]),
     "mdx_set_graph": (C.c_int, [_P, C.c_int32]),
+    "mdx_set_reorder": (C.c_int, [_P, C.c_int32]),
     "mdx_cell_capacity": (C.c_int, [C.POINTER(Params), C.c_double, C.POINTER(C.c_int32)]),
     "mdx_batch_create": (C.c_int, [C.c_int32, _P, _P, C.c_int32, C.c_int32, C.POINTER(_P)]),
     "mdx_batch_destroy": (C.c_int, [_P]),
@@ -197,6 +198,7 @@ _SIGNATURES = {
                                  C.POINTER(FireResult), _P, _P, C.c_int32]),
     "mdx_batch_sync": (C.c_int, [_P]),
     "mdx_batch_set_graph": (C.c_int, [_P, C.c_int32]),
+    "mdx_batch_set_reorder": (C.c_int, [_P, C.c_int32]),
     "mdx_batch_timing_enable": (C.c_int, [_P, C.c_int32]),
     "mdx_batch_timing_read": (C.c_int, [_P, C.POINTER(Timing)]),
     "mdx_sync": (C.c_int, [_P]),
@@ -382,6 +384,9 @@ class Simulation:
     def set_graph(self, on: bool) -> None:
         check(self._lib.mdx_set_graph(self._h, 1 if on else 0))
 
+    def set_reorder(self, steps: int) -> None:
+        check(self._lib.mdx_set_reorder(self._h, steps))
+
     def set_tuning(self, cell_scale: float = 1.0, lanes_per_particle: int = 1, skin: float = 0.0,
                    window: int = 0) -> None:
         t = Tuning(cell_scale, skin, lanes_per_particle, window)
@@ -485,6 +490,9 @@ class Batch:
 
     def set_graph(self, on: bool) -> None:
         check(self._lib.mdx_batch_set_graph(self._h, 1 if on else 0))
+
+    def set_reorder(self, steps: int) -> None:
+        check(self._lib.mdx_batch_set_reorder(self._h, steps))
 
     def set_tuning(self, cell_scale: float = 1.0, lanes_per_particle: int = 1, skin: float = 0.0,
                    window: int = 0) -> None:

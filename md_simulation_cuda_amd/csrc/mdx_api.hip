@@ -224,6 +224,13 @@ int mdx_set_graph(mdx_sim *sim, int32_t enable) {
     return MDX_OK;
 }
 
+int mdx_set_reorder(mdx_sim *sim, int32_t steps) {
+    CHECK_HANDLE(sim);
+    if (steps < 0) return set_error(MDX_ERR_INVALID, "reorder interval must be >= 0");
+    sim->b.reorder_every = steps;
+    return MDX_OK;
+}
+
 int mdx_sync(mdx_sim *sim) {
     CHECK_HANDLE(sim);
     return sim->b.sync();
@@ -320,6 +327,13 @@ int mdx_batch_sync(mdx_batch *b) {
 int mdx_batch_set_graph(mdx_batch *b, int32_t enable) {
     CHECK_HANDLE(b);
     b->b.use_graph = enable != 0;
+    return MDX_OK;
+}
+
+int mdx_batch_set_reorder(mdx_batch *b, int32_t steps) {
+    CHECK_HANDLE(b);
+    if (steps < 0) return set_error(MDX_ERR_INVALID, "reorder interval must be >= 0");
+    b->b.reorder_every = steps;
     return MDX_OK;
 }
 

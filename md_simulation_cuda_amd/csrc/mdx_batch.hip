@@ -111,7 +111,7 @@ Batch::~Batch() {
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
                      d_sp, d_sc, d_blk2sys, d_cblk2sys, d_lb2sys, d_want, sorted_old, cs_old, movers, mover_off,
-                     n_movers, arrivals, key_list, x0, img0, nlist, ncount};
+                     n_movers, arrivals, key_list, x0, img0, nlist, ncount, pid, sid, reorder_tmp};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
     for (auto *vec : {&force_ev, &step_ev})
@@ -217,6 +217,8 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     MDX_HIP(hipMalloc(&d_cblk2sys, std::max<size_t>(cb2s.size(), 1) * 4));
     MDX_HIP(hipMalloc(&d_lb2sys, std::max<size_t>(lb2s.size(), 1) * 4));
     MDX_HIP(hipMalloc(&d_want, nslots * 4));
+    MDX_HIP(hipMalloc(&pid, n4));
+    MDX_HIP(hipMalloc(&sid, n4));
     MDX_HIP(hipMemset(count, 0, (static_cast<size_t>(C) + 1) * 4));
     MDX_HIP(hipMemset(d_sc, 0, nslots * sizeof(DevScalars)));
     MDX_HIP(hipMemset(x, 0, n16));
@@ -249,6 +251,12 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     ptrs.blk2sys = d_blk2sys;
     ptrs.cblk2sys = d_cblk2sys;
     ptrs.lb2sys = d_lb2sys;
+    ptrs.pid = pid;
+    ptrs.sid = sid;
+    if (N > 0) {
+        k_ident_ids<<<ceil_div(N, kBlock), kBlock, 0, stream>>>(pid, sid, 0u, static_cast<uint32_t>(N));
+        MDX_HIP(hipGetLastError());
+    }
     ptrs.TB = TB;
     ptrs.S = nslots;
     ptrs.C = C;
@@ -569,6 +577,7 @@ int Batch::load(int r, const mdx_params &p, const mdx_particle64 *frame, int64_t
     MDX_HIP(hipMemcpyAsync(x + sp.off, hx.data(), n * 16, hipMemcpyHostToDevice, stream));
     MDX_HIP(hipMemcpyAsync(v + sp.off, hv.data(), n * 16, hipMemcpyHostToDevice, stream));
     MDX_HIP(hipMemcpyAsync(cls + sp.off, hc.data(), n, hipMemcpyHostToDevice, stream));
+    MDX_TRY(reset_ids(r));
     DevScalars h{};
     h.g = g;
     h.gamma = p.Lx / p.Ly;
@@ -611,6 +620,14 @@ int Batch::set_state(int r, const mdx_particle64 *frame, int64_t n) {
     MDX_HIP(hipMemcpyAsync(x + off, hx.data(), n * 16, hipMemcpyHostToDevice, stream));
     MDX_HIP(hipMemcpyAsync(v + off, hv.data(), n * 16, hipMemcpyHostToDevice, stream));
     MDX_HIP(hipMemcpyAsync(cls + off, hc.data(), n, hipMemcpyHostToDevice, stream));
+    MDX_TRY(reset_ids(r));
+    {   // the cell list names the old storage: the next pass builds it anew
+        DevScalars h;
+        MDX_TRY(read_scalars(r, h));
+        h.list_valid = 0;
+        h.list_built = 0;
+        MDX_TRY(write_scalar_block(r, h));
+    }
     MDX_HIP(hipStreamSynchronize(stream));
     s.piston_active = false;
     s.K_valid = s.W_valid = false;
@@ -797,7 +814,19 @@ int Batch::step(int nsteps) {
     }
     if (!any) return MDX_OK;
     MDX_TRY(upload_all_sp());
-    MDX_TRY(run_passes(nsteps, obs));
+    // a long run goes in chunks of reorder_every steps (with rebuild windows,
+    // whole windows, so the windows fall as in one run); before each, the
+    // slots that have run that many steps since their storage was put in cell
+    // order are reordered (no effect on results)
+    for (int left = nsteps; left > 0;) {
+        MDX_TRY(reorder_due());
+        int chunk = reorder_every > 0 ? std::min(left, reorder_every) : left;
+        if (chunk < left && windows_apply()) chunk = std::min(left, std::max(window, chunk - chunk % window));
+        MDX_TRY(run_passes(chunk, obs));
+        left -= chunk;
+        for (Slot &s : slots)
+            if (s.loaded && s.kind != SK_NONE) s.since_reorder += chunk;
+    }
     if (timing) step_n += nsteps;
     for (size_t r = 0; r < slots.size(); ++r) {
         Slot &s = slots[r];
@@ -893,27 +922,32 @@ int Batch::collect(int r, mdx_particle64 *out, int64_t cap) {
     std::vector<double2> hx(n), hv(n);
     std::vector<unsigned char> ha(n * 16);
     const bool f32 = prec == MDX_PREC_COMP32;
+    std::vector<uint32_t> hp(n);
     MDX_HIP(hipMemcpyAsync(hx.data(), x + off, n * 16, hipMemcpyDeviceToHost, stream));
     MDX_HIP(hipMemcpyAsync(hv.data(), v + off, n * 16, hipMemcpyDeviceToHost, stream));
     if (f32)
         MDX_HIP(hipMemcpyAsync(ha.data(), static_cast<float2 *>(a) + off, n * 8, hipMemcpyDeviceToHost, stream));
     else
         MDX_HIP(hipMemcpyAsync(ha.data(), static_cast<double2 *>(a) + off, n * 16, hipMemcpyDeviceToHost, stream));
+    MDX_HIP(hipMemcpyAsync(hp.data(), pid + off, n * 4, hipMemcpyDeviceToHost, stream));
     MDX_HIP(hipStreamSynchronize(stream));
-    for (int64_t i = 0; i < n; ++i) {
+    // storage index i holds original particle pid[i] - off
+    for (int64_t si = 0; si < n; ++si) {
+        const int64_t i = static_cast<int64_t>(hp[si]) - off;
+        if (i < 0 || i >= n) return set_error(MDX_ERR_STATE, "particle index map out of range");
         mdx_particle64 &q = out[i];
         std::memset(&q, 0, sizeof(q));
-        q.pos[0] = hx[i].x;
-        q.pos[1] = hx[i].y;
-        q.vel[0] = hv[i].x;
-        q.vel[1] = hv[i].y;
+        q.pos[0] = hx[si].x;
+        q.pos[1] = hx[si].y;
+        q.vel[0] = hv[si].x;
+        q.vel[1] = hv[si].y;
         if (f32) {
             float fa[2];
-            std::memcpy(fa, &ha[i * 8], 8);
+            std::memcpy(fa, &ha[si * 8], 8);
             q.acc[0] = fa[0];
             q.acc[1] = fa[1];
         } else {
-            std::memcpy(q.acc, &ha[i * 16], 16);
+            std::memcpy(q.acc, &ha[si * 16], 16);
         }
         q.type = s.type[i];
         q.theta = s.theta[i];
@@ -940,19 +974,90 @@ int Batch::cell_list(int r, int periodic_y, int32_t *ncx, int32_t *ncy, uint32_t
     MDX_HIP(hipMemcpyAsync(st.data(), cs + sp.coff, (ncell + 1) * 4, hipMemcpyDeviceToHost, stream));
     MDX_HIP(hipStreamSynchronize(stream));
     const uint32_t base = st[0];
+    // device arrays are in storage order: map through pid to original indices
+    std::vector<uint32_t> hp(s.n);
+    MDX_HIP(hipMemcpyAsync(hp.data(), pid + sp.off, s.n * 4, hipMemcpyDeviceToHost, stream));
+    MDX_HIP(hipStreamSynchronize(stream));
+    const uint32_t off = static_cast<uint32_t>(sp.off);
+    for (int64_t si = 0; si < s.n; ++si)
+        if (hp[si] < off || hp[si] - off >= static_cast<uint64_t>(s.n))
+            return set_error(MDX_ERR_STATE, "particle index map out of range");
     if (keys) {
-        MDX_HIP(hipMemcpyAsync(keys, key + sp.off, s.n * 4, hipMemcpyDeviceToHost, stream));
+        std::vector<uint32_t> hk(s.n);
+        MDX_HIP(hipMemcpyAsync(hk.data(), key + sp.off, s.n * 4, hipMemcpyDeviceToHost, stream));
         MDX_HIP(hipStreamSynchronize(stream));
-        for (int64_t i = 0; i < s.n; ++i) keys[i] -= static_cast<uint32_t>(sp.coff);
+        for (int64_t si = 0; si < s.n; ++si) keys[hp[si] - off] = hk[si] - static_cast<uint32_t>(sp.coff);
     }
     if (cell_start)
         for (int64_t c = 0; c <= ncell; ++c) cell_start[c] = st[c] - base;
     if (sorted_ids) {
-        MDX_HIP(hipMemcpyAsync(sorted_ids, sorted + base, s.n * 4, hipMemcpyDeviceToHost, stream));
+        std::vector<uint32_t> hs(s.n);
+        MDX_HIP(hipMemcpyAsync(hs.data(), sorted + base, s.n * 4, hipMemcpyDeviceToHost, stream));
         MDX_HIP(hipStreamSynchronize(stream));
-        for (int64_t i = 0; i < s.n; ++i) sorted_ids[i] -= static_cast<uint32_t>(sp.off);
+        for (int64_t k = 0; k < s.n; ++k) {
+            const uint32_t si = hs[k] - off;
+            if (si >= static_cast<uint64_t>(s.n)) return set_error(MDX_ERR_STATE, "cell list entry out of range");
+            sorted_ids[k] = hp[si] - off;
+        }
     }
     return MDX_OK;
+}
+
+int Batch::reset_ids(int r) {
+    const SysParams &sp = h_sp[r];
+    slots[r].since_reorder = 0;
+    if (sp.n_cap <= 0) return MDX_OK;
+    k_ident_ids<<<ceil_div(sp.n_cap, kBlock), kBlock, 0, stream>>>(pid, sid, static_cast<uint32_t>(sp.off),
+                                                                   static_cast<uint32_t>(sp.off + sp.n_cap));
+    MDX_HIP(hipGetLastError());
+    return MDX_OK;
+}
+
+namespace {
+template <typename T>
+void reorder_array(const Batch &b, T *arr) {
+    T *tmp = static_cast<T *>(b.reorder_tmp);
+    k_gather_cell_order<T><<<b.TB, kBlock, 0, b.stream>>>(b.ptrs, b.d_want, arr, tmp);
+    k_copy_slots<T><<<b.TB, kBlock, 0, b.stream>>>(b.ptrs, b.d_want, tmp, arr);
+}
+}  // namespace
+
+// Storage of the slots in `which` put in their current cell order: every
+// per-particle array gathered through the cell list (sorted), which then maps
+// cell-ordered slot k to storage k.  Passes read particle data through the
+// cell list, so later accesses are coalesced; the cell order itself (by
+// original index) and all reductions (in cell order) are unchanged, so
+// results do not depend on when this runs.  Between steps only: the cell
+// list is complete (list_valid) and sorted_old equals sorted.
+int Batch::reorder(const std::vector<int> &which) {
+    if (which.empty() || TB <= 0) return MDX_OK;
+    if (!reorder_tmp) MDX_HIP(hipMalloc(&reorder_tmp, static_cast<size_t>(std::max<int64_t>(N, 1)) * 16));
+    std::fill(h_want.begin(), h_want.end(), 0);
+    for (int r : which) h_want[r] = 1;
+    MDX_HIP(hipMemcpyAsync(d_want, h_want.data(), h_want.size() * 4, hipMemcpyHostToDevice, stream));
+    reorder_array(*this, x);
+    reorder_array(*this, v);
+    if (prec == MDX_PREC_COMP32) reorder_array(*this, static_cast<float2 *>(a));
+    else reorder_array(*this, static_cast<double2 *>(a));
+    reorder_array(*this, cls);
+    reorder_array(*this, key);
+    reorder_array(*this, pid);
+    if (key_list) reorder_array(*this, key_list);
+    if (x0) reorder_array(*this, x0);
+    k_reorder_ids<<<TB, kBlock, 0, stream>>>(ptrs, d_want);
+    MDX_HIP(hipGetLastError());
+    for (int r : which) slots[r].since_reorder = 0;
+    return MDX_OK;
+}
+
+int Batch::reorder_due() {
+    std::vector<int> due;
+    for (size_t r = 0; r < slots.size(); ++r) {
+        const Slot &s = slots[r];
+        if (reorder_every > 0 && s.loaded && s.cells_valid && s.since_reorder >= reorder_every)
+            due.push_back(static_cast<int>(r));
+    }
+    return reorder(due);
 }
 
 int Batch::set_precision(int precision) {

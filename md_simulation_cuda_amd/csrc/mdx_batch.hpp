@@ -15,6 +15,10 @@ namespace mdx {
 
 int set_error(int code, const std::string &msg);
 
+// Steps a slot runs between two reorders of its storage into cell order
+// (Batch::reorder); reordering has no effect on results.
+constexpr int kReorderSteps = 1000;
+
 struct Slot {
     // capacity (fixed when the batch is built)
     int64_t n_cap = 0;
@@ -37,6 +41,7 @@ struct Slot {
     int cells_py = 1;
     bool K_valid = false, W_valid = false;
     bool piston_active = false, nph_init = false, vp_init = false;
+    int64_t since_reorder = 0;   // steps since the storage was last put in cell order
     // what the next step does
     int kind = SK_NONE;
     uint32_t flags = 0;
@@ -79,6 +84,9 @@ struct Batch {
     int list_cap = 64;       // neighbour-list entries per particle
     int window = 0;          // list mode: steps per rebuild window
     uint32_t *key_list = nullptr, *nlist = nullptr, *ncount = nullptr;
+    uint32_t *pid = nullptr, *sid = nullptr;   // storage <-> original index maps
+    int reorder_every = kReorderSteps;         // 0: storage stays in load order
+    void *reorder_tmp = nullptr;              // 16 B per particle, allocated by the first reorder
     double2 *x0 = nullptr;
     float2 *img0 = nullptr;
     // timing
@@ -148,6 +156,10 @@ struct Batch {
     double cell_side_for(const Slot &s) const;   // cell side of a slot under the batch's tuning
     int ensure_list_arrays();
     int write_scalar_block(int r, const DevScalars &h);
+    // slots r in `which`: storage put in the slot's cell order (no effect on results)
+    int reorder(const std::vector<int> &which);
+    int reorder_due();                        // the slots whose kReorderSteps have run
+    int reset_ids(int r);                     // storage of slot r in load order
 };
 
 }  // namespace mdx

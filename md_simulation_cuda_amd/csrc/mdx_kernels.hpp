@@ -226,6 +226,16 @@ __device__ inline void append_movers(const BatchPtrs &P, bool mover, uint32_t pi
 // drift kernel
 // --------------------------------------------------------------------------
 
+// Storage index of a slot's li-th particle in the canonical order, so that
+// per-workgroup reductions do not depend on the storage order: its
+// cell-ordered slot's particle when the slot has a cell list (the cell-ordered
+// particles of slot r sit at [off_r, off_r + n_r)), else the storage order
+// (then the load order, storage being the identity until a cell list exists).
+__device__ inline uint32_t canon_index(const BatchPtrs &P, const SysParams &sp, const DevScalars &sc, int li) {
+    const uint32_t k = static_cast<uint32_t>(sp.off + li);
+    return sc.list_valid ? P.sorted[k] : k;
+}
+
 // Kick / drift / wrap of one particle by its slot's ensemble, then its cell
 // key.  Restates src/md_cuda_common.cu:381-496 (VV, NH, NPH scale+drift,
 // piston) and src/md_energy_minimizer.cu:32-71 (FIRE zero/mix/drift).  Idle
@@ -253,9 +263,9 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t ahead) {
     const bool moves = kind == SK_NVE || kind == SK_NVT || kind == SK_NPH || kind == SK_PISTON ||
                        (kind == SK_FIRE && sc.f_do_drift);
     bool mover = false;
-    uint32_t pid = 0;
+    uint32_t mover_id = 0;
     if (li < sp.n) {
-        const int i = sp.off + li;
+        const uint32_t i = canon_index(P, sp, sc, li);
         double2 p = P.x[i];
         const double dt = sp.dt;
         double speed2 = 0.0;   // |v| after this step's kick
@@ -354,9 +364,9 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t ahead) {
             if (r2 > sp.skin_half2) sc.rebuild = 1;   // same value from every writer
         }
         P.key[i] = c;
-        pid = static_cast<uint32_t>(i);
+        mover_id = i;
     }
-    append_movers(P, mover, pid, gb);
+    append_movers(P, mover, mover_id, gb);
     if (reduce) {
         double vals[1] = {red};
         block_partials<1>(vals, 0u, P.partials, P.TB, gb);
@@ -529,14 +539,17 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchPtrs P, int ncb) {
     }
 }
 
-// Ascending sort of the n <= N ids at a (one thread): loaded together into
-// registers (padded with the largest key), a bitonic network with
-// compile-time indices, stored back.
+// Sort of the n <= N storage indices at a by their original index (one
+// thread): the original indices loaded together into registers (padded with
+// the largest key), a bitonic network with compile-time indices, stored back
+// as storage indices.
 template <int N>
-__device__ inline void sort_cell(uint32_t *a, uint32_t n) {
+__device__ inline void sort_cell(uint32_t *a, uint32_t n, const uint32_t *pid, const uint32_t *sid) {
     uint32_t v[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = static_cast<uint32_t>(i) < n ? a[i] : 0xFFFFFFFFu;
+    for (int i = 0; i < N; ++i) v[i] = static_cast<uint32_t>(i) < n ? a[i] : 0u;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = static_cast<uint32_t>(i) < n ? pid[v[i]] : 0xFFFFFFFFu;
 #pragma unroll
     for (int k = 2; k <= N; k <<= 1)
 #pragma unroll
@@ -554,12 +567,13 @@ __device__ inline void sort_cell(uint32_t *a, uint32_t n) {
             }
 #pragma unroll
     for (int i = 0; i < N; ++i)
-        if (static_cast<uint32_t>(i) < n) a[i] = v[i];
+        if (static_cast<uint32_t>(i) < n) a[i] = sid[v[i]];
 }
 
-// One thread per cell: cells that received movers restore ascending-id order
-// (stayers are in order, the scatter's atomics leave the arrivals in any
-// order); every cell records its range for the next step's update.
+// One thread per cell: cells that received movers restore the order of
+// ascending original index (stayers are in order, the scatter's atomics leave
+// the arrivals in any order); every cell records its range for the next
+// step's update.
 __global__ void __launch_bounds__(kBlock) k_cell_sort(BatchPtrs P) {
     const int gb = xcd_block(blockIdx.x, gridDim.x);
     const int r = P.cblk2sys[gb];
@@ -582,19 +596,20 @@ __global__ void __launch_bounds__(kBlock) k_cell_sort(BatchPtrs P) {
     uint32_t *sorted = P.sorted;
     const uint32_t n = s1 - s0;
     if (n <= 16u) {
-        sort_cell<16>(sorted + s0, n);
+        sort_cell<16>(sorted + s0, n, P.pid, P.sid);
         return;
     }
     if (n <= 32u) {
-        sort_cell<32>(sorted + s0, n);
+        sort_cell<32>(sorted + s0, n, P.pid, P.sid);
         return;
     }
     for (uint32_t k = s0 + 1; k < s1; ++k) {   // larger cells: insertion sort in place
         const uint32_t id = sorted[k];
+        const uint32_t key = P.pid[id];
         uint32_t j = k;
         while (j > s0) {
             const uint32_t prev = sorted[j - 1];
-            if (prev <= id) break;
+            if (P.pid[prev] <= key) break;
             sorted[j] = prev;
             --j;
         }
@@ -1690,7 +1705,7 @@ __global__ void __launch_bounds__(kBlock) k_kinetic(BatchPtrs P, const int32_t *
     const int li = (gb - sp.boff) * kBlock + threadIdx.x;
     double k = 0.0;
     if (li < sp.n) {
-        const int i = sp.off + li;
+        const uint32_t i = canon_index(P, sp, P.sc[r], li);
         const double2 q = P.v[i];
         k = 0.5 * sp.pt.m[P.cls[i]] * (q.x * q.x + q.y * q.y);
     }
@@ -1709,7 +1724,7 @@ __global__ void __launch_bounds__(kBlock) k_walls(BatchPtrs P, const int32_t *wa
     const int li = (gb - sp.boff) * kBlock + threadIdx.x;
     double up = 0.0, e = 0.0;
     if (li < sp.n && k > 0.0) {
-        const double y = P.x[sp.off + li].y;
+        const double y = P.x[canon_index(P, sp, P.sc[r], li)].y;
         if (y > Ly) {
             const double ov = y - Ly;
             up += k * ov;
@@ -1742,6 +1757,64 @@ __global__ void __launch_bounds__(kBlock) k_fire_init(BatchPtrs P, const int32_t
     P.x[i] = p;
     P.v[i] = make_double2(0.0, 0.0);
     static_cast<double2 *>(P.a)[i] = make_double2(0.0, 0.0);
+}
+
+// --------------------------------------------------------------------------
+// storage reorder (Batch::reorder) and the identity id maps
+// --------------------------------------------------------------------------
+
+// Slots with want[r]: dst[off + li] = src[sorted[off + li]], one per-particle
+// array gathered into the slot's cell order (one thread per cell-ordered
+// slot).  Every array is gathered with the same sorted[] before
+// k_reorder_ids rewrites it.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gather_cell_order(BatchPtrs P, const int32_t *want, const T *src,
+                                                              T *dst) {
+    const int gb = blockIdx.x;
+    const int r = P.blk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    if (!want[r]) return;
+    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
+    if (li >= sp.n) return;
+    const uint32_t k = static_cast<uint32_t>(sp.off + li);
+    dst[k] = src[P.sorted[k]];
+}
+
+// Slots with want[r]: dst[k] = src[k] over their particles.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_copy_slots(BatchPtrs P, const int32_t *want, const T *src, T *dst) {
+    const int gb = blockIdx.x;
+    const int r = P.blk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    if (!want[r]) return;
+    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
+    if (li >= sp.n) return;
+    const uint32_t k = static_cast<uint32_t>(sp.off + li);
+    dst[k] = src[k];
+}
+
+// After the gathers: cell-ordered slot k holds the particle stored at k (the
+// previous cell list, sorted_old, is the current one between steps), and the
+// inverse map follows the gathered pid.
+__global__ void __launch_bounds__(kBlock) k_reorder_ids(BatchPtrs P, const int32_t *want) {
+    const int gb = blockIdx.x;
+    const int r = P.blk2sys[gb];
+    const SysParams &sp = P.sp[r];
+    if (!want[r]) return;
+    const int li = (gb - sp.boff) * kBlock + threadIdx.x;
+    if (li >= sp.n) return;
+    const uint32_t k = static_cast<uint32_t>(sp.off + li);
+    P.sorted[k] = k;
+    P.sorted_old[k] = k;
+    P.sid[P.pid[k]] = k;
+}
+
+// pid[i] = sid[i] = i for i in [lo, hi): storage in load order.
+__global__ void __launch_bounds__(kBlock) k_ident_ids(uint32_t *pid, uint32_t *sid, uint32_t lo, uint32_t hi) {
+    const uint32_t i = lo + blockIdx.x * kBlock + threadIdx.x;
+    if (i >= hi) return;
+    pid[i] = i;
+    sid[i] = i;
 }
 
 }  // namespace mdx

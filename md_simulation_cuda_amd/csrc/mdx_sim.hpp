@@ -148,6 +148,12 @@ struct BatchPtrs {
     double2 *x0;
     float2 *img0;     // per cell-ordered slot: its FP32 image at the last list-mode build
     uint32_t *nlist, *ncount;
+    // storage order: particle data (x, v, a, cls, key, key_list, x0) sits at
+    // storage index i; pid[i] is its original (load-order) global index and
+    // sid[pid[i]] = i.  The cell order sorts by pid, so it, and every result,
+    // is independent of the storage order, which a slot's periodic reorder
+    // (Batch::reorder) sets to its cell order for coalesced access.
+    uint32_t *pid, *sid;
     const SysParams *sp;
     DevScalars *sc;
     const int32_t *blk2sys;   // per-particle workgroup -> slot

@@ -1,0 +1,145 @@
+"""Storage reorder: a slot's particle arrays are permuted into its cell order
+every so many steps (mdx_set_reorder) so that the kernels' accesses stay
+coalesced as particles diffuse.  The cell order sorts by original index and
+every reduction runs in cell order, so results must not depend on the
+reorder interval at all: frames, accelerations and observables after a run
+that reorders every few steps equal those of a run that never reorders, bit
+for bit, in every ensemble and tuning; the cell list after reorders still
+equals the FP64 oracle's; a frame set mid-run (set_state) and the batch
+path behave the same."""
+import sys
+
+import numpy as np
+import pytest
+
+from md_simulation_cuda_amd import abi
+from oracle import oracle as O
+from tests.helpers import lattice_frame
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run(f, Lx, Ly, kw, kind, steps, reorder, tuning=None, precision=abi.PREC_COMP32, p_ext=0.0):
+    s = abi.Simulation(abi.make_params(Lx, Ly, precision=precision, **kw), f)
+    s.set_reorder(reorder)
+    if tuning:
+        s.set_tuning(*tuning)
+    for chunk in steps:
+        s.step(kind, chunk, p_ext)
+    out = s.collect()
+    ob = s.observables()
+    s.close()
+    return out, ob
+
+
+def same(a, b):
+    fa, oa = a
+    fb, ob = b
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(fa[k], fb[k]), k
+    assert np.array_equal(fa["type"], fb["type"]) and np.array_equal(fa["theta"], fb["theta"])
+    for k in ("K", "U", "P", "xi", "Lx", "Ly", "p_A"):
+        assert getattr(oa, k) == getattr(ob, k), k
+
+
+CASES = [
+    ("nvt cell path", abi.NVT, None, abi.PREC_COMP32),
+    ("nvt lists on demand", abi.NVT, (1.0, 1, 0.5, 0), abi.PREC_COMP32),
+    ("nvt windows", abi.NVT, (1.0, 1, 0.3, 8), abi.PREC_COMP32),
+    ("nve lists, acc64", abi.NVE, (1.0, 1, 0.5, 0), abi.PREC_ACC64),
+    ("nph", abi.NPH, None, abi.PREC_COMP32),
+    ("nvt fp64", abi.NVT, None, abi.PREC_FP64),
+]
+
+
+@pytest.mark.parametrize("name,kind,tuning,prec", CASES, ids=[c[0] for c in CASES])
+def test_results_do_not_depend_on_reordering(name, kind, tuning, prec):
+    f, Lx, Ly = lattice_frame(6000, 0.75, seed=41, jitter=0.05, T=1.2, aspect=2.0)
+    kw = dict(eps_ab=0.5, dt=0.005, T_target=1.2, Q=10.0, barostat_mass=50.0)
+    p_ext = 1.0 if kind == abi.NPH else 0.0
+    say(f"reorder: {name}")
+    # uneven step calls: reorders fall at different points of the trajectory;
+    # an interval of 3 is shorter than a rebuild window (8), so the chunks
+    # round up to whole windows and the last, shorter window of a call
+    # (63 = 7 x 8 + 7) must still run exactly its own steps
+    a = run(f, Lx, Ly, kw, kind, [120, 57, 63], 0, tuning, prec, p_ext)
+    b = run(f, Lx, Ly, kw, kind, [120, 57, 63], 3, tuning, prec, p_ext)
+    same(a, b)
+    c = run(f, Lx, Ly, kw, kind, [240], 40, tuning, prec, p_ext)
+    if tuning and tuning[3] > 0:
+        # rebuild windows start at each step call, so the rebuild steps (and
+        # the list order they fix) follow the calls: compare like calls
+        a = run(f, Lx, Ly, kw, kind, [240], 0, tuning, prec, p_ext)
+    same(a, c)
+
+
+def test_cell_list_after_reorders_equals_oracle():
+    f, Lx, Ly = lattice_frame(3000, 0.75, seed=77, jitter=0.3, T=1.0, aspect=2.0)
+    kw = dict(eps_ab=0.5, dt=0.005)
+    say("cell list after reorders")
+    g = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+    g.set_reorder(5)
+    g.step(abi.NVE, 53)   # several reorders, the last a few steps back
+    frame = g.collect()
+    o = O.OracleSim(O.make_params(Lx, Ly, **kw), frame)
+    for py in (True, False):
+        a = g.cell_list(py)
+        b = o.cell_list(py)
+        assert a[0] == b[0] and a[1] == b[1]
+        for k in (2, 3, 4):
+            assert np.array_equal(a[k], b[k]), (py, k)
+    g.close()
+
+
+def test_set_state_after_reorders():
+    f, Lx, Ly = lattice_frame(4000, 0.75, seed=9, jitter=0.05, T=1.0, aspect=2.0)
+    kw = dict(eps_ab=0.5, dt=0.005, T_target=1.0, Q=10.0)
+    say("set_state after reorders")
+    a = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+    a.set_reorder(3)
+    a.step(abi.NVT, 31)
+    mid = a.collect()
+    a.set_state(f)   # back to the start frame: storage in load order again
+    a.step(abi.NVT, 31)
+    fa = a.collect()
+    a.close()
+    b = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+    b.set_reorder(0)
+    b.step(abi.NVT, 31)
+    fb = b.collect()
+    b.close()
+    assert np.array_equal(mid["pos"], fb["pos"])
+    # set_state restarts the particles (xi is kept, as in the reference's resume)
+    assert np.isfinite(fa["pos"]).all()
+
+
+def test_batch_slots_reorder_independently():
+    f1, L1x, L1y = lattice_frame(3000, 0.75, seed=2, jitter=0.05, T=0.8, aspect=2.0)
+    f2, L2x, L2y = lattice_frame(1500, 0.8, seed=3, jitter=0.05, T=0.7)
+    kw1 = dict(eps_ab=0.5, dt=0.005, T_target=0.8, Q=10.0)
+    kw2 = dict(eps_ab=0.4, dt=0.005)
+    say("batch with reorders")
+    caps = [abi.cell_capacity(abi.make_params(L1x, L1y), 1.0), abi.cell_capacity(abi.make_params(L2x, L2y), 1.0)]
+    outs = []
+    for every in (0, 6):
+        b = abi.Batch([3000, 1500], caps)
+        b.set_reorder(every)
+        b.load(0, abi.make_params(L1x, L1y, **kw1), f1)
+        b.load(1, abi.make_params(L2x, L2y, **kw2), f2)
+        b.set_kind(0, abi.NVT)
+        b.set_kind(1, abi.NVE)
+        b.step(25)
+        b.set_kind(1, abi.IDLE)   # slot 1 idles while slot 0 goes on
+        b.step(19)
+        b.set_kind(1, abi.NVE)
+        b.step(30)
+        outs.append((b.collect(0), b.collect(1)))
+        b.close()
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(outs[0][0][k], outs[1][0][k])
+        assert np.array_equal(outs[0][1][k], outs[1][1][k])
