@@ -45,7 +45,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=1048576)
-    ap.add_argument("--precision", default="comp32", choices=["comp32", "acc64", "fp64"])
+    ap.add_argument("--precision", default=None, choices=["comp32", "acc64", "fp64"],
+                    help="pair-force form (default: comp32; the nve4k workload: acc64, FP32 pairs with FP64 "
+                         "accumulation as that configuration names)")
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64-accumulating comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -562,20 +564,49 @@ def run_long(args):
 
 def run_nve4k(args):
     """NVE, N=4096 square lattice (density 0.8, T=0.7, eps_AB 0.5, dt 0.005,
-    seed 1001), 1000 steps, with the FP64 oracle timed on the host beside it."""
+    seed 1001), 1000 steps, FP32 pairs with FP64 accumulation; the tuning
+    from the autotuner (or --tuning), the total energy's drift over the run
+    reported beside the FP64 oracle's over the same steps, and the oracle
+    timed on the host cores."""
     from md_simulation_cuda_amd import abi
 
     n = 4096
+    precision = args.precision or "acc64"
+    prec = {"comp32": abi.PREC_COMP32, "acc64": abi.PREC_ACC64, "fp64": abi.PREC_FP64}[precision]
     f, Lx, Ly = abi.gen_lattice(n, n // 2, 0.8, 0.05, 0.7, 1001)
-    sim = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, eps_ab=0.5, dt=0.005), f)
-    sim.step(abi.NVE, args.warmup)
-    sim.sync()
+    params = abi.make_params(Lx, Ly, precision=prec, eps_ab=0.5, dt=0.005)
+    tune = None
+    if args.tuning or args.tune_steps > 0:
+        probe = abi.Simulation(params, f)
+        if args.tuning:
+            parts = args.tuning.split(",") + ["0"]
+            t = (float(parts[0]), int(parts[1]), float(parts[2]), int(parts[3]))
+        else:
+            b = probe.autotune(abi.NVE, args.tune_steps).best
+            t = (b.cell_scale, b.lanes_per_particle, b.skin, b.window)
+        probe.close()
+        tune = {"cell_scale": t[0], "lanes_per_particle": t[1], "skin": t[2], "window": t[3]}
     steps = args.steps if args.steps != 2000 else 1000
-    t0 = time.perf_counter()
-    sim.step(abi.NVE, steps)
-    sim.sync()
-    wall = time.perf_counter() - t0
-    sim.close()
+
+    def run(timed):
+        sim = abi.Simulation(params, f)
+        if tune:
+            sim.set_tuning(tune["cell_scale"], tune["lanes_per_particle"], tune["skin"], tune["window"])
+        ob = sim.observables()
+        e0 = ob.K + ob.U
+        if timed:   # warm-up run: the same steps, then the timed run starts again from the frame
+            sim.step(abi.NVE, args.warmup)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(abi.NVE, steps)
+        sim.sync()
+        wall = time.perf_counter() - t0
+        ob = sim.observables()
+        sim.close()
+        return wall, e0, ob.K + ob.U
+
+    wall = min(run(True)[0] for _ in range(3))
+    _, e0, e1 = run(False)   # the energy drift over exactly `steps` steps from the start frame
     out = {
         "metric": "atom-steps/s NVE N=4096",
         "value": n * steps / wall,
@@ -587,18 +618,24 @@ def run_nve4k(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair + fp32 two-sum accumulation, fp64 state",
+        "dtype": {"comp32": "fp32 pair + fp32 two-sum accumulation, fp64 state",
+                  "acc64": "fp32 pair + fp64 accumulation, fp64 state", "fp64": "fp64"}[precision],
         "data": "synthetic (seeded jittered lattice, MB velocities)",
-        "config": {"workload": "NVE N=4096 square box rho=0.8, eps_AB 0.5, dt 0.005, T0 0.7"},
+        "config": {"workload": "NVE N=4096 square box rho=0.8, eps_AB 0.5, dt 0.005, T0 0.7",
+                   "precision": precision},
+        "tuning": tune,
+        "accuracy": {"energy_drift_per_atom": (e1 - e0) / n},
     }
     if not args.no_cpu:
         from oracle import oracle as O
 
         o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, eps_ab=0.5, dt=0.005), f)
         o.compute_forces()
+        eo0 = o.K() + o.U()
         t0 = time.perf_counter()
         o.step_nve(steps)
         el = time.perf_counter() - t0
+        out["accuracy"]["oracle_energy_drift_per_atom"] = (o.K() + o.U() - eo0) / n
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = {"value": n * steps / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
                                "sample": f"FP64 oracle (cell list, OpenMP {threads} threads), all {steps} NVE steps"}
@@ -633,6 +670,7 @@ def main():
         return
     from md_simulation_cuda_amd import abi
 
+    args.precision = args.precision or "comp32"
     prec = {"comp32": abi.PREC_COMP32, "acc64": abi.PREC_ACC64, "fp64": abi.PREC_FP64}[args.precision]
     frame, Lx, Ly, kw = make_workload(args.n)
     n = len(frame)

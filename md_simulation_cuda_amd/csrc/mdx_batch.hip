@@ -130,6 +130,10 @@ void Batch::invalidate_graphs() {
                     (void)hipGraphExecDestroy(g);
                     g = nullptr;
                 }
+    if (graph_window) {
+        (void)hipGraphExecDestroy(graph_window);
+        graph_window = nullptr;
+    }
 }
 
 int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int precision, int device_ordinal) {
@@ -499,8 +503,29 @@ int Batch::run_windows(int count_) {
     const int S = static_cast<int>(slots.size());
     for (int left = count_; left > 0;) {
         const int w = std::min(window, left);
-        MDX_TRY(replay(false, STEP_HEAD, 1));   // step parity 0
-        for (int j = 1; j < w; ++j) MDX_TRY(replay(false, STEP_LEAN, 1, j & 1));
+        if (w == window && use_graph && !timing) {
+            // a whole window as one graph: one launch per window instead of
+            // one per step, which is what small systems wait for
+            if (!graph_window) {
+                hipGraph_t g = nullptr;
+                MDX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                int rc = launch_step_kernels(false, STEP_HEAD, 0);
+                for (int j = 1; j < w && rc == MDX_OK; ++j) rc = launch_step_kernels(false, STEP_LEAN, j & 1);
+                hipError_t e = hipStreamEndCapture(stream, &g);
+                if (rc != MDX_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                MDX_HIP(e);
+                e = hipGraphInstantiate(&graph_window, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                MDX_HIP(e);
+            }
+            MDX_HIP(hipGraphLaunch(graph_window, stream));
+        } else {
+            MDX_TRY(replay(false, STEP_HEAD, 1));   // step parity 0
+            for (int j = 1; j < w; ++j) MDX_TRY(replay(false, STEP_LEAN, 1, j & 1));
+        }
         left -= w;
     }
     k_finalize<<<S, kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);

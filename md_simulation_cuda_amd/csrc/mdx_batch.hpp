@@ -76,6 +76,7 @@ struct Batch {
     BatchPtrs ptrs{};
     // captured step graphs, by [obs]
     hipGraphExec_t graph[2][4][2] = {};   // [obs][StepMode][step parity (lean steps)]
+    hipGraphExec_t graph_window = nullptr;  // one whole rebuild window: the head step and window - 1 lean steps
     bool use_graph = true;
     bool graph_uni = true;   // pair-kernel variant the captured graphs hold
     int lpp = 1;             // lanes per particle of the FP32 pair kernel (1, 2, 4)

@@ -151,6 +151,33 @@ def test_windows_keep_the_exact_pair_set(T, skin, window):
     assert mx < 1e-4 and rm < 5e-6
 
 
+@pytest.mark.parametrize("kind", [abi.NVT, abi.NVE])
+def test_window_graphs_equal_eager_steps(kind):
+    """A whole rebuild window replays as one graph; calls of uneven length
+    mix such replays with step-by-step shorter windows, with host reads and
+    synchronisations between them; every result equals eager launches."""
+    f, Lx, Ly = lattice_frame(12000, 0.75, seed=43, jitter=0.05, T=0.8, aspect=2.0)
+    kw = dict(eps_ab=0.5, dt=0.005, T_target=0.8, Q=10.0)
+    say(f"window graphs vs eager: kind={kind}")
+    res = []
+    for graph in (True, False):
+        s = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+        s.set_graph(graph)
+        s.set_tuning(1.0, 1, 0.5, 8)
+        obs = []
+        for count in (20, 13, 8, 8, 3, 16):
+            s.step(kind, count)
+            s.sync()
+            ob = s.observables()
+            obs.append((ob.K, ob.U, ob.xi))
+        res.append((s.collect(), obs))
+        s.close()
+    (fa, oa), (fb, ob_) = res
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(fa[k], fb[k]), k
+    assert oa == ob_
+
+
 def test_windows_nve_drift_against_oracle():
     n = 4096
     f, Lx, Ly = abi.gen_lattice(n, n // 2, 0.8, 0.05, 0.7, 1001)
