@@ -134,6 +134,12 @@ void Batch::invalidate_graphs() {
         (void)hipGraphExecDestroy(graph_window);
         graph_window = nullptr;
     }
+    for (auto &row : graph_multi)
+        for (auto &g : row)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
 }
 
 int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int precision, int device_ordinal) {
@@ -415,26 +421,40 @@ bool Batch::uniform_sigma() const {
 
 // count steps of one mode, replayed from a captured hipGraph unless kernel
 // timing is on.
+// Captures `steps` steps of one mode and parity into ge (when it is empty).
+int Batch::capture_steps(hipGraphExec_t &ge, bool obs, int mode, int parity, int steps) {
+    if (ge) return MDX_OK;
+    hipGraph_t g = nullptr;
+    MDX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int rc = MDX_OK;
+    for (int k = 0; k < steps && rc == MDX_OK; ++k) rc = launch_step_kernels(obs, mode, parity);
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    if (rc != MDX_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    MDX_HIP(e);
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    MDX_HIP(e);
+    return MDX_OK;
+}
+
 int Batch::replay(bool obs, int mode, int count_, int parity) {
     if (count_ <= 0) return MDX_OK;
     if (mode != STEP_LEAN) parity = 0;
     if (use_graph && !timing) {
-        hipGraphExec_t &ge = graph[obs ? 1 : 0][mode][parity];
-        if (!ge) {
-            hipGraph_t g = nullptr;
-            MDX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            const int rc = launch_step_kernels(obs, mode, parity);
-            hipError_t e = hipStreamEndCapture(stream, &g);
-            if (rc != MDX_OK) {
-                if (g) (void)hipGraphDestroy(g);
-                return rc;
-            }
-            MDX_HIP(e);
-            e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            MDX_HIP(e);
+        int k = 0;
+        if (count_ >= kGraphSteps && mode != STEP_LEAN) {   // fewer launches: kGraphSteps steps per graph
+            hipGraphExec_t &gm = graph_multi[obs ? 1 : 0][mode];
+            MDX_TRY(capture_steps(gm, obs, mode, 0, kGraphSteps));
+            for (; k + kGraphSteps <= count_; k += kGraphSteps) MDX_HIP(hipGraphLaunch(gm, stream));
         }
-        for (int k = 0; k < count_; ++k) MDX_HIP(hipGraphLaunch(ge, stream));
+        if (k < count_) {
+            hipGraphExec_t &ge = graph[obs ? 1 : 0][mode][parity];
+            MDX_TRY(capture_steps(ge, obs, mode, parity, 1));
+            for (; k < count_; ++k) MDX_HIP(hipGraphLaunch(ge, stream));
+        }
     } else {
         for (int k = 0; k < count_; ++k) MDX_TRY(launch_step_kernels(obs, mode, parity));
     }
@@ -842,15 +862,24 @@ int Batch::step(int nsteps) {
     // a long run goes in chunks of reorder_every steps (with rebuild windows,
     // whole windows, so the windows fall as in one run); before each, the
     // slots that have run that many steps since their storage was put in cell
-    // order are reordered (no effect on results)
+    // order are reordered (no effect on results).  A slot loaded or set since
+    // is due at once: it takes one step (or window) for its first cell list
     for (int left = nsteps; left > 0;) {
         MDX_TRY(reorder_due());
-        int chunk = reorder_every > 0 ? std::min(left, reorder_every) : left;
+        int chunk = left;
+        if (reorder_every > 0) {
+            chunk = std::min(left, reorder_every);
+            for (const Slot &s : slots)
+                if (s.loaded && s.kind != SK_NONE && !s.cells_valid && s.since_reorder >= reorder_every) chunk = 1;
+        }
         if (chunk < left && windows_apply()) chunk = std::min(left, std::max(window, chunk - chunk % window));
         MDX_TRY(run_passes(chunk, obs));
         left -= chunk;
         for (Slot &s : slots)
-            if (s.loaded && s.kind != SK_NONE) s.since_reorder += chunk;
+            if (s.loaded && s.kind != SK_NONE) {
+                s.since_reorder += chunk;
+                s.cells_valid = true;   // every stepping kind leaves a complete cell list
+            }
     }
     if (timing) step_n += nsteps;
     for (size_t r = 0; r < slots.size(); ++r) {
@@ -1030,7 +1059,7 @@ int Batch::cell_list(int r, int periodic_y, int32_t *ncx, int32_t *ncy, uint32_t
 
 int Batch::reset_ids(int r) {
     const SysParams &sp = h_sp[r];
-    slots[r].since_reorder = 0;
+    slots[r].since_reorder = kReorderDue;   // frames come in any order: reorder at the first chance
     if (sp.n_cap <= 0) return MDX_OK;
     k_ident_ids<<<ceil_div(sp.n_cap, kBlock), kBlock, 0, stream>>>(pid, sid, static_cast<uint32_t>(sp.off),
                                                                    static_cast<uint32_t>(sp.off + sp.n_cap));
@@ -1435,7 +1464,9 @@ int Batch::fire(int r, const mdx_fire_params &fp, double ftol, int max_steps, md
         sp.periodic_y = 1;
         sp.wrap_keys = 0;
         MDX_TRY(upload_all_sp());
+        if (reorder_every > 0 && s.cells_valid && s.since_reorder >= reorder_every) MDX_TRY(reorder({r}));
         MDX_TRY(run_passes(end - j, false));
+        s.since_reorder += end - j;
         j = end;
         MDX_TRY(read_scalars(r, h));
         const bool sample = (j - 1) >= interval && (j - 1) % interval == 0 && h.f_drifted >= j - 1;

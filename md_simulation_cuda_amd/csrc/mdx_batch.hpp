@@ -48,6 +48,11 @@ struct Slot {
     double p_ext = 0.0;
 };
 
+// steps per graph launch when a call runs many steps of one mode
+constexpr int kGraphSteps = 8;
+// since_reorder of a slot whose storage is in frame order: due at once
+constexpr int kReorderDue = 1 << 30;
+
 struct Batch {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -77,6 +82,7 @@ struct Batch {
     // captured step graphs, by [obs]
     hipGraphExec_t graph[2][4][2] = {};   // [obs][StepMode][step parity (lean steps)]
     hipGraphExec_t graph_window = nullptr;  // one whole rebuild window: the head step and window - 1 lean steps
+    hipGraphExec_t graph_multi[2][4] = {};  // [obs][StepMode]: kGraphSteps steps of one mode (not lean)
     bool use_graph = true;
     bool graph_uni = true;   // pair-kernel variant the captured graphs hold
     int lpp = 1;             // lanes per particle of the FP32 pair kernel (1, 2, 4)
@@ -141,6 +147,7 @@ struct Batch {
     int launch_step_kernels(bool obs, int mode, int parity);
     int run_passes(int count, bool obs);
     int replay(bool obs, int mode, int count, int parity = 0);
+    int capture_steps(hipGraphExec_t &ge, bool obs, int mode, int parity, int steps);
     int run_windows(int count);
     bool windows_apply();
     int pre_step();

@@ -94,8 +94,11 @@ def test_graph_replay_equals_eager():
         b.set_graph(graph)
         for r in range(len(syss)):
             b.set_kind(r, abi.NVT)
-        for _ in range(5):   # replays with eager host work and syncs between them
-            b.step(7)
+        # replays with eager host work and syncs between them; calls of 8
+        # steps and more run graphs of several steps, shorter ones and the
+        # remainders one step per launch
+        for count in (7, 19, 8, 3, 16):
+            b.step(count)
             b.sync()
             _ = b.observables(1, abi.OBS_K)
         res.append([b.collect(r) for r in range(len(syss))])

@@ -143,3 +143,25 @@ def test_batch_slots_reorder_independently():
     for k in ("pos", "vel", "acc"):
         assert np.array_equal(outs[0][0][k], outs[1][0][k])
         assert np.array_equal(outs[0][1][k], outs[1][1][k])
+
+
+def test_fire_does_not_depend_on_reordering():
+    # a random start: storage in frame order is far from cell order, and FIRE
+    # reorders at its chunk boundaries (every 256 iterations at most)
+    f, Lx, Ly = abi.gen_uniform(3000, 1500, 0.8, 0.0, 77)
+    kw = dict(eps_ab=0.5, dt=0.005)
+    fp = abi.FireParams(0.005, 0.01, 0.1, 1.1, 0.5, 5, 100, 0.1, 1e-3)
+    say("fire with reorders")
+    out = []
+    for every in (0, 50):
+        s = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+        s.set_reorder(every)
+        res, es, et = s.fire(fp, 1e-6, 900)
+        out.append((res, list(es), list(et), s.collect()))
+        s.close()
+    (ra, ea, ta, fa), (rb, eb, tb, fb) = out
+    assert (ra.steps, ra.converged, ra.max_force, ra.potential_energy) == \
+        (rb.steps, rb.converged, rb.max_force, rb.potential_energy)
+    assert ea == eb and ta == tb
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(fa[k], fb[k]), k
