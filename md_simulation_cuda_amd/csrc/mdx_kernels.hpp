@@ -727,6 +727,14 @@ __global__ void __launch_bounds__(kLeanThreads) k_lean(BatchPtrs P, int32_t pari
             }
         }
     }
+    // the grid and the build images too (after the stores below, the
+    // compiler could not move these loads ahead of them)
+    const Grid g = sc.g;
+    const bool check = moves && sc.list_valid;
+    float2 i0s[kLeanPer];
+#pragma unroll
+    for (int j = 0; j < kLeanPer; ++j)
+        if (act[j] && check) i0s[j] = P.img0[k0 + static_cast<uint32_t>(base + j * kLeanThreads)];
     double xi1 = 0.0;
     if (kind == SK_NVT && sp.nblk > 0) {   // uniform per workgroup
         double out[3];
@@ -746,8 +754,6 @@ __global__ void __launch_bounds__(kLeanThreads) k_lean(BatchPtrs P, int32_t pari
         }
     }
     if (first) sc.fallback[parity ^ 1] = 0;
-    const Grid g = sc.g;
-    const bool check = moves && sc.list_valid;
     const float thr = static_cast<float>(sp.skin_half2) * (1.0f - 1.0e-5f);
     const float reach = 0.5f * static_cast<float>(2.0 * fmin(g.w, g.h) - sp.rc_max);
     const float thr_v = reach * reach * (1.0f - 1.0e-5f);
@@ -798,7 +804,7 @@ __global__ void __launch_bounds__(kLeanThreads) k_lean(BatchPtrs P, int32_t pari
             // displacement since the build: the two images share the build
             // cell's centre (FP32; the thresholds are lowered by 1e-5 relative
             // so that a rounding near them errs towards the fallback)
-            const float2 i0 = P.img0[k];
+            const float2 i0 = i0s[j];
             const float ex = im.x - i0.x, ey = im.y - i0.y;
             const float d2 = __builtin_fmaf(ex, ex, ey * ey);
             fb = fb || d2 > thr;
@@ -1469,12 +1475,17 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
     double r0 = 0.0, r1 = 0.0, r2v = 0.0;
     ForceSum<kFp64 ? ACC_FP64 : ACC> F;
     ObsSum ob;
+    // the particle's storage index and class, loaded with its image so the
+    // epilogue's velocity load does not wait on them
+    uint32_t id = 0;
+    int ci = 0;
     if (li < sp.n) {
         const uint32_t s = P.cs[sp.coff] + static_cast<uint32_t>(li);
+        id = P.sorted[s];
         const uint32_t lc = P.scell[s];
         const int cx = static_cast<int>(lc % static_cast<uint32_t>(g.ncx));
         const int cy = static_cast<int>(lc / static_cast<uint32_t>(g.ncx));
-        const int ci = P.icls[s];
+        ci = P.icls[s];
         const IT pi = static_cast<const IT *>(P.img)[s];
         PairF32 pc{};
         if constexpr (!kFp64) {
@@ -1566,9 +1577,10 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
         if constexpr (LPP > 1) combine_lanes<LPP, kFp64 ? ACC_FP64 : ACC>(F, ob);
     }
     if (li < sp.n && lead) {
-        const uint32_t s = P.cs[sp.coff] + static_cast<uint32_t>(li);
-        const int ci = P.icls[s];
-        const uint32_t id = P.sorted[s];
+        const bool kicks = pmode == PM_NVE || pmode == PM_NPH || pmode == PM_PISTON || pmode == PM_NVT ||
+                           pmode == PM_FIRE;
+        double2 q = make_double2(0.0, 0.0);
+        if (kicks) q = P.v[id];   // in flight while the acceleration is formed and stored
         double axd, ayd;
         if constexpr (kFp64 || ACC == ACC_FP64) {
             axd = F.X() / pt.m[ci];
@@ -1608,14 +1620,12 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
             ayd = static_cast<double>(out.y);
         }
         if (pmode == PM_NVE || pmode == PM_NPH || pmode == PM_PISTON) {
-            double2 q = P.v[id];
             q.x += 0.5 * sp.dt * axd;
             q.y += 0.5 * sp.dt * ayd;
             P.v[id] = q;
             if (pmode == PM_NPH) r0 = 0.5 * pt.m[ci] * (q.x * q.x + q.y * q.y);
         } else if (pmode == PM_NVT) {
             const double xi_mid = sc.xi_mid[parity];
-            double2 q = P.v[id];
             double vx = q.x, vy = q.y;
             vx += 0.5 * sp.dt * (axd - xi_mid * vx);
             vy += 0.5 * sp.dt * (ayd - xi_mid * vy);
@@ -1627,7 +1637,6 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
             r2v = 0.5 * pt.m[ci] * (axd * axd + ayd * ayd);
         } else if (pmode == PM_FIRE) {
             const double fdt = sc.f_dt;
-            double2 q = P.v[id];
             q.x += fdt * axd;
             q.y += fdt * ayd;
             P.v[id] = q;
