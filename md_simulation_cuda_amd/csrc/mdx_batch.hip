@@ -1266,6 +1266,11 @@ int Batch::autotune(int steps, mdx_tune_report *rep, int kind_override) {
         for (double sk : {0.3, 0.5, 0.65, 0.8})
             for (int w : {0, 8, 12, 16, 20}) cands.push_back(mdx_tuning{1.0, sk, 1, w});
         for (double sk : {0.5, 0.8}) cands.push_back(mdx_tuning{1.0, sk, 2, 0});
+        // longer windows: a whole window is one graph launch
+        cands.push_back(mdx_tuning{1.0, 0.8, 1, 24});
+        cands.push_back(mdx_tuning{1.0, 1.0, 1, 20});
+        cands.push_back(mdx_tuning{1.0, 1.0, 1, 24});
+        cands.push_back(mdx_tuning{1.0, 1.0, 1, 32});
     }
     double best = 1e300;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1278,9 +1283,14 @@ int Batch::autotune(int steps, mdx_tune_report *rep, int kind_override) {
         rc = c.set_tuning(t);
         if (rc == MDX_ERR_CAPACITY) continue;   // too many cells for a slot: not a candidate
         if (rc != MDX_OK) break;
-        if ((rc = c.step(3)) != MDX_OK) break;   // warm-up: graph capture, caches
+        // warm-up: every graph the timed steps launch is captured here (the
+        // per-step graphs, a whole window's, eight steps'), and caches
+        if ((rc = c.step(3)) != MDX_OK) break;
+        if ((rc = c.step(std::max(kGraphSteps, t.window))) != MDX_OK) break;
+        // whole windows only, as in a long run
+        const int ts = t.window >= 2 ? ((steps + t.window - 1) / t.window) * t.window : steps;
         (void)hipEventRecord(e0, c.stream);
-        if ((rc = c.step(steps)) != MDX_OK) break;
+        if ((rc = c.step(ts)) != MDX_OK) break;
         (void)hipEventRecord(e1, c.stream);
         if (hipEventSynchronize(e1) != hipSuccess) {
             rc = set_error(MDX_ERR_HIP, "autotune: event wait failed");
@@ -1288,7 +1298,7 @@ int Batch::autotune(int steps, mdx_tune_report *rep, int kind_override) {
         }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        const double per = static_cast<double>(ms) / steps;
+        const double per = static_cast<double>(ms) / ts;
         rep->tried[rep->n_tried] = t;
         rep->ms_per_step[rep->n_tried] = per;
         ++rep->n_tried;

@@ -127,7 +127,7 @@ def test_autotune_reports_candidates_and_keeps_state():
     before = s.collect()
     rep = s.autotune(abi.NVT, 10)
     say(f"  tried {rep.table()} best {rep.best.cell_scale} {rep.best.lanes_per_particle}")
-    assert rep.n_tried == 28
+    assert rep.n_tried == 32
     assert all(row[-1] > 0 for row in rep.table())
     assert rep.best_ms_per_step == min(row[-1] for row in rep.table())
     t = s.get_tuning()
