@@ -133,3 +133,27 @@ def test_synthetic_generators():
     assert np.all((u["pos"] >= 0) & (u["pos"] < L2)) and not u["vel"].any()
     f2, _, _ = abi.gen_lattice(4096, 2048, 0.8, 0.05, 0.7, seed=1001)
     assert f.tobytes() == f2.tobytes()
+
+
+def test_particle_csv_and_xyz_text(tmp_path):
+    # the reference's CSV snapshot (src/md_particle.cpp:34-86) and its NPH
+    # batch driver's XYZ frames (run/series_NPH_batch_xyz_saving.cpp:106-141)
+    a, b, x = tmp_path / "a.csv", tmp_path / "b.csv", tmp_path / "f.xyz"
+    r = run("particle_text_io", a, b, x)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert a.read_text().splitlines() == [
+        "box_w,10.000000000000,box_h,8.000000000000,sigma_aa,1.000000000000,sigma_bb,1.200000000000,"
+        "draw_box,1,n_particles,3",
+        "x,1.500000000000,y,2.250000000000,type,0",
+        "x,-0.125000000000,y,7.000000000000,type,1",
+        "x,3.000000000000,y,4.000000000000,type,5",
+    ]
+    # no box given: the extents (3.125 in x, 4.75 in y), draw_box 0
+    assert b.read_text().splitlines()[0] == (
+        "box_w,3.125000000000,box_h,4.750000000000,sigma_aa,1.000000000000,sigma_bb,1.000000000000,"
+        "draw_box,0,n_particles,3")
+    lines = x.read_text().splitlines()
+    assert lines[0] == "3" and lines[1].startswith("phase=NVT global_step=0")
+    assert lines[2:5] == ["A 1.5000000000 2.2500000000 0.0", "B -0.1250000000 7.0000000000 0.0",
+                          "X 3.0000000000 4.0000000000 0.0"]
+    assert lines[5:7] == ["3", "second"] and len(lines) == 10
