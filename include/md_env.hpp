@@ -38,6 +38,9 @@ public:
     double cal_total_K();
     double cal_total_U();
     double cal_partial_U_lambda(double epsilon_lambda);
+    // counts of type 0, then of other types, in n_bins_per_rank slabs along x
+    // (src/md_env.cu:3920-3988 with one rank: 2 * n_bins_per_rank values)
+    std::vector<int> get_N_profile(int n_bins_per_rank);
     double get_Lx() const { return cfg_manager.config.box_w_global; }
     double get_Ly() const { return cfg_manager.config.box_h_global; }
 

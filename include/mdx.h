@@ -187,6 +187,14 @@ int mdx_sync(mdx_sim *sim);
 /* Canonical cell list of the current positions (for parity checks):
  * keys[n], cell_start[ncell+1], sorted[n] (original indices, ascending within
  * a cell).  Any array may be NULL; ncx/ncy always written. */
+/* Particle counts in n_bins slabs of the box along x, [0, Lx] with the current
+ * box: counts[0, n_bins) type 0, counts[n_bins, 2 n_bins) every other type;
+ * bin = int(x / Lx * n_bins) clamped to [0, n_bins).  Replaces get_N_profile
+ * (src/md_env.cu:3920-3988, kernel src/md_cuda_common.cu:827-877) on one
+ * domain: the reference with one rank.  n_bins in [1, 8192]; counts holds
+ * 2 n_bins values. */
+int mdx_N_profile(mdx_sim *sim, int32_t n_bins, int32_t *counts);
+
 int mdx_cell_list(mdx_sim *sim, int32_t periodic_y, int32_t *ncx, int32_t *ncy,
                   uint32_t *keys, uint32_t *cell_start, uint32_t *sorted,
                   int64_t cap_n, int64_t cap_cells);

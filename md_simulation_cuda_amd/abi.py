@@ -202,6 +202,7 @@ _SIGNATURES = {
     "mdx_batch_timing_enable": (C.c_int, [_P, C.c_int32]),
     "mdx_batch_timing_read": (C.c_int, [_P, C.POINTER(Timing)]),
     "mdx_sync": (C.c_int, [_P]),
+    "mdx_N_profile": (C.c_int, [_P, C.c_int32, _P]),
     "mdx_cell_list": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P, _P, _P,
                                 C.c_int64, C.c_int64]),
     "mdx_timing_enable": (C.c_int, [_P, C.c_int32]),
@@ -359,6 +360,12 @@ class Simulation:
 
     def set_precision(self, prec: int) -> None:
         check(self._lib.mdx_set_precision(self._h, prec))
+
+    def N_profile(self, n_bins: int):
+        """Counts per x slab: type 0 in [0, n_bins), other types after."""
+        out = np.zeros(2 * n_bins, np.int32)
+        check(self._lib.mdx_N_profile(self._h, n_bins, _ptr(out)))
+        return out
 
     def cell_list(self, periodic_y: bool = True):
         ncx, ncy = C.c_int32(), C.c_int32()

@@ -242,6 +242,11 @@ int mdx_cell_list(mdx_sim *sim, int32_t periodic_y, int32_t *ncx, int32_t *ncy, 
     return sim->b.cell_list(0, periodic_y ? 1 : 0, ncx, ncy, keys, cell_start, sorted, cap_n, cap_cells);
 }
 
+int mdx_N_profile(mdx_sim *sim, int32_t n_bins, int32_t *counts) {
+    CHECK_HANDLE(sim);
+    return sim->b.n_profile(0, n_bins, counts);
+}
+
 int mdx_timing_enable(mdx_sim *sim, int32_t enable) {
     CHECK_HANDLE(sim);
     return timing_enable(sim->b, enable);

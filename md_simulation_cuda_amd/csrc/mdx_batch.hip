@@ -1104,6 +1104,30 @@ int Batch::reorder(const std::vector<int> &which) {
     return MDX_OK;
 }
 
+// Particle counts per x slab of slot r (k_n_profile), copied to out[2 nb].
+int Batch::n_profile(int r, int nb, int32_t *out) {
+    if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
+        return set_error(MDX_ERR_STATE, "slot not loaded");
+    if (nb < 1 || nb > kProfileMaxBins) return set_error(MDX_ERR_INVALID, "n_bins must be in [1, 8192]");
+    if (!out) return set_error(MDX_ERR_INVALID, "null output");
+    MDX_HIP(hipSetDevice(device));
+    int32_t *d = nullptr;
+    MDX_HIP(hipMalloc(&d, static_cast<size_t>(2 * nb) * sizeof(int32_t)));
+    hipError_t e = hipMemsetAsync(d, 0, static_cast<size_t>(2 * nb) * sizeof(int32_t), stream);
+    const int64_t n = slots[r].n;
+    if (e == hipSuccess && n > 0) {
+        k_n_profile<<<static_cast<int>((n + kBlock - 1) / kBlock), kBlock, static_cast<size_t>(2 * nb) * sizeof(int32_t),
+                      stream>>>(ptrs, r, nb, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, d, static_cast<size_t>(2 * nb) * sizeof(int32_t), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(d);
+    MDX_HIP(e);
+    return MDX_OK;
+}
+
 int Batch::reorder_due() {
     std::vector<int> due;
     for (size_t r = 0; r < slots.size(); ++r) {

@@ -50,6 +50,8 @@ struct Slot {
 
 // steps per graph launch when a call runs many steps of one mode
 constexpr int kGraphSteps = 8;
+// most bins of an N profile (two int counters each in LDS)
+constexpr int kProfileMaxBins = 8192;
 // since_reorder of a slot whose storage is in frame order: due at once
 constexpr int kReorderDue = 1 << 30;
 
@@ -148,6 +150,7 @@ struct Batch {
     int run_passes(int count, bool obs);
     int replay(bool obs, int mode, int count, int parity = 0);
     int capture_steps(hipGraphExec_t &ge, bool obs, int mode, int parity, int steps);
+    int n_profile(int r, int nb, int32_t *out);
     int run_windows(int count);
     bool windows_apply();
     int pre_step();

@@ -989,6 +989,34 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
     if (li == 0) sc.list_built = 1;
 }
 
+// Particle counts in n_bins slabs along x of slot r, type class 0 (A) in
+// counts[0, nb), every other type (B) in counts[nb, 2 nb): the reference's
+// local_density_profile_kernel (src/md_cuda_common.cu:827-877) on one domain
+// [xmin, xmax] = [0, Lx] with the current box, bin = int((x - xmin) / (xmax -
+// xmin) * nb) clamped to [0, nb), particles outside [xmin, xmax] not
+// counted.  Integer counts: the result does not depend on the order of the
+// atomics.  Dynamic LDS: 2 nb counters.
+__global__ void __launch_bounds__(kBlock) k_n_profile(BatchPtrs P, int32_t r, int32_t nb, int32_t *counts) {
+    extern __shared__ int32_t hist[];
+    const SysParams &sp = P.sp[r];
+    for (int j = threadIdx.x; j < 2 * nb; j += kBlock) hist[j] = 0;
+    __syncthreads();
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < sp.n) {
+        const int64_t k = sp.off + i;
+        const double x = P.x[k].x;
+        const double xmin = 0.0, xmax = P.sc[r].g.Lx;
+        if (x >= xmin && x <= xmax) {
+            int bin = static_cast<int>((x - xmin) / (xmax - xmin) * nb);
+            bin = bin < 0 ? 0 : (bin >= nb ? nb - 1 : bin);
+            atomicAdd(&hist[(P.cls[k] == 0 ? 0 : nb) + bin], 1);
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < 2 * nb; j += kBlock)
+        if (hist[j] != 0) atomicAdd(&counts[j], hist[j]);
+}
+
 // --------------------------------------------------------------------------
 // pair kernel
 // --------------------------------------------------------------------------

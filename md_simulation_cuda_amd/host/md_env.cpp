@@ -197,6 +197,13 @@ double MDSimulation::cal_partial_U_lambda(double epsilon_lambda) {
     return v;
 }
 
+std::vector<int> MDSimulation::get_N_profile(int n_bins_per_rank) {
+    std::vector<int> out(static_cast<size_t>(n_bins_per_rank > 0 ? 2 * n_bins_per_rank : 0));
+    if (n_bins_per_rank > 0)
+        check(mdx_N_profile(sim_, n_bins_per_rank, reinterpret_cast<int32_t *>(out.data())), "get_N_profile");
+    return out;
+}
+
 void MDSimulation::sample_collect() {
     h_particles.resize(static_cast<size_t>(cfg_manager.config.n_particles_global));
     check(mdx_collect(sim_, reinterpret_cast<mdx_particle64 *>(h_particles.data()),

@@ -211,3 +211,21 @@ class OracleSim:
         self._lib.orc_cell_list(self._h, 1 if periodic_y else 0, C.byref(ncx), C.byref(ncy),
                                 keys.ctypes.data, start.ctypes.data, srt.ctypes.data)
         return ncx.value, ncy.value, keys, start, srt
+
+
+def n_profile(x, types, Lx, n_bins):
+    """Counts per x slab, type 0 then the other types (2 n_bins values): the
+    reference's local_density_profile_kernel (src/md_cuda_common.cu:827-877,
+    driven by get_N_profile, src/md_env.cu:3920-3988) with one rank,
+    xmin = 0, xmax = Lx: bin = int((x - xmin) / (xmax - xmin) * n_bins),
+    clamped to [0, n_bins); particles outside [xmin, xmax] are not counted."""
+    x = np.asarray(x, np.float64)
+    types = np.asarray(types)
+    xmin, xmax = 0.0, float(Lx)
+    inside = (x >= xmin) & (x <= xmax)
+    b = ((x[inside] - xmin) / (xmax - xmin) * float(n_bins)).astype(np.int64)   # truncation, as int()
+    b = np.clip(b, 0, n_bins - 1)
+    t = types[inside]
+    a = np.bincount(b[t == 0], minlength=n_bins)
+    o = np.bincount(b[t != 0], minlength=n_bins)
+    return np.concatenate([a, o]).astype(np.int32)
