@@ -41,6 +41,8 @@ public:
     // counts of type 0, then of other types, in n_bins_per_rank slabs along x
     // (src/md_env.cu:3920-3988 with one rank: 2 * n_bins_per_rank values)
     std::vector<int> get_N_profile(int n_bins_per_rank);
+    // per slab (n_A - n_B) / (n_A + n_B), 0 for an empty slab (src/md_env.cu:4086-4104)
+    std::vector<double> get_density_profile(int n_bins_per_rank);
     double get_Lx() const { return cfg_manager.config.box_w_global; }
     double get_Ly() const { return cfg_manager.config.box_h_global; }
 

@@ -204,6 +204,18 @@ std::vector<int> MDSimulation::get_N_profile(int n_bins_per_rank) {
     return out;
 }
 
+std::vector<double> MDSimulation::get_density_profile(int n_bins_per_rank) {
+    const std::vector<int> n = get_N_profile(n_bins_per_rank);
+    const size_t nb = n.size() / 2;
+    std::vector<double> out(nb, 0.0);
+    for (size_t i = 0; i < nb; ++i) {
+        const double a = static_cast<double>(n[i]), b = static_cast<double>(n[i + nb]);
+        const double sum = a + b;
+        out[i] = sum == 0.0 ? 0.0 : (a - b) / sum;
+    }
+    return out;
+}
+
 void MDSimulation::sample_collect() {
     h_particles.resize(static_cast<size_t>(cfg_manager.config.n_particles_global));
     check(mdx_collect(sim_, reinterpret_cast<mdx_particle64 *>(h_particles.data()),

@@ -39,3 +39,9 @@ def test_reference_minimizer_smoke():
     m = re.search(r"converged=(\d) r=([\d.]+)", r.stdout)
     assert m.group(1) == "1"
     assert abs(float(m.group(2)) - 2 ** (1 / 6)) < 1e-4
+
+
+def test_profiles_through_the_cpp_class():
+    r = run(CPP / "profile_smoke", GOLDEN / "nph_config.json")
+    assert r.returncode == 0
+    assert "profile ok n=512" in r.stdout
