@@ -121,13 +121,22 @@ def run_gpu(frame, Lx, Ly, kw, prec, steps, warmup, tune_steps, fixed_tuning=Non
         tune = {"cell_scale": rep.best.cell_scale, "lanes_per_particle": rep.best.lanes_per_particle,
                 "skin": rep.best.skin, "window": rep.best.window,
                 "candidates_ms_per_step": [[c, l, k, w, round(m, 5)] for c, l, k, w, m in rep.table()]}
-    sim.step(abi.NVT, warmup)
+    # warm-up: at least one whole rebuild window, so every graph the timed
+    # steps launch (a whole window's, and the per-step graphs of a trailing
+    # partial window) is captured before timing
+    window = int((tune or {}).get("window", 0) or 0)
+    if window >= 2:
+        warm = -(-max(warmup, window) // window) * window + steps % window
+    else:
+        warm = max(warmup, 8)
+    sim.step(abi.NVT, warm)
     sim.sync()
     t0 = time.perf_counter()
     sim.step(abi.NVT, steps)
     sim.sync()
     wall = time.perf_counter() - t0
     obs = sim.observables()
+    run_gpu.warmup_steps = warm
     sim.timing_enable(True)
     sim.step(abi.NVT, min(steps, timed_steps))
     tm = sim.timing_read()
@@ -206,6 +215,12 @@ def accuracy(frame, Lx, Ly, kw):
                      "force_rms_rel_outside_cutoff_band": float(np.sqrt((err[~band] ** 2).mean()) / rms),
                      "pe_per_particle_err": abs(o.U - o64.U) / len(frame),
                      "pressure_rel_err": abs(o.P - o64.P) / abs(o64.P)}
+    # the accelerations the timed run's last pair pass stored (list pass of
+    # the bench tuning, compensated FP32) at the same positions
+    err = np.sqrt(((frame["acc"] - a64) ** 2).sum(1))
+    out["comp32_timed_pass_stored"] = {"force_max_rel": float(err.max() / rms),
+                                       "force_max_rel_outside_cutoff_band": float(err[~band].max() / rms),
+                                       "force_rms_rel": float(np.sqrt((err ** 2).mean()) / rms)}
     n1 = 4096
     f1, L1x, L1y = abi.gen_lattice(n1, n1 // 2, 0.8, 0.05, 0.7, 1001)
     for name, prec in (("fp64", abi.PREC_FP64), ("comp32", abi.PREC_COMP32), ("acc64", abi.PREC_ACC64)):
@@ -221,30 +236,51 @@ def accuracy(frame, Lx, Ly, kw):
     return out
 
 
-def cpu_baseline(frame, Lx, Ly, kw, budget_s):
+def oracle_rate(frame, Lx, Ly, kw, budget_s, threads, kind="nvt", max_steps=10000):
+    """FP64 oracle (cell list) steps of `kind` on `threads` host threads (0:
+    OpenMP's default) until budget_s has passed: (atom-steps/s, steps,
+    seconds, setup seconds)."""
     from oracle import oracle as O
 
     n = len(frame)
-    sim = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, **kw), frame)
+    sim = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, n_threads=threads, **kw), frame)
     t0 = time.perf_counter()
     sim.compute_forces()
     setup = time.perf_counter() - t0
     steps = 0
     t0 = time.perf_counter()
     while True:
-        sim.step_nvt(1)
+        if kind == "nvt":
+            sim.step_nvt(1)
+        else:
+            sim.step_nve(1)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 10000:
+        if el >= budget_s or steps >= max_steps:
             break
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sim.close()
+    return n * steps / el, steps, el, setup
+
+
+def host_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def cpu_baseline(frame, Lx, Ly, kw, budget_s):
+    """The oracle on all host threads (the baseline) and on one thread."""
+    n = len(frame)
+    threads = host_threads()
+    v, steps, el, setup = oracle_rate(frame, Lx, Ly, kw, budget_s, threads)
+    v1, steps1, el1, _ = oracle_rate(frame, Lx, Ly, kw, max(2.0, budget_s / 3.0), 1)
     return {
-        "value": n * steps / el,
+        "value": v,
         "unit": "atom-steps/s",
         "cores": threads,
         "kind": "port",
         "sample": f"FP64 oracle (cell list, OpenMP {threads} threads), {steps} NVT steps of the same "
                   f"N={n} workload in {el:.1f} s (+{setup:.1f} s initial forces)",
+        "single_thread": {"value": v1, "unit": "atom-steps/s", "cores": 1,
+                          "sample": f"{steps1} NVT steps of the same workload on 1 thread in {el1:.1f} s"},
     }
 
 
@@ -687,9 +723,13 @@ def main():
     # list mode: the pair kernel reads the list length (4 B) and one 4-B entry
     # per partner within cutoff + skin instead of walking cell starts
     list_len = rho * math.pi * (2.5 + skin) ** 2 if skin > 0 else 0.0
+    # bytes this design's pair kernel moves per atom (list rows included)
     bytes_per_atom = PAIR_BYTES_PER_ATOM[prec] + (4.0 + 4.0 * list_len - 1.0 if skin > 0 else 0.0)
-    pair_bytes = bytes_per_atom * n
-    achieved = pair_bytes / (pair_ms * 1e-3) / 1e9
+    # roofline of the dominant kernel: the algorithmic bytes of one atom-step
+    # (114 B compensated, 130 B FP64-accumulating) times the atoms one pair
+    # launch covers, over the launch's average duration
+    alg_bytes = STEP_BYTES_PER_ATOM[prec] * n
+    achieved = alg_bytes / (pair_ms * 1e-3) / 1e9
     flops = FLOP_PER_PAIR * in_cutoff_pairs_per_atom(rho) * n
     out = {
         "metric": "atom-steps/s @N=1M NVT",
@@ -697,7 +737,7 @@ def main():
         "unit": "atom-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": getattr(run_gpu, "warmup_steps", args.warmup),
         "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -719,7 +759,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": pair_traffic(tune) if (n == 1048576 and prec == abi.PREC_COMP32) else None,
-            "bytes_per_atom": bytes_per_atom,
+            "algorithmic_bytes_per_launch": alg_bytes,
+            "algorithmic_bytes_per_atom_step": STEP_BYTES_PER_ATOM[prec],
+            "design_bytes_per_atom": bytes_per_atom,
             "mean_list_entries": list_len,
             "avg_launch_ms": pair_ms,
             "launches": tm.force_launches,

@@ -104,6 +104,9 @@ typedef struct mdx_obs {
     double t;           /* simulated time                       */
     int32_t piston_active;
     int32_t pad;
+    double eta;         /* integral of xi dt over the NVT steps (no counterpart in the
+                           reference): K + U + Q xi^2 / 2 + (2N - 2) T0 eta is the
+                           Nose-Hoover conserved energy */
 } mdx_obs;
 
 /* FIRE parameters (include/md_energy_minimizer.hpp:11-25) plus the documented

@@ -58,7 +58,7 @@ class Obs(C.Structure):
         ("xi", C.c_double), ("p_A", C.c_double), ("area_rate", C.c_double),
         ("v_p", C.c_double), ("F_p", C.c_double), ("last_P", C.c_double),
         ("Lx", C.c_double), ("Ly", C.c_double), ("t", C.c_double),
-        ("piston_active", C.c_int32), ("pad", C.c_int32),
+        ("piston_active", C.c_int32), ("pad", C.c_int32), ("eta", C.c_double),
     ]
 
 

@@ -69,6 +69,8 @@ PairTable make_pair_table(const mdx_params &p, double &rc_max) {
         pt.fb[k] = static_cast<float>(pt.eps24[k] * q6);
         pt.ua[k] = static_cast<float>(pt.eps4[k] * q6 * q6);
         pt.ub[k] = static_cast<float>(pt.eps4[k] * q6);
+        const double rco = kCoreRatio * sig[k];
+        pt.rcore2f[k] = static_cast<float>(rco * rco);
     }
     pt.m[0] = p.mass_a;
     pt.m[1] = p.mass_b;
@@ -111,7 +113,7 @@ Batch::~Batch() {
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
                      d_sp, d_sc, d_blk2sys, d_cblk2sys, d_lb2sys, d_want, sorted_old, cs_old, movers, mover_off,
-                     n_movers, arrivals, key_list, x0, img0, nlist, ncount, pid, sid, reorder_tmp};
+                     n_movers, arrivals, key_list, x0, img0, img_lo, nlist, ncount, pid, sid, reorder_tmp};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
     for (auto *vec : {&force_ev, &step_ev})
@@ -201,6 +203,8 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     MDX_HIP(hipMalloc(&v, n16));
     MDX_HIP(hipMalloc(&a, n16));
     MDX_HIP(hipMalloc(&img, n16));
+    MDX_HIP(hipMalloc(&img_lo, n16 / 2));
+    MDX_HIP(hipMemset(img_lo, 0, n16 / 2));
     MDX_HIP(hipMalloc(&cls, n1));
     MDX_HIP(hipMalloc(&icls, n1));
     MDX_HIP(hipMalloc(&key, n4));
@@ -248,6 +252,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     ptrs.sorted = sorted;
     ptrs.scell = scell;
     ptrs.img = img;
+    ptrs.img_lo = prec == MDX_PREC_COMP32 ? img_lo : nullptr;
     ptrs.icls = icls;
     ptrs.partials = partials;
     ptrs.sorted_old = sorted_old;
@@ -952,6 +957,7 @@ int Batch::observables(int r, uint32_t mask, mdx_obs *out) {
     out->Ly = Ly;
     out->t = s.t;
     out->piston_active = s.piston_active ? 1 : 0;
+    out->eta = h.eta;
     return MDX_OK;
 }
 
@@ -1145,6 +1151,7 @@ int Batch::set_precision(int precision) {
     MDX_HIP(hipSetDevice(device));
     MDX_HIP(hipStreamSynchronize(stream));
     prec = precision;
+    ptrs.img_lo = prec == MDX_PREC_COMP32 ? img_lo : nullptr;
     invalidate_graphs();
     // accelerations and the cell image change representation: recompute
     std::vector<int> per, pis;
@@ -1443,6 +1450,7 @@ int Batch::observables_many(const std::vector<int> &which, uint32_t mask, mdx_ob
         o.Lx = d.g.Lx;
         o.Ly = d.g.Ly;
         o.t = slots[r].t;
+        o.eta = d.eta;
     }
     MDX_HIP(hipStreamSynchronize(stream));
     return MDX_OK;

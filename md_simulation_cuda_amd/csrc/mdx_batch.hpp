@@ -98,6 +98,7 @@ struct Batch {
     void *reorder_tmp = nullptr;              // 16 B per particle, allocated by the first reorder
     double2 *x0 = nullptr;
     float2 *img0 = nullptr;
+    float2 *img_lo = nullptr;   // rounding remainder of the FP32 images (compensated form)
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> force_ev, step_ev;

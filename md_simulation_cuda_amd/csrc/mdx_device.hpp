@@ -30,7 +30,18 @@ struct PairTable {
     float fa[3], fb[3], ua[3], ub[3];
     float r2_guard;        // FP32 pairs closer than this are evaluated in FP64
     float pad;
+    // compensated FP32 form: pairs with r^2 below (kCoreRatio sigma_ij)^2,
+    // the repulsive core, are evaluated in FP64 from an FP64 displacement
+    float rcore2f[3];
+    float pad2;
 };
+
+// Radius of the repulsive core, in units of sigma_ij, inside which the
+// compensated FP32 form evaluates a pair in FP64.  Inside it |dF/dr| grows as
+// r^-14, so one FP32 rounding of the displacement (~6e-8 sigma) moves a pair
+// force by up to ~1e-4 epsilon/sigma; about 1.4 of the ~15 in-cutoff partners
+// of a particle lie inside it in the liquid at density 0.75, T 0.6.
+constexpr double kCoreRatio = 1.1;
 
 // Periodic wrap of include/md_cuda_common.hpp:20-29: fmod(x, L), plus L when
 // negative.  The common cases return exactly what fmod returns (x in [0, L):

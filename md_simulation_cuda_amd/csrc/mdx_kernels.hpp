@@ -443,7 +443,11 @@ __device__ inline void finalize_drift(const BatchPtrs &P, const SysParams &sp, D
     slot_reduce<1>(0u, P.partials, P.TB, sp.boff, sp.nblk, out);
     if (threadIdx.x == 0) {
         sc.K_half = out[0];
-        if (sp.kind == SK_NVT) sc.xi_mid[0] = nh_half_update(nh_half_update(sc.xi, sc.K, sp), out[0], sp);
+        if (sp.kind == SK_NVT) {
+            const double xi1 = nh_half_update(sc.xi, sc.K, sp);
+            sc.xi_mid[0] = nh_half_update(xi1, out[0], sp);
+            sc.eta += 0.5 * sp.dt * (xi1 + sc.xi_mid[0]);
+        }
     }
 }
 
@@ -655,6 +659,9 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
         }
         const float2 im = make_float2(static_cast<float>(dx), static_cast<float>(dy));
         img[k] = im;
+        if (P.img_lo)   // the rounding remainder (dx - im is exact in FP64)
+            P.img_lo[k] = make_float2(static_cast<float>(dx - static_cast<double>(im.x)),
+                                      static_cast<float>(dy - static_cast<double>(im.y)));
         if (reb && sp.list_mode) P.img0[k] = im;   // build-time image, for the lean steps' check
     } else {
         img[k] = p;
@@ -750,6 +757,7 @@ __global__ void __launch_bounds__(kLeanThreads) k_lean(BatchPtrs P, int32_t pari
             sc.xi = xi;
             sc.K_half = Kh;
             sc.xi_mid[parity] = nh_half_update(xi1, Kh, sp);
+            sc.eta += 0.5 * sp.dt * (xi1 + sc.xi_mid[parity]);
             sc.pair_pending = 0;
         }
     }
@@ -800,6 +808,9 @@ __global__ void __launch_bounds__(kLeanThreads) k_lean(BatchPtrs P, int32_t pari
         else if (dy < -0.5 * g.Ly) dy += g.Ly;
         const float2 im = make_float2(static_cast<float>(dx), static_cast<float>(dy));
         static_cast<float2 *>(P.img)[k] = im;
+        if (P.img_lo)
+            P.img_lo[k] = make_float2(static_cast<float>(dx - static_cast<double>(im.x)),
+                                      static_cast<float>(dy - static_cast<double>(im.y)));
         if (check) {
             // displacement since the build: the two images share the build
             // cell's centre (FP32; the thresholds are lowered by 1e-5 relative
@@ -1156,22 +1167,127 @@ __device__ inline void finalize_pair(int pmode, const double (&out)[3], const Sy
 struct PairF32 {
     float rc2s, rc2o, sig2s, sig2o, e24s, e24o, e4s, e4o, r2_guard;
     float fas, fao, fbs, fbo, uas, uao, ubs, ubo;   // PairTable fa, fb, ua, ub (list path)
+    float rcore2s, rcore2o;                         // PairTable rcore2f (compensated form)
 };
 
+// Repulsive-core pairs of the compensated FP32 form.  A pair whose FP32 r^2
+// is below rcore2f (r < kCoreRatio sigma_ij) is left out of the FP32 sums and
+// evaluated in FP64 instead, from the FP64 displacement that the image and
+// its rounding remainder (img + img_lo) give.  Such pairs are deferred: the
+// pass notes them (list rows in a 64-bit row mask, cell candidates in a
+// per-lane LDS queue of kCoreQ entries) and evaluates them after its FP32
+// loop, in the order noted, so the FP32 loop stays branch-free and the FP64
+// work runs once per noted pair instead of masked in every iteration.
+constexpr int kCoreQ = 9;
+
+// The particle's FP64 cell-relative position and what the core evaluation
+// needs of the grid.
+struct CoreCtx {
+    double px, py;
+    double w, h, Lx, Ly;
+    bool small_x, small_y;
+};
+
+__device__ inline CoreCtx make_core(const BatchPtrs &P, uint32_t s, const Grid &g) {
+    // read again here (volatile: not merged with the pass's earlier read of
+    // the image), so the image need not stay in registers through the pass
+    const volatile float *hi = static_cast<const volatile float *>(P.img) + 2 * static_cast<size_t>(s);
+    const volatile float *lo = reinterpret_cast<const volatile float *>(P.img_lo) + 2 * static_cast<size_t>(s);
+    CoreCtx c;
+    c.px = static_cast<double>(hi[0]) + static_cast<double>(lo[0]);   // exact in FP64
+    c.py = static_cast<double>(hi[1]) + static_cast<double>(lo[1]);
+    c.w = g.w;
+    c.h = g.h;
+    c.Lx = g.Lx;
+    c.Ly = g.Ly;
+    c.small_x = g.small_x != 0;
+    c.small_y = g.periodic_y && g.small_y;
+    return c;
+}
+
+// FP64 pair constants of one type pair (PairTable index).
+struct CoreConst {
+    double sig2, eps24, eps4, vc;
+};
+__device__ inline CoreConst core_const(const PairTable &pt, int pidx) {
+    return CoreConst{pt.sig2[pidx], pt.eps24[pidx], pt.eps4[pidx], pt.vc[pidx]};
+}
+__device__ inline CoreConst core_select(bool same, const CoreConst &a, const CoreConst &b) {
+    return CoreConst{same ? a.sig2 : b.sig2, same ? a.eps24 : b.eps24, same ? a.eps4 : b.eps4,
+                     same ? a.vc : b.vc};
+}
+
+// FP64 force (and energy sums) of one pair from its FP64 displacement, added
+// to (gx, gy).  1/r^2 from the hardware FP64 reciprocal refined by two Newton
+// steps (relative error at the FP64 rounding level).  Only core pairs come
+// here, all well inside the cutoff.
+template <bool OBS>
+__device__ inline void pair_core64(double dx, double dy, const CoreConst &k, double &gx, double &gy, ObsSum &ob,
+                                   bool want_obs) {
+    const double r2 = __builtin_fma(dx, dx, dy * dy);
+    if (!(r2 > 0.0)) return;   // coincident particles: no pair, as in the FP32 pass
+    double y = __builtin_amdgcn_rcp(r2);
+    y = __builtin_fma(__builtin_fma(-r2, y, 1.0), y, y);
+    y = __builtin_fma(__builtin_fma(-r2, y, 1.0), y, y);
+    const double sr2 = k.sig2 * y;
+    const double sr6 = sr2 * sr2 * sr2;
+    const double sr12 = sr6 * sr6;
+    const double tmp = k.eps24 * __builtin_fma(2.0, sr12, -sr6) * y;
+    const double fx = tmp * dx, fy = tmp * dy;
+    gx += fx;
+    gy += fy;
+    if constexpr (OBS) {
+        if (want_obs) {
+            ob.u += 0.5 * (k.eps4 * (sr12 - sr6) - k.vc);
+            ob.w += 0.5 * (dx * fx + dy * fy);
+            ob.lam += 0.5 * (fx * dx - fy * dy);
+        }
+    }
+}
+
+// FP64 displacement of a core pair: the particle's FP64 position moved by
+// minus the partner cell's offset (sx, sy) = (ox w, oy h), minus the partner's
+// FP64 position (image q plus remainder ql), as the FP32 pass forms it in FP32.
+__device__ inline void core_disp(const CoreCtx &c, float2 q, float2 ql, double sx, double sy, double &dx,
+                                 double &dy) {
+    dx = (c.px - sx) - (static_cast<double>(q.x) + static_cast<double>(ql.x));
+    dy = (c.py - sy) - (static_cast<double>(q.y) + static_cast<double>(ql.y));
+    if (c.small_x) dx -= c.Lx * rint(dx / c.Lx);
+    if (c.small_y) dy -= c.Ly * rint(dy / c.Ly);
+}
+
+// Core pair of the particle with the partner at cell-ordered slot k, whose
+// cell is (ox, oy) cells away ((sx, sy) = (ox w, oy h)).
+template <bool OBS>
+__device__ inline void core_pair(const BatchPtrs &P, const CoreCtx &c, uint32_t k, double sx, double sy,
+                                 const CoreConst &kc, double &gx, double &gy, ObsSum &ob, bool want_obs) {
+    const float2 q = static_cast<const float2 *>(P.img)[k];
+    const float2 ql = P.img_lo[k];
+    double dx, dy;
+    core_disp(c, q, ql, sx, sy, dx, dy);
+    pair_core64<OBS>(dx, dy, kc, gx, gy, ob, want_obs);
+}
+
 // FP32 pairs of particle (pix, piy) -- its cell-relative image already moved
-// by minus this neighbour cell's offset -- with the candidates [k0, k1) of one
-// neighbour cell, in ascending slot order.  The cell's pair forces are summed
-// in FP32 and the cell total enters the particle's accumulator once: a
-// compensated (two-sum) add in the COMP32 form, an FP64 add in the ACC64 form
-// (which adds every pair in FP64, as the reference does).  Self is excluded by
-// r^2 > 0.  UNI: all type pairs share sigma (rc and sigma^2 need no select);
-// the arithmetic is identical to the general form.
-//
+// by minus this neighbour cell's offset (ox, oy) -- with the candidates
+// [k0, k1) of one neighbour cell, in ascending slot order.  The cell's pair
+// forces are summed in FP32 and the cell total enters the particle's
+// accumulator once: a compensated (two-sum) add in the COMP32 form, an FP64
+// add in the ACC64 form (which adds every pair in FP64, as the reference
+// does).  Self is excluded by r^2 > 0.  UNI: all type pairs share sigma (rc
+// and sigma^2 need no select); the arithmetic is identical to the general
+// form.  COMP32: core pairs go to the caller's queue cq (one lane's column,
+// entry j at cq[j * kBlock + lane]: slot k and the offset code (ox + 2) +
+// 5 (oy + 2)) while it has room, else (and with LPP > 1) they are evaluated
+// at once.  ACC64: pairs closer than r2_guard, whose FP32 (sigma/r)^12 could
+// overflow, are evaluated in FP64 from the FP32 displacement.
 template <int ACC, bool OBS, bool UNI, int LPP>
-__device__ __forceinline__ void pair_cell_f32(const char *__restrict__ imgb, const uint8_t *__restrict__ clsb,
-                                              uint32_t k0, uint32_t k1, float pix, float piy, int ci,
-                                              const Grid &g, const PairTable &pt, const PairF32 &pc,
-                                              ForceSum<ACC> &F, ObsSum &ob, bool want_obs) {
+__device__ __forceinline__ void pair_cell_f32(const BatchPtrs &P, uint32_t k0, uint32_t k1, float pix, float piy,
+                                              int ci, int ox, int oy, const Grid &g, const PairTable &pt,
+                                              const PairF32 &pc, ForceSum<ACC> &F, ObsSum &ob, bool want_obs,
+                                              const CoreCtx &cc, uint2 *cq, int &qn) {
+    const char *__restrict__ imgb = static_cast<const char *>(P.img);
+    const uint8_t *__restrict__ clsb = P.icls;
     float fxc = 0.0f, fyc = 0.0f;
     // with LPP lanes per particle, lane q of the group takes candidates
     // k0 + q, k0 + q + LPP, ...
@@ -1186,12 +1302,29 @@ __device__ __forceinline__ void pair_cell_f32(const char *__restrict__ imgb, con
         const bool same = cj == ci;
         const float rc2 = UNI ? pc.rc2s : (same ? pc.rc2s : pc.rc2o);
         if (rr < rc2 && rr > 0.0f) {
-            if (rr < pc.r2_guard) {
-                double fx, fy;
-                pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), pair_index(ci, cj), pt, fx, fy,
-                               ob, want_obs);
-                F.addd(fx, fy);
+            bool fp32 = true;
+            if constexpr (ACC == ACC_COMP32) {
+                const float rco = UNI ? pc.rcore2s : (same ? pc.rcore2s : pc.rcore2o);
+                if (rr < rco) {
+                    fp32 = false;
+                    if (LPP == 1 && qn < kCoreQ) {
+                        cq[qn * kBlock + threadIdx.x] = make_uint2(k, static_cast<uint32_t>((ox + 2) + 5 * (oy + 2)));
+                        ++qn;
+                    } else {
+                        core_pair<OBS>(P, cc, k, ox * cc.w, oy * cc.h, core_const(pt, pair_index(ci, cj)), F.gx,
+                                       F.gy, ob, want_obs);
+                    }
+                }
             } else {
+                if (rr < pc.r2_guard) {
+                    fp32 = false;
+                    double fx, fy;
+                    pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), pair_index(ci, cj), pt, fx, fy,
+                                   ob, want_obs);
+                    F.addd(fx, fy);
+                }
+            }
+            if (fp32) {
                 const float sig2 = UNI ? pc.sig2s : (same ? pc.sig2s : pc.sig2o);
                 const float e24 = same ? pc.e24s : pc.e24o;
                 const float r2i = fast_rcp(rr);
@@ -1222,19 +1355,37 @@ __device__ __forceinline__ void pair_cell_f32(const char *__restrict__ imgb, con
     if constexpr (ACC == ACC_COMP32) F.add(fxc, fyc);
 }
 
+// The queued core pairs of a cell pass (pair_cell_f32), in queue order.
+template <bool OBS>
+__device__ inline void cell_core_queue(const BatchPtrs &P, uint32_t s, int ci, const Grid &g, const PairTable &pt,
+                                       const uint2 *cq, int qn, double &gx, double &gy, ObsSum &ob, bool want_obs) {
+    if (qn <= 0) return;
+    const CoreCtx cc = make_core(P, s, g);
+    for (int j = 0; j < qn; ++j) {
+        const uint2 e = cq[j * kBlock + threadIdx.x];
+        const int ox = static_cast<int>(e.y % 5u) - 2, oy = static_cast<int>(e.y / 5u) - 2;
+        core_pair<OBS>(P, cc, e.x, ox * cc.w, oy * cc.h, core_const(pt, pair_index(ci, P.icls[e.x])), gx, gy, ob,
+                       want_obs);
+    }
+}
+
 // One neighbour-list entry of a particle whose image, moved by minus the
 // entry's neighbour-cell offset, is (psx, psy), with the partner image q: the
 // FP32 pair of pair_cell_f32 (the same displacement), its force evaluated in
 // powers of 1/r^2 as F/r = r^-2 r^-6 (fa r^-6 - fb).  The force is computed
 // unconditionally and kept only when the pair is inside the cutoff, so no
-// branch is taken per entry.  GUARD: pairs closer than the FP64 guard are left
-// out here; `slow` collects, as a bit per lane, whether one of them is also
-// farther than 0 (0 < r^2 < r2_guard, compared as IEEE bit patterns, which
-// order like the values for r^2 >= 0), and the caller evaluates those in FP64.
-template <int ACC, bool OBS, bool UNI, bool GUARD>
+// branch is taken per entry.  CHK selects what else is checked:
+//   0  nothing;
+//   1  (ACC64) pairs closer than the FP64 guard are left out here; `flag`
+//      collects, as a bit per lane, whether one of them is also farther than
+//      0 (0 < r^2 < r2_guard, compared as IEEE bit patterns, which order like
+//      the values for r^2 >= 0), and the caller evaluates those in FP64;
+//   2  (COMP32) core pairs (r^2 < rcore2) are left out here; `flag` is set
+//      to whether this entry is one.
+template <int ACC, bool OBS, bool UNI, int CHK>
 __device__ __forceinline__ void list_pair_f32(uint32_t v, float2 q, float psx, float psy, int ci,
                                               const PairTable &pt, const PairF32 &pc, float &fxc, float &fyc,
-                                              ForceSum<ACC> &F, ObsSum &ob, bool want_obs, bool &slow) {
+                                              ForceSum<ACC> &F, ObsSum &ob, bool want_obs, bool &flag) {
     const float dx = psx - q.x;
     const float dy = psy - q.y;
     const float rr = __builtin_fmaf(dx, dx, dy * dy);
@@ -1245,7 +1396,17 @@ __device__ __forceinline__ void list_pair_f32(uint32_t v, float2 q, float psx, f
     const float r2i = fast_rcp(rr);
     const float r6i = r2i * r2i * r2i;
     const float t = __builtin_fmaf(fa, r6i, -fb) * r6i * r2i;
-    const bool in = GUARD ? (rr < rc2 && rr >= pc.r2_guard) : rr < rc2;
+    bool in;
+    if constexpr (CHK == 2) {
+        const float rco = UNI ? pc.rcore2s : (same ? pc.rcore2s : pc.rcore2o);
+        flag = rr < rco;
+        in = rr < rc2 && !flag;
+    } else if constexpr (CHK == 1) {
+        in = rr < rc2 && rr >= pc.r2_guard;
+        flag = flag || (__float_as_uint(rr) - 1u < __float_as_uint(pc.r2_guard) - 1u);
+    } else {
+        in = rr < rc2;
+    }
     const float tmp = in ? t : 0.0f;
     if constexpr (ACC == ACC_COMP32) {   // the run's FP32 sum, one rounding per term
         fxc = __builtin_fmaf(tmp, dx, fxc);
@@ -1253,8 +1414,6 @@ __device__ __forceinline__ void list_pair_f32(uint32_t v, float2 q, float psx, f
     } else {
         F.add(tmp * dx, tmp * dy);
     }
-    if constexpr (GUARD)
-        slow = slow || (__float_as_uint(rr) - 1u < __float_as_uint(pc.r2_guard) - 1u);
     if constexpr (OBS) {
         if (want_obs && in) {
             const float fx = tmp * dx, fy = tmp * dy;
@@ -1290,18 +1449,21 @@ __device__ __forceinline__ void list_pairs_guarded(const uint32_t (&v)[U], const
     }
 }
 
-// Rows [0, n) of slot s's list with one lane per particle; n is the same for
-// the whole wave, so the loop is uniform.  Each pass takes four rows; the
-// next pass's rows are loaded before this pass's pairs are evaluated (the last
-// pass reloads its own rows).  Rows are addressed from a scalar base plus the
-// lane's 32-bit byte offset.  shb: the workgroup's [9][kBlock] table of the
-// particles' images moved by minus each neighbour-cell offset, read at the
-// entry's code (list_shift_off).
-template <int ACC, bool OBS, bool UNI, bool GUARD>
-__device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, uint32_t n, const char *shb, int ci,
-                                              const PairTable &pt, const PairF32 &pc, ForceSum<ACC> &F,
-                                              ObsSum &ob, bool want_obs, float &fxc, float &fyc, double &gx,
-                                              double &gy) {
+// Rows [e0, e1) of slot s's list with one lane per particle; the bounds are
+// the same for the whole wave, so the loop is uniform.  Each pass takes four
+// rows; the next pass's rows are loaded before this pass's pairs are
+// evaluated (the last pass reloads its own rows).  Rows are addressed from a
+// scalar base plus the lane's 32-bit byte offset.  shb: the workgroup's
+// [9][kBlock] table of the particles' images moved by minus each
+// neighbour-cell offset, read at the entry's code (list_shift_off).  CHK as
+// list_pair_f32; with CHK == 2 (at most 32 rows) each row shifts its core flag
+// into m from the right (m = 2 m + flag), so row e0 + j ends at bit
+// e1 - e0 - 1 - j.
+template <int ACC, bool OBS, bool UNI, int CHK>
+__device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, uint32_t e0, uint32_t e1,
+                                              const char *shb, int ci, const PairTable &pt, const PairF32 &pc,
+                                              ForceSum<ACC> &F, ObsSum &ob, bool want_obs, float &fxc, float &fyc,
+                                              double &gx, double &gy, uint32_t &m) {
     constexpr int U = 4;
     const char *imgb = static_cast<const char *>(P.img);
     const char *lb = reinterpret_cast<const char *>(P.nlist);
@@ -1309,26 +1471,37 @@ __device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, ui
     const uint32_t soff = s * 4u;
     const uint32_t lane8 = threadIdx.x * 8u;
     uint32_t v[U];
+    {
+        const char *row = lb + static_cast<size_t>(e0) * rowb;
 #pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const uint32_t *>(lb + u * rowb + soff);
-    for (uint32_t e = 0; e < n; e += U) {
+        for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const uint32_t *>(row + u * rowb + soff);
+    }
+    for (uint32_t e = e0; e < e1; e += U) {
         float2 q[U], ps[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             q[u] = *reinterpret_cast<const float2 *>(imgb + (v[u] & kListOff));
             ps[u] = *reinterpret_cast<const float2 *>(shb + (list_shift_off(v[u]) | lane8));
         }
-        const uint32_t en = e + U < n ? e + U : e;
+        const uint32_t en = e + U < e1 ? e + U : e;
         const char *row = lb + static_cast<size_t>(en) * rowb;
         uint32_t vn[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) vn[u] = *reinterpret_cast<const uint32_t *>(row + u * rowb + soff);
         bool slow = false;
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            list_pair_f32<ACC, OBS, UNI, GUARD>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob, want_obs,
-                                                slow);
-        if constexpr (GUARD) {
+        for (int u = 0; u < U; ++u) {
+            if constexpr (CHK == 2) {
+                bool cf = false;
+                list_pair_f32<ACC, OBS, UNI, CHK>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
+                                                  want_obs, cf);
+                m = m + m + (cf ? 1u : 0u);   // one add with carry-in
+            } else {
+                list_pair_f32<ACC, OBS, UNI, CHK>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
+                                                  want_obs, slow);
+            }
+        }
+        if constexpr (CHK == 1) {
             if (__any(slow)) list_pairs_guarded<U, OBS>(v, q, ps, ci, pt, pc, gx, gy, ob, want_obs);
         }
         if constexpr (ACC == ACC_COMP32) {
@@ -1342,24 +1515,83 @@ __device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, ui
     }
 }
 
+// Takes the first (lowest-numbered) row left in a core mask of list_rows_f32:
+// mask m holds rows [base, base + cnt), row base + j at bit cnt - 1 - j.
+__device__ inline bool core_take(uint32_t &m, uint32_t base, uint32_t cnt, uint32_t &e) {
+    if (m == 0u) return false;
+    const uint32_t pos = 31u - static_cast<uint32_t>(__builtin_clz(m));
+    m ^= 1u << pos;
+    e = base + cnt - 1u - pos;
+    return true;
+}
+
+// The core rows of slot s's list marked by list_rows_f32 in m1 (rows [0, n1))
+// and m2 (rows [32, 32 + n2)), in ascending row order, evaluated in FP64.
+// Each pass takes up to four rows per lane and issues their loads together
+// (entry, then partner image and remainder), so a wave waits on one chain of
+// loads per pass; a wave runs as many passes as its lane with the most core
+// rows needs (usually one).
+template <bool OBS>
+__device__ inline void list_core_rows(const BatchPtrs &P, uint32_t s, int ci, const Grid &g, const PairTable &pt,
+                                      uint32_t m1, uint32_t n1, uint32_t m2, uint32_t n2, double &gx, double &gy,
+                                      ObsSum &ob, bool want_obs) {
+    if (!__any((m1 | m2) != 0u)) return;
+    constexpr int B = 2;
+    const CoreCtx cc = make_core(P, s, g);
+    const CoreConst ks = core_const(pt, ci), ko = core_const(pt, 2);
+    const size_t stride = static_cast<size_t>(P.N);
+    const float2 *img = static_cast<const float2 *>(P.img);
+    while (__any((m1 | m2) != 0u)) {
+        uint32_t e[B], v[B];
+        bool ok[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            e[j] = 0u;
+            ok[j] = core_take(m1, 0u, n1, e[j]) || core_take(m2, 32u, n2, e[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) v[j] = ok[j] ? P.nlist[static_cast<size_t>(e[j]) * stride + s] : 0u;
+        float2 q[B], ql[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            if (ok[j]) {
+                const uint32_t k = (v[j] & kListOff) >> 3;
+                q[j] = img[k];
+                ql[j] = P.img_lo[k];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            if (ok[j]) {
+                double dx, dy;
+                core_disp(cc, q[j], ql[j], static_cast<double>(list_ox(v[j])) * cc.w,
+                          static_cast<double>(list_oy(v[j])) * cc.h, dx, dy);
+                pair_core64<OBS>(dx, dy, core_select((v[j] & kListSame) != 0u, ks, ko), gx, gy, ob, want_obs);
+            }
+        }
+    }
+}
+
 // FP32 pairs of cell-ordered slot s from its neighbour list (list mode):
 // entries in the cell order of the build, each carrying its neighbour cell's
 // offset, so the displacement is formed exactly as in pair_cell_f32.  Pair
 // forces are summed in FP32 over runs of eight entries, each run entering the
 // compensated accumulator once (COMP32); ACC64 adds every pair in FP64.  With
 // LPP lanes per particle, lane q takes rows q, q + LPP, ...; rows past the
-// list stand for list_pad.
+// list stand for list_pad.  COMP32: core pairs are evaluated in FP64 (after
+// the rows with one lane per particle, at once with several).  shift: the
+// workgroup's [9][kBlock] LDS table (LPP == 1).
 template <int ACC, bool OBS, bool UNI, int LPP>
 __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, float2 pi, int ci, const Grid &g,
                                               const PairTable &pt, const PairF32 &pc, ForceSum<ACC> &F,
-                                              ObsSum &ob, bool want_obs) {
+                                              ObsSum &ob, bool want_obs, float2 (*shift)[kBlock],
+                                              uint32_t *stash) {
     constexpr int U = 4;
     float fxc = 0.0f, fyc = 0.0f;
     double gx = 0.0, gy = 0.0;
     if constexpr (LPP == 1) {
         // the nine shifted images of each particle, one table row per cell
         // code (list_entry), the lane's own column
-        __shared__ float2 shift[9][kBlock];
 #pragma unroll
         for (int oy = -1; oy <= 1; ++oy)
 #pragma unroll
@@ -1370,15 +1602,34 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, fl
         const char *shb = reinterpret_cast<const char *>(&shift[0][0]);
         const uint32_t nr = __builtin_amdgcn_readfirstlane(P.ncount[s]);
         const uint32_t n = nr & ~kListGuard;
-        if (nr & kListGuard)
-            list_rows_f32<ACC, OBS, UNI, true>(P, s, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy);
-        else
-            list_rows_f32<ACC, OBS, UNI, false>(P, s, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy);
+        uint32_t m1 = 0u, m2 = 0u;
+        if constexpr (ACC == ACC_COMP32) {
+            // core flags of rows [0, 32) in m1, of rows [32, n) in m2; m1 waits
+            // in LDS while the rows past 32 (long lists only) run
+            const uint32_t n1 = n < 32u ? n : 32u;
+            list_rows_f32<ACC, OBS, UNI, 2>(P, s, 0u, n1, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy, m1);
+            if (n > 32u) {
+                stash[threadIdx.x] = m1;
+                list_rows_f32<ACC, OBS, UNI, 2>(P, s, 32u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy,
+                                                m2);
+                m1 = static_cast<const volatile uint32_t *>(stash)[threadIdx.x];
+            }
+            list_core_rows<OBS>(P, s, ci, g, pt, m1, n1, m2, n > 32u ? n - 32u : 0u, gx, gy, ob, want_obs);
+        } else {
+            if (nr & kListGuard)
+                list_rows_f32<ACC, OBS, UNI, 1>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy, m1);
+            else
+                list_rows_f32<ACC, OBS, UNI, 0>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy, m1);
+        }
     } else {
+        (void)shift;
+        (void)stash;
         const char *imgb = static_cast<const char *>(P.img);
         const size_t stride = static_cast<size_t>(P.N);
         const uint32_t n = P.ncount[s] & ~kListGuard;
         const uint32_t pad = list_pad(s);
+        CoreCtx cc{};
+        if constexpr (ACC == ACC_COMP32) cc = make_core(P, s, g);
         uint32_t run = 0;
         for (uint32_t e = threadIdx.x % LPP; e < n; e += U * LPP) {
             uint32_t v[U];
@@ -1397,17 +1648,27 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, fl
             bool slow = false;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                list_pair_f32<ACC, OBS, UNI, true>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
-                                                   want_obs, slow);
                 if constexpr (ACC == ACC_COMP32) {
+                    bool core = false;
+                    list_pair_f32<ACC, OBS, UNI, 2>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
+                                                    want_obs, core);
+                    if (core)
+                        core_pair<OBS>(P, cc, (v[u] & kListOff) >> 3, static_cast<double>(list_ox(v[u])) * cc.w,
+                                       static_cast<double>(list_oy(v[u])) * cc.h,
+                                       core_const(pt, (v[u] & kListSame) ? ci : 2), gx, gy, ob, want_obs);
                     if (++run == 8u) {
                         F.add(fxc, fyc);
                         fxc = fyc = 0.0f;
                         run = 0;
                     }
+                } else {
+                    list_pair_f32<ACC, OBS, UNI, 1>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
+                                                    want_obs, slow);
                 }
             }
-            if (slow) list_pairs_guarded<U, OBS>(v, q, ps, ci, pt, pc, gx, gy, ob, want_obs);
+            if constexpr (ACC != ACC_COMP32) {
+                if (slow) list_pairs_guarded<U, OBS>(v, q, ps, ci, pt, pc, gx, gy, ob, want_obs);
+            }
         }
     }
     if constexpr (ACC == ACC_COMP32) F.add(fxc, fyc);
@@ -1507,6 +1768,13 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
     // epilogue's velocity load does not wait on them
     uint32_t id = 0;
     int ci = 0;
+    // one lane per particle, FP32 forms: the list pass's table of shifted
+    // images, or the cell pass's queue of core pairs (kCoreQ uint2 per lane)
+    __shared__ float2 lds_tab[(LPP == 1 && !kFp64) ? 9 : 1][kBlock];
+    __shared__ uint32_t lds_stash[(LPP == 1 && !kFp64) ? kBlock : 1];
+    static_assert(9 * sizeof(float2) >= kCoreQ * sizeof(uint2), "core queue fits the shift table");
+    uint2 *cq = reinterpret_cast<uint2 *>(&lds_tab[0][0]);
+    int qn = 0;
     if (li < sp.n) {
         const uint32_t s = P.cs[sp.coff] + static_cast<uint32_t>(li);
         id = P.sorted[s];
@@ -1534,7 +1802,12 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
             pc.uao = pt.ua[2];
             pc.ubs = pt.ub[ci];
             pc.ubo = pt.ub[2];
+            pc.rcore2s = pt.rcore2f[ci];
+            pc.rcore2o = pt.rcore2f[2];
         }
+        // own FP64 position for core pairs evaluated at once (LPP > 1)
+        CoreCtx cc{};
+        if constexpr (!kFp64 && ACC == ACC_COMP32 && LPP > 1) cc = make_core(P, s, g);
         bool by_list = false;
         if constexpr (!kFp64) {
             if (LEAN && (sc.fallback[parity] || !sc.list_on || !sc.list_built || sc.list_overflow)) {
@@ -1554,13 +1827,13 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
                         if (xx < 0) xx += g.ncx;
                         else if (xx >= g.ncx) xx -= g.ncx;
                         const uint32_t k0 = P.cs[rowbase + xx], k1 = P.cs[rowbase + xx + 1];
-                        pair_cell_f32<ACC, OBS, UNI, LPP>(imgb, P.icls, k0, k1, pi.x - static_cast<float>(ox) * g.wf,
-                                                          piy, ci, g, pt, pc, F, ob, want_obs);
+                        pair_cell_f32<ACC, OBS, UNI, LPP>(P, k0, k1, pi.x - static_cast<float>(ox) * g.wf, piy, ci,
+                                                          ox, oy, g, pt, pc, F, ob, want_obs, cc, cq, qn);
                     }
                 }
                 by_list = true;
             } else if (sc.list_on && !sc.list_overflow) {
-                pair_list_f32<ACC, OBS, UNI, LPP>(P, s, pi, ci, g, pt, pc, F, ob, want_obs);
+                pair_list_f32<ACC, OBS, UNI, LPP>(P, s, pi, ci, g, pt, pc, F, ob, want_obs, lds_tab, lds_stash);
                 by_list = true;
             }
         }
@@ -1593,15 +1866,17 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
                 pair_cell_f64<OBS>(img, P.icls, b0, b1, s, pi, ci, 0.0, sy, g, pt, F, ob, want_obs);
                 pair_cell_f64<OBS>(img, P.icls, c0, c1, s, pi, ci, sxr, sy, g, pt, F, ob, want_obs);
             } else {
-                const char *imgb = static_cast<const char *>(P.img);
                 const float piy = pi.y - static_cast<float>(oy) * g.hf;
-                pair_cell_f32<ACC, OBS, UNI, LPP>(imgb, P.icls, a0, a1, pi.x + g.wf, piy, ci, g, pt, pc, F, ob,
-                                                  want_obs);
-                pair_cell_f32<ACC, OBS, UNI, LPP>(imgb, P.icls, b0, b1, pi.x, piy, ci, g, pt, pc, F, ob, want_obs);
-                pair_cell_f32<ACC, OBS, UNI, LPP>(imgb, P.icls, c0, c1, pi.x - g.wf, piy, ci, g, pt, pc, F, ob,
-                                                  want_obs);
+                pair_cell_f32<ACC, OBS, UNI, LPP>(P, a0, a1, pi.x + g.wf, piy, ci, -1, oy, g, pt, pc, F, ob, want_obs,
+                                                  cc, cq, qn);
+                pair_cell_f32<ACC, OBS, UNI, LPP>(P, b0, b1, pi.x, piy, ci, 0, oy, g, pt, pc, F, ob, want_obs, cc, cq,
+                                                  qn);
+                pair_cell_f32<ACC, OBS, UNI, LPP>(P, c0, c1, pi.x - g.wf, piy, ci, 1, oy, g, pt, pc, F, ob, want_obs,
+                                                  cc, cq, qn);
             }
         }
+        if constexpr (!kFp64 && ACC == ACC_COMP32 && LPP == 1)
+            cell_core_queue<OBS>(P, s, ci, g, pt, cq, qn, F.gx, F.gy, ob, want_obs);
         if constexpr (LPP > 1) combine_lanes<LPP, kFp64 ? ACC_FP64 : ACC>(F, ob);
     }
     if (li < sp.n && lead) {

@@ -93,6 +93,7 @@ struct DevScalars {
     int32_t pair_parity;   // the step parity that pass ran with (0 outside rebuild windows)
     int32_t pad_s;
     double xi;           // Nose-Hoover variable after the last completed half update
+    double eta;          // integral of xi dt: sum over NVT steps of dt (xi1 + xi_mid) / 2
     // xi of a step in flight (after both half updates), by step parity: steps
     // outside rebuild windows use [0]; a window's lean steps alternate, each
     // reading the previous step's entry while writing its own
@@ -147,6 +148,11 @@ struct BatchPtrs {
     uint32_t *key_list;
     double2 *x0;
     float2 *img0;     // per cell-ordered slot: its FP32 image at the last list-mode build
+    // per cell-ordered slot, compensated FP32 form only (else null): the
+    // rounding remainder of the FP32 image, lo = fl32(d - img) with d the FP64
+    // cell-relative position, so img + lo holds d to about 2^-48 relative;
+    // the pair pass forms FP64 displacements of close pairs from it
+    float2 *img_lo;
     uint32_t *nlist, *ncount;
     // storage order: particle data (x, v, a, cls, key, key_list, x0) sits at
     // storage index i; pid[i] is its original (load-order) global index and

@@ -19,6 +19,7 @@ struct Sim {
     orc_params p;
     std::vector<orc_particle> P;
     double xi = 0.0;          // src/md_env.cu:24
+    double eta = 0.0;         // integral of xi dt (not in the reference: for the NH conserved energy)
     double t = 0.0;
     double Lx0 = 0.0, Ly0 = 0.0;   // src/md_env.cu:30-31
     double last_P = 0.0;
@@ -365,6 +366,7 @@ void step_nvt(Sim &s, bool middle_wrap) {
         vy += 0.5 * dt * (q.ay - xi2 * vy);
         q.vx = vx; q.vy = vy;
     }
+    s.eta += 0.5 * dt * (xi1 + xi2);
     s.t += dt;
 }
 
@@ -584,6 +586,7 @@ void orc_get_obs(const orc_sim *s, orc_obs *o) {
     o->t = sim.t;
     o->last_P = sim.last_P;
     o->area_rate = sim.p_A / std::max(sim.p.barostat_mass, 1e-12);
+    o->eta = sim.eta;
 }
 
 void orc_step_nve(orc_sim *s, int32_t nsteps) {

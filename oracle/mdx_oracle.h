@@ -83,6 +83,8 @@ typedef struct orc_obs {
     double t;
     double last_P;         /* get_last_instant_pressure() */
     double area_rate;      /* get_nph_area_rate() */
+    double eta;            /* integral of xi dt over the NVT steps (Nose-Hoover conserved energy
+                              K + U + Q xi^2 / 2 + g T0 eta; not in the reference) */
 } orc_obs;
 
 typedef struct orc_fire_params {

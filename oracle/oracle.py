@@ -39,7 +39,7 @@ class OParams(C.Structure):
 
 class OObs(C.Structure):
     _fields_ = [(k, C.c_double) for k in
-                ("K", "U", "W", "P", "xi", "p_A", "v_p", "F_p", "Lx", "Ly", "t", "last_P", "area_rate")]
+                ("K", "U", "W", "P", "xi", "p_A", "v_p", "F_p", "Lx", "Ly", "t", "last_P", "area_rate", "eta")]
 
 
 class OFireParams(C.Structure):

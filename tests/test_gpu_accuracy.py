@@ -23,7 +23,9 @@ pytestmark = pytest.mark.gpu
 N, RHO, T0, SEED = 4096, 0.8, 0.7, 1001
 KW = dict(eps_ab=0.5, dt=0.005)
 STEPS, EVERY = 10000, 500
-FORCE_MAX_REL = 1.2e-5
+# compensated form (core pairs in FP64): 1e-5; the FP64-accumulating form
+# evaluates every pair in FP32, as the reference does, and is held to 1.2e-5
+FORCE_MAX_REL = {abi.PREC_COMP32: 1e-5, abi.PREC_ACC64: 1.2e-5}
 
 
 def say(msg):
@@ -100,12 +102,12 @@ def test_force_pe_pressure_error_at_start():
         dp = abs(ob.P - P_o) / abs(P_o)
         say(f"prec={prec}: force max rel {fe:.2e} rms rel {fr:.2e}, PE/particle err {du:.2e}, "
             f"pressure rel err {dp:.2e}")
-        # FP32 pair evaluation from cell-relative FP32 images; measured on this
-        # frame (compensated form): force max 1.04e-5, rms 1.7e-6 (rounding of
-        # the images and of the pair arithmetic in equal parts, amplified where
-        # near-contact pair forces cancel), PE 1.5e-9 per particle, pressure
-        # 2.0e-6 relative (the virial sums FP32 pair forces)
-        assert fe <= FORCE_MAX_REL
+        # FP32 pair evaluation from cell-relative FP32 images: the rounding of
+        # the images and of the pair arithmetic, amplified where near-contact
+        # pair forces cancel.  All-FP32 pairs (the FP64-accumulating form)
+        # measured force max 1.04e-5, rms 1.7e-6 on this frame; PE 1.5e-9 per
+        # particle, pressure 2.0e-6 relative (the virial sums FP32 pair forces)
+        assert fe <= FORCE_MAX_REL[prec]
         assert fr <= 2.5e-6
         assert du <= 1e-7
         assert dp <= 5e-6

@@ -1,0 +1,201 @@
+"""The timed path against the FP64 oracle.
+
+The headline workload runs Nose-Hoover NVT in the compensated FP32 form with
+neighbour lists rebuilt in windows: a window's lean steps take the kinetic
+energy after the first kick in closed form from the last pair pass's sums
+(K(v') = c^2 K + (c dt/2) sum m v.a + (dt^2/4) sum m |a|^2 / 2 with
+c = 1 - dt xi / 2), not from a reduction over the particles.  These tests run
+that path and the oracle's NVT (src/md_env.cu:1639-1728 restated) from the
+same frame and compare:
+
+- per-particle forces at the bench configuration (N = 1,048,576, 2:1 box,
+  density 0.75) after 100 windowed steps: the accelerations the list pass
+  stored, and a fresh cell-path pass, against the oracle's FP64 cell-list
+  forces at the same positions, max_i |dF_i| / rms |F| <= 1e-5 over the
+  particles without a pair in the cutoff's FP32 rounding band;
+- over 1000 steps at N = 4096 (square lattice, density 0.8) and on a 2:1
+  slab of 3000 particles: xi(t) and K(t) while the trajectories coincide, then
+  the drift of the Nose-Hoover conserved energy H = K + U + Q xi^2 / 2 +
+  g T0 eta and the mean temperature; over 50 steps at the bench
+  configuration: xi, K and positions.  Tolerances are stated at each assert.
+
+A factor error in the closed form (a dropped 1/2, a wrong c) moves xi by
+O(1e-2) within a few windows and breaks the conservation of H; FP32 rounding
+moves xi by O(1e-7).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from md_simulation_cuda_amd import abi
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FORCE_MAX_REL = 1e-5
+FORCE_RMS_REL = 2e-6
+BENCH_TUNING = (1.0, 1, 0.65, 16)   # cell_scale, lanes, skin, window: the autotuner's pick at N = 1M
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bench_frame():
+    n = 1048576
+    Ly = math.sqrt(n / 0.75 / 2.0)
+    Lx = 2.0 * Ly
+    f = abi.init_particles(n, n // 2, Lx, Ly, 0.6, devide_p=0.5)
+    kw = dict(eps_aa=1.0, eps_bb=1.0, eps_ab=0.5, dt=0.005, T_target=0.6, Q=10.0, cutoff=2.5)
+    return f, Lx, Ly, kw
+
+
+def cutoff_band(frame, Lx, Ly, rc=2.5, band=1e-6):
+    """Particles with a pair whose r^2 is within band * rc^2 of rc^2 (all
+    sigma = 1): FP32 and FP64 may put such a pair on either side of the
+    cutoff, where the unshifted force jumps by F(rc) ~ 0.04."""
+    from scipy.spatial import cKDTree
+
+    pos = np.mod(frame["pos"], [Lx, Ly])
+    tree = cKDTree(pos, boxsize=[Lx, Ly])
+    pairs = tree.query_pairs(rc * math.sqrt(1.0 + band), output_type="ndarray")
+    d = pos[pairs[:, 0]] - pos[pairs[:, 1]]
+    d -= np.round(d / [Lx, Ly]) * [Lx, Ly]
+    r2 = (d ** 2).sum(1)
+    near = pairs[np.abs(r2 - rc * rc) < band * rc * rc]
+    m = np.zeros(len(frame), bool)
+    m[near.ravel()] = True
+    return m
+
+
+def force_errors(acc, acc_o, band):
+    rms = float(np.sqrt((acc_o ** 2).sum(1).mean()))
+    err = np.sqrt(((acc - acc_o) ** 2).sum(1))
+    return float(err[~band].max() / rms), float(np.sqrt((err ** 2).mean()) / rms)
+
+
+def test_bench_config_forces_after_windowed_steps():
+    f, Lx, Ly, kw = bench_frame()
+    say("N=1M NVT, skin 0.65 windows of 16, 100 steps (compensated FP32)")
+    s = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), f)
+    s.set_tuning(*BENCH_TUNING)
+    s.step(abi.NVT, 100)
+    fr = s.collect()
+    say("  fresh cell-path pass")
+    acc_cell = s.compute_forces(True)
+    s.close()
+    say("  oracle FP64 cell-list forces at the same positions")
+    o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, **kw), fr)
+    acc_o, _, _, _ = o.pair_pass(True)
+    o.close()
+    band = cutoff_band(fr, Lx, Ly)
+    mx_l, rms_l = force_errors(fr["acc"], acc_o, band)
+    mx_c, rms_c = force_errors(acc_cell, acc_o, band)
+    say(f"  list pass (stored FP32 accelerations): max {mx_l:.3e} rms {rms_l:.3e}; "
+        f"cell pass: max {mx_c:.3e} rms {rms_c:.3e}; band particles {int(band.sum())}")
+    assert mx_l <= FORCE_MAX_REL and rms_l <= FORCE_RMS_REL
+    assert mx_c <= FORCE_MAX_REL and rms_c <= FORCE_RMS_REL
+
+
+def nvt_series(f, Lx, Ly, kw, tuning, steps, every):
+    """The windowed GPU run and the oracle's NVT from the same frame; records
+    (xi, K, Nose-Hoover conserved energy H = K + U + Q xi^2 / 2 + g T0 eta)
+    every `every` steps, and the position difference at the end."""
+    g = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), f)
+    g.set_tuning(*tuning)
+    o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, **kw), f)
+    o.compute_forces()
+    gdof = 2 * len(f) - 2
+    Q, T0 = kw["Q"], kw["T_target"]
+    rec = {k: [] for k in ("xi_g", "xi_o", "K_g", "K_o", "H_g", "H_o")}
+
+    def record():
+        ob = g.observables(abi.OBS_K | abi.OBS_U)
+        oo = o.obs()
+        Ko, Uo = o.K(), o.U()
+        rec["xi_g"].append(ob.xi)
+        rec["K_g"].append(ob.K)
+        rec["H_g"].append(ob.K + ob.U + 0.5 * Q * ob.xi ** 2 + gdof * T0 * ob.eta)
+        rec["xi_o"].append(oo.xi)
+        rec["K_o"].append(Ko)
+        rec["H_o"].append(Ko + Uo + 0.5 * Q * oo.xi ** 2 + gdof * T0 * oo.eta)
+
+    record()
+    for _ in range(steps // every):
+        g.step(abi.NVT, every)
+        o.step_nvt(every)
+        record()
+    fg, fo = g.collect(), o.frame()
+    g.close()
+    o.close()
+    dpos = fg["pos"] - fo["pos"]
+    dpos -= np.round(dpos / [Lx, Ly]) * [Lx, Ly]
+    r = {k: np.array(v) for k, v in rec.items()}
+    r["dx_max"] = float(np.abs(dpos).max())
+    r["dx_rms"] = float(np.sqrt((dpos ** 2).sum(1).mean()))
+    return r
+
+
+CASES = {
+    # N = 4096 square lattice, density 0.8, Nose-Hoover target 0.7
+    "square_4096": (lambda: abi.gen_lattice(4096, 2048, 0.8, 0.05, 0.7, seed=1001),
+                    dict(eps_ab=0.5, dt=0.005, T_target=0.7, Q=10.0)),
+    # 2:1 slab: phase-separated constructor start, 3000 particles, density 0.75
+    "slab_2to1_3000": (lambda: (abi.init_particles(3000, 1500, math.sqrt(3000 / 0.75 * 2.0),
+                                                   math.sqrt(3000 / 0.75 / 2.0), 0.6, devide_p=0.5),
+                                math.sqrt(3000 / 0.75 * 2.0), math.sqrt(3000 / 0.75 / 2.0)),
+                       dict(eps_ab=0.5, dt=0.005, T_target=0.6, Q=10.0)),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_windowed_nvt_follows_oracle(case):
+    """1000 steps.  While the FP32 and FP64 trajectories are still the same
+    (the first 100 steps, two records) xi and K agree to FP32 rounding; after
+    that they separate chaotically particle by particle (the positions differ
+    by O(1) at 1000 steps), and what stays comparable is the Nose-Hoover
+    conserved energy, whose drift (first order in dt in this scheme) must
+    stay within 2x of the oracle's, and the mean kinetic temperature."""
+    make, kw = CASES[case]
+    f, Lx, Ly = make()
+    n = len(f)
+    say(f"{case}: windowed list NVT (skin 0.65, windows of 16) vs oracle NVT, 1000 steps")
+    r = nvt_series(f, Lx, Ly, kw, BENCH_TUNING, 1000, 50)
+    scale = float(np.abs(r["xi_o"]).max())
+    dxi = np.abs(r["xi_g"] - r["xi_o"]) / scale
+    dk = np.abs(r["K_g"] - r["K_o"]) / np.abs(r["K_o"])
+    drift_g = float(np.abs(r["H_g"] - r["H_g"][0]).max() / n)
+    drift_o = float(np.abs(r["H_o"] - r["H_o"][0]).max() / n)
+    t_g = float(r["K_g"][10:].mean() / (n - 1))
+    t_o = float(r["K_o"][10:].mean() / (n - 1))
+    say(f"  xi range [{r['xi_o'].min():.4f}, {r['xi_o'].max():.4f}]; first 100 steps: max |dxi|/max|xi| "
+        f"{dxi[:3].max():.2e}, max |dK|/K {dk[:3].max():.2e}; 1000 steps: |dxi| {dxi.max():.2e}, "
+        f"|dK|/K {dk.max():.2e}, max |dx| {r['dx_max']:.2e}; H_NH drift per particle gpu {drift_g:.3e} "
+        f"oracle {drift_o:.3e} (ratio {drift_g / drift_o:.2f}); mean T (last 500 steps) gpu {t_g:.4f} "
+        f"oracle {t_o:.4f}")
+    assert dxi[:3].max() <= 1e-5
+    assert dk[:3].max() <= 3e-6
+    assert drift_g <= 2.0 * drift_o
+    assert abs(t_g - t_o) <= 0.03 * t_o
+
+
+def test_windowed_nvt_bench_config_50_steps():
+    """At the bench configuration the global xi and K follow the oracle to
+    FP32 rounding over 50 steps (three windows); particle positions agree to
+    1e-6 rms (the largest single difference, from the chaotic growth of FP32
+    rounding in a few close encounters, to 1e-3)."""
+    f, Lx, Ly, kw = bench_frame()
+    say("N=1M: windowed list NVT vs oracle NVT, 50 steps")
+    r = nvt_series(f, Lx, Ly, kw, BENCH_TUNING, 50, 10)
+    scale = float(np.abs(r["xi_o"]).max())
+    dxi = float(np.abs(r["xi_g"] - r["xi_o"]).max() / scale)
+    dk = float((np.abs(r["K_g"] - r["K_o"]) / np.abs(r["K_o"])).max())
+    say(f"  xi {r['xi_g'][-1]:.6e} vs {r['xi_o'][-1]:.6e}; max |dxi|/max|xi| {dxi:.2e}; max |dK|/K {dk:.2e}; "
+        f"max |dx| {r['dx_max']:.2e}, rms |dx| {r['dx_rms']:.2e}")
+    assert dxi <= 1e-6
+    assert dk <= 1e-8
+    assert r["dx_rms"] <= 1e-6
+    assert r["dx_max"] <= 1e-3

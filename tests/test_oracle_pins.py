@@ -1,0 +1,224 @@
+"""Pins of the oracle's integrators and FIRE schedule that do not rest on the
+oracle's own formulas.
+
+- FIRE: an independent numpy FIRE written from Bitzek et al., PRL 97, 170201
+  (2006) -- MD kick, power P = F.v, velocity mixing v <- (1 - alpha) v +
+  alpha |v| F/|F|, and the adaptive rules (P > 0 for more than N_min steps:
+  dt <- min(f_inc dt, dt_max), alpha <- 0.99 alpha; P <= 0: dt <- f_dec dt,
+  alpha <- alpha0, v <- 0) -- taken in the reference's step order (kick with
+  the current forces, then P and the schedule, then the mixing, then the
+  drift; src/md_energy_minimizer.cu:170-215).  On two particles the
+  reference's per-particle mixing equals the paper's global mixing (both
+  particles have the same |v| and |F|), so the trajectories must agree step
+  for step.  The run is checked to visit every branch of the schedule: a
+  streak beyond N_min, dt capped at dt_max, alpha decay, and resets.  Two
+  particles keep v parallel to F, where the mixing (and alpha) has no effect;
+  an irregular four-particle cluster pins the reference's per-particle
+  mixing and alpha schedule.
+- Nose-Hoover NVT (src/md_env.cu:1639-1728): the conserved energy
+  H = K + U + Q xi^2 / 2 + g T0 eta (eta = integral of xi dt, g = 2N - 2)
+  drifts by an amount that halves with dt.  The reference applies the
+  friction of each half kick explicitly with the freshly updated xi, which
+  makes the scheme first order in H; a dropped 1/2, a wrong g or a swapped K
+  would leave an O(1) drift that does not scale with dt.
+- NPH area barostat (src/md_env.cu:1795-1918): H = K + U + p_A^2 / (2 W) +
+  P_ext A is conserved to O(dt^2): its drift falls 4x per halving of dt.
+"""
+import math
+
+import numpy as np
+
+from oracle import oracle as O
+from tests.helpers import lattice_frame
+
+
+# ---------------------------------------------------------------------------
+# FIRE
+# ---------------------------------------------------------------------------
+
+def lj_force_pair(d, eps=1.0, sig=1.0, rc=2.5):
+    """Force on the particle at +d from the one at 0: F = 24 eps (2 s^12/r^13
+    - s^6/r^7) d/r inside the cutoff (closed form of the LJ potential)."""
+    r = math.hypot(d[0], d[1])
+    if r >= rc * sig or r == 0.0:
+        return np.zeros(2)
+    f = 24.0 * eps * (2.0 * sig ** 12 / r ** 13 - sig ** 6 / r ** 7)
+    return f * np.asarray(d) / r
+
+
+def lj_forces(x, L):
+    """Forces on every particle (all pairs, minimum image, one type)."""
+    F = np.zeros_like(x)
+    for i in range(len(x)):
+        for j in range(len(x)):
+            if i != j:
+                d = x[i] - x[j]
+                d -= L * np.round(d / L)
+                F[i] += lj_force_pair(d)
+    return F
+
+
+def numpy_fire(pos, L, dt0, dt_max, alpha0, f_inc, f_dec, n_min, steps, per_particle=False):
+    """Bitzek et al.'s FIRE (mass 1, periodic square box L) in the reference's
+    step order; per_particle mixes each particle's velocity with its own force
+    (the reference's variant, src/md_energy_minimizer.cu:40-60) instead of the
+    paper's global vectors.  Returns per-step positions, velocities, dt, alpha
+    and the branch taken."""
+    x = np.array(pos, float)
+    v = np.zeros_like(x)
+    dt, alpha, streak = dt0, alpha0, 0
+    out = []
+    for _ in range(steps):
+        F = lj_forces(x, L)
+        v = v + dt * F                       # MD kick with the current forces
+        P = float((F * v).sum())             # power
+        if P > 0.0:
+            streak += 1
+            branch = "grow" if streak > n_min else "hold"
+            if streak > n_min:
+                dt = min(dt * f_inc, dt_max)
+                alpha = alpha * 0.99
+        else:
+            branch = "reset"
+            dt *= f_dec
+            alpha = alpha0
+            streak = 0
+            v = np.zeros_like(v)
+        if per_particle:
+            vn = np.linalg.norm(v, axis=1, keepdims=True)
+            fn = np.linalg.norm(F, axis=1, keepdims=True)
+            ok = (vn > 0.0) & (fn > 0.0)
+            v = np.where(ok, (1.0 - alpha) * v + alpha * vn * F / np.where(ok, fn, 1.0), v)
+        else:                                # global mixing
+            vn = np.linalg.norm(v)
+            fn = np.linalg.norm(F)
+            if vn > 0.0 and fn > 0.0:
+                v = (1.0 - alpha) * v + alpha * vn * F / fn
+        x = np.mod(x + dt * v, L)            # drift
+        out.append((x.copy(), v.copy(), dt, alpha, branch))
+    return out
+
+
+def oracle_fire_frames(pos, L, dt0, dt_max, alpha0, f_inc, f_dec, n_min, steps):
+    """The oracle's FIRE stopped after k = 1..steps iterations (a fresh
+    oracle each time; force_tol 0 so it never converges early)."""
+    frames = []
+    for k in range(1, steps + 1):
+        f = np.zeros(len(pos), O.PARTICLE_DTYPE)
+        f["pos"] = pos
+        o = O.OracleSim(O.make_params(L, L, force_mode=0, eps_ab=1.0), f)
+        o.fire(dt0, dt_max, alpha0, f_inc, f_dec, n_min, max_steps=k, force_tol=0.0)
+        frames.append(o.frame())
+        o.close()
+    return frames
+
+
+def test_fire_schedule_against_numpy_fire():
+    L = 20.0
+    pos = [[9.55, 10.0], [10.45, 10.0]]   # 0.9 sigma apart: repulsive start
+    args = (0.005, 0.012, 0.1, 1.1, 0.5, 5)
+    steps = 120
+    ref = numpy_fire(pos, L, *args, steps)
+    got = oracle_fire_frames(pos, L, *args, steps)
+    for k, ((x, v, dt, alpha, _), fr) in enumerate(zip(ref, got)):
+        assert np.allclose(fr["pos"], x, rtol=0, atol=1e-12), k
+        assert np.allclose(fr["vel"], v, rtol=0, atol=1e-12), k
+    branches = [b for *_, b in ref]
+    dts = [dt for _, _, dt, _, _ in ref]
+    alphas = [a for _, _, _, a, _ in ref]
+    assert "grow" in branches and "reset" in branches and "hold" in branches
+    assert max(dts) == 0.012 and dts.count(0.012) > 5   # dt held at the cap
+    assert min(alphas) < 0.1 * 0.99 ** 5         # alpha decayed over a streak
+    assert any(b == "reset" for b in branches[10:])
+
+
+def test_fire_per_particle_mixing_against_numpy():
+    """Four particles of an irregular cluster: velocities turn away from
+    their forces, so the mixing (and alpha) shapes the path; the reference's
+    per-particle mixing is followed step for step."""
+    L = 30.0
+    pos = [[14.0, 15.0], [15.05, 15.2], [14.6, 16.0], [15.9, 16.3]]
+    args = (0.004, 0.012, 0.1, 1.1, 0.5, 5)
+    steps = 90
+    ref = numpy_fire(pos, L, *args, steps, per_particle=True)
+    got = oracle_fire_frames(pos, L, *args, steps)
+    for k, ((x, v, _, _, _), fr) in enumerate(zip(ref, got)):
+        assert np.allclose(fr["pos"], x, rtol=0, atol=1e-11), k
+        assert np.allclose(fr["vel"], v, rtol=0, atol=1e-11), k
+    # the mixing matters here: the paper's global mixing gives another path
+    glob = numpy_fire(pos, L, *args, steps, per_particle=False)
+    assert np.abs(glob[-1][0] - ref[-1][0]).max() > 1e-6
+    branches = [b for *_, b in ref]
+    assert "grow" in branches and "reset" in branches
+
+
+def test_fire_converges_to_lj_minimum_like_numpy():
+    """Both end at r = 2^(1/6) sigma when run long enough."""
+    L = 20.0
+    pos = [[9.55, 10.0], [10.45, 10.0]]
+    ref = numpy_fire(pos, L, 0.005, 0.05, 0.1, 1.1, 0.5, 5, 3000)
+    x = ref[-1][0]
+    assert abs(abs(x[1, 0] - x[0, 0]) - 2.0 ** (1.0 / 6.0)) < 1e-6
+    f = np.zeros(2, O.PARTICLE_DTYPE)
+    f["pos"] = pos
+    o = O.OracleSim(O.make_params(L, L, force_mode=0, eps_ab=1.0), f)
+    res, _, _ = o.fire(0.005, 0.05, 0.1, 1.1, 0.5, 5, max_steps=3000, force_tol=1e-9)
+    fr = o.frame()
+    assert res.converged == 1
+    assert abs(abs(fr["pos"][1, 0] - fr["pos"][0, 0]) - 2.0 ** (1.0 / 6.0)) < 1e-8
+
+
+# ---------------------------------------------------------------------------
+# conserved energies of NVT and NPH
+# ---------------------------------------------------------------------------
+
+def nvt_H_drift(dt, f, Lx, Ly, Q, T0, t_total):
+    o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, eps_ab=0.5, dt=dt, T_target=T0, Q=Q), f)
+    o.compute_forces()
+    g = 2 * len(f) - 2
+
+    def H():
+        ob = o.obs()
+        return o.K() + o.U() + 0.5 * Q * ob.xi ** 2 + g * T0 * ob.eta
+
+    H0, d = H(), 0.0
+    n = int(round(t_total / dt))
+    for _ in range(10):
+        o.step_nvt(n // 10)
+        d = max(d, abs(H() - H0))
+    o.close()
+    return d / len(f)
+
+
+def test_nose_hoover_conserved_energy_drift_is_first_order():
+    f, Lx, Ly = lattice_frame(256, 0.8, seed=5, jitter=0.05, T=0.7)
+    for Q in (10.0, 1.0):
+        d = [nvt_H_drift(dt, f, Lx, Ly, Q, 1.0, 1.0) for dt in (0.004, 0.002, 0.001)]
+        r1, r2 = d[0] / d[1], d[1] / d[2]
+        assert 1.7 < r1 < 2.3 and 1.7 < r2 < 2.3, (Q, d)
+        assert d[2] < 5e-3, (Q, d)
+
+
+def nph_H_drift(dt, f, Lx, Ly, W, p_ext, t_total):
+    o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, eps_ab=0.5, dt=dt, barostat_mass=W), f)
+    o.compute_forces()
+
+    def H():
+        ob = o.obs()
+        return o.K() + o.U() + ob.p_A ** 2 / (2.0 * W) + p_ext * ob.Lx * ob.Ly
+
+    H0, d = H(), 0.0
+    n = int(round(t_total / dt))
+    for _ in range(10):
+        o.step_nph(n // 10, p_ext)
+        d = max(d, abs(H() - H0))
+    o.close()
+    return d / len(f)
+
+
+def test_nph_conserved_energy_drift_is_second_order():
+    f, Lx, Ly = lattice_frame(256, 0.8, seed=5, jitter=0.05, T=0.7)
+    d = [nph_H_drift(dt, f, Lx, Ly, 50.0, 2.0, 1.0) for dt in (0.004, 0.002, 0.001)]
+    r1, r2 = d[0] / d[1], d[1] / d[2]
+    assert 3.5 < r1 < 4.6 and 3.5 < r2 < 4.6, d
+    assert d[2] < 1e-4, d
