@@ -1188,14 +1188,10 @@ struct CoreCtx {
     bool small_x, small_y;
 };
 
-__device__ inline CoreCtx make_core(const BatchPtrs &P, uint32_t s, const Grid &g) {
-    // read again here (volatile: not merged with the pass's earlier read of
-    // the image), so the image need not stay in registers through the pass
-    const volatile float *hi = static_cast<const volatile float *>(P.img) + 2 * static_cast<size_t>(s);
-    const volatile float *lo = reinterpret_cast<const volatile float *>(P.img_lo) + 2 * static_cast<size_t>(s);
+__device__ inline CoreCtx make_core_from(float2 hi, float2 lo, const Grid &g) {
     CoreCtx c;
-    c.px = static_cast<double>(hi[0]) + static_cast<double>(lo[0]);   // exact in FP64
-    c.py = static_cast<double>(hi[1]) + static_cast<double>(lo[1]);
+    c.px = static_cast<double>(hi.x) + static_cast<double>(lo.x);   // exact in FP64
+    c.py = static_cast<double>(hi.y) + static_cast<double>(lo.y);
     c.w = g.w;
     c.h = g.h;
     c.Lx = g.Lx;
@@ -1205,12 +1201,23 @@ __device__ inline CoreCtx make_core(const BatchPtrs &P, uint32_t s, const Grid &
     return c;
 }
 
+__device__ inline CoreCtx make_core(const BatchPtrs &P, uint32_t s, const Grid &g) {
+    return make_core_from(static_cast<const float2 *>(P.img)[s], P.img_lo[s], g);
+}
+
 // FP64 pair constants of one type pair (PairTable index).
 struct CoreConst {
     double sig2, eps24, eps4, vc;
 };
 __device__ inline CoreConst core_const(const PairTable &pt, int pidx) {
     return CoreConst{pt.sig2[pidx], pt.eps24[pidx], pt.eps4[pidx], pt.vc[pidx]};
+}
+// The same for a per-lane class ci in {0, 1, 2}, by selects between the
+// table's three (wave-uniform) entries rather than a per-lane load.
+__device__ inline CoreConst core_const_sel(const PairTable &pt, int ci) {
+    const CoreConst a = core_const(pt, 0), b = core_const(pt, 1), c = core_const(pt, 2);
+    return CoreConst{ci == 0 ? a.sig2 : (ci == 1 ? b.sig2 : c.sig2), ci == 0 ? a.eps24 : (ci == 1 ? b.eps24 : c.eps24),
+                     ci == 0 ? a.eps4 : (ci == 1 ? b.eps4 : c.eps4), ci == 0 ? a.vc : (ci == 1 ? b.vc : c.vc)};
 }
 __device__ inline CoreConst core_select(bool same, const CoreConst &a, const CoreConst &b) {
     return CoreConst{same ? a.sig2 : b.sig2, same ? a.eps24 : b.eps24, same ? a.eps4 : b.eps4,
@@ -1248,12 +1255,17 @@ __device__ inline void pair_core64(double dx, double dy, const CoreConst &k, dou
 // FP64 displacement of a core pair: the particle's FP64 position moved by
 // minus the partner cell's offset (sx, sy) = (ox w, oy h), minus the partner's
 // FP64 position (image q plus remainder ql), as the FP32 pass forms it in FP32.
+// MIC: grids with cells narrower than the cutoff take the per-pair minimum
+// image there too (never in list mode, whose cells are at least r_list wide).
+template <bool MIC = true>
 __device__ inline void core_disp(const CoreCtx &c, float2 q, float2 ql, double sx, double sy, double &dx,
                                  double &dy) {
     dx = (c.px - sx) - (static_cast<double>(q.x) + static_cast<double>(ql.x));
     dy = (c.py - sy) - (static_cast<double>(q.y) + static_cast<double>(ql.y));
-    if (c.small_x) dx -= c.Lx * rint(dx / c.Lx);
-    if (c.small_y) dy -= c.Ly * rint(dy / c.Ly);
+    if constexpr (MIC) {
+        if (c.small_x) dx -= c.Lx * rint(dx / c.Lx);
+        if (c.small_y) dy -= c.Ly * rint(dy / c.Ly);
+    }
 }
 
 // Core pair of the particle with the partner at cell-ordered slot k, whose
@@ -1307,11 +1319,20 @@ __device__ __forceinline__ void pair_cell_f32(const BatchPtrs &P, uint32_t k0, u
                 const float rco = UNI ? pc.rcore2s : (same ? pc.rcore2s : pc.rcore2o);
                 if (rr < rco) {
                     fp32 = false;
-                    if (LPP == 1 && qn < kCoreQ) {
-                        cq[qn * kBlock + threadIdx.x] = make_uint2(k, static_cast<uint32_t>((ox + 2) + 5 * (oy + 2)));
-                        ++qn;
+                    if constexpr (LPP == 1) {
+                        if (qn < kCoreQ) {
+                            cq[qn * kBlock + threadIdx.x] =
+                                make_uint2(k, static_cast<uint32_t>((ox + 2) + 5 * (oy + 2)));
+                            ++qn;
+                        } else {   // more than kCoreQ core partners (overlapping starts only): FP64
+                                   // arithmetic on the FP32 displacement
+                            double fx, fy;
+                            pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), pair_index(ci, cj), pt,
+                                           fx, fy, ob, want_obs);
+                            F.addd(fx, fy);
+                        }
                     } else {
-                        core_pair<OBS>(P, cc, k, ox * cc.w, oy * cc.h, core_const(pt, pair_index(ci, cj)), F.gx,
+                        core_pair<OBS>(P, cc, k, ox * cc.w, oy * cc.h, core_const_sel(pt, pair_index(ci, cj)), F.gx,
                                        F.gy, ob, want_obs);
                     }
                 }
@@ -1364,8 +1385,8 @@ __device__ inline void cell_core_queue(const BatchPtrs &P, uint32_t s, int ci, c
     for (int j = 0; j < qn; ++j) {
         const uint2 e = cq[j * kBlock + threadIdx.x];
         const int ox = static_cast<int>(e.y % 5u) - 2, oy = static_cast<int>(e.y / 5u) - 2;
-        core_pair<OBS>(P, cc, e.x, ox * cc.w, oy * cc.h, core_const(pt, pair_index(ci, P.icls[e.x])), gx, gy, ob,
-                       want_obs);
+        core_pair<OBS>(P, cc, e.x, ox * cc.w, oy * cc.h, core_const_sel(pt, pair_index(ci, P.icls[e.x])), gx, gy,
+                       ob, want_obs);
     }
 }
 
@@ -1536,9 +1557,14 @@ __device__ inline void list_core_rows(const BatchPtrs &P, uint32_t s, int ci, co
                                       uint32_t m1, uint32_t n1, uint32_t m2, uint32_t n2, double &gx, double &gy,
                                       ObsSum &ob, bool want_obs) {
     if (!__any((m1 | m2) != 0u)) return;
-    constexpr int B = 2;
-    const CoreCtx cc = make_core(P, s, g);
-    const CoreConst ks = core_const(pt, ci), ko = core_const(pt, 2);
+    constexpr int B = 1;
+    // the particle's image is loaded again here: the index passes through an
+    // empty asm, so the compiler cannot reuse the pass's first load of it and
+    // keep the image in registers through the list loop
+    uint32_t so = s;
+    __asm__ __volatile__("" : "+v"(so));
+    const CoreCtx cc = make_core_from(static_cast<const float2 *>(P.img)[so], P.img_lo[so], g);
+    const CoreConst ks = core_const_sel(pt, ci), ko = core_const(pt, 2);
     const size_t stride = static_cast<size_t>(P.N);
     const float2 *img = static_cast<const float2 *>(P.img);
     while (__any((m1 | m2) != 0u)) {
@@ -1564,12 +1590,122 @@ __device__ inline void list_core_rows(const BatchPtrs &P, uint32_t s, int ci, co
         for (int j = 0; j < B; ++j) {
             if (ok[j]) {
                 double dx, dy;
-                core_disp(cc, q[j], ql[j], static_cast<double>(list_ox(v[j])) * cc.w,
-                          static_cast<double>(list_oy(v[j])) * cc.h, dx, dy);
+                const uint32_t c = list_code(v[j]);
+                const int oy1 = static_cast<int>((c * 11u) >> 5);
+                const int ox1 = static_cast<int>(c) - 3 * oy1;
+                core_disp<false>(cc, q[j], ql[j], static_cast<double>(ox1 - 1) * cc.w,
+                                 static_cast<double>(oy1 - 1) * cc.h, dx, dy);
                 pair_core64<OBS>(dx, dy, core_select((v[j] & kListSame) != 0u, ks, ko), gx, gy, ob, want_obs);
             }
         }
     }
+}
+
+// Core rows of one particle's list left for the caller (list_rows_f32 masks).
+struct CoreRows {
+    uint32_t m1, n1, m2, n2;
+};
+
+// Rows of a workgroup's core queue (wg_core_rows).
+constexpr int kCoreWG = 704;
+
+// Core rows of the workgroup's lists, shared out over all its lanes (one lane
+// per particle, no energy sums).  Each lane's list pass marked its core rows
+// below 32 in m1 (rows at and past 32, long lists only, in m2); a lane with
+// many of them would otherwise keep its wave, and the workgroup, waiting while
+// the other lanes idle.  After the list loops (a barrier) the workgroup's
+// [9][kBlock] LDS table, no longer needed, holds every lane's FP64 position
+// (image and remainder), a queue of (lane, row, class) entries in lane and
+// then row order, and one FP64 force per entry: lane t evaluates entries t,
+// t + kBlock, ..., and each lane then adds its own entries' forces in row
+// order, so the sums are those the lane alone would form.  A queue longer than
+// kCoreWG entries (dense overlapping starts) falls back to each lane's own
+// evaluation.  Every thread of the workgroup must call this (barriers); s0 is
+// the workgroup's first cell-ordered slot, active whether this thread has a
+// particle.
+__device__ inline void wg_core_rows(const BatchPtrs &P, uint32_t s0, bool active, int ci, const Grid &g,
+                                    const PairTable &pt, const CoreRows &cr, double &gx, double &gy,
+                                    float2 (*tab)[kBlock], uint32_t *wsum) {
+    const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, wid = tid >> 6;
+    const uint32_t s = s0 + static_cast<uint32_t>(tid);
+    const uint32_t m1 = active ? cr.m1 : 0u;
+    // the own image is read from the table (offset (0, 0)) before the barrier
+    // after which the table is overwritten
+    const float2 hi = active ? tab[4][tid] : make_float2(0.0f, 0.0f);
+    const float2 lo = active ? P.img_lo[s] : make_float2(0.0f, 0.0f);
+    ObsSum ob_unused;
+    if (!__syncthreads_or(m1 != 0u)) {
+        if (active) list_core_rows<false>(P, s, ci, g, pt, 0u, 0u, cr.m2, cr.n2, gx, gy, ob_unused, false);
+        return;
+    }
+    // exclusive scan of the lanes' entry counts over the workgroup
+    const uint32_t cnt = static_cast<uint32_t>(__popc(m1));
+    uint32_t inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    uint32_t base = inc - cnt, total = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const uint32_t t = wsum[w];
+        base += w < wid ? t : 0u;
+        total += t;
+    }
+    if (total > static_cast<uint32_t>(kCoreWG)) {   // the same decision in every lane
+        if (active) list_core_rows<false>(P, s, ci, g, pt, m1, cr.n1, cr.m2, cr.n2, gx, gy, ob_unused, false);
+        return;
+    }
+    char *lds = reinterpret_cast<char *>(&tab[0][0]);
+    float4 *own = reinterpret_cast<float4 *>(lds);
+    uint32_t *ent = reinterpret_cast<uint32_t *>(lds + kBlock * sizeof(float4));
+    double2 *res = reinterpret_cast<double2 *>(lds + kBlock * sizeof(float4) + kCoreWG * sizeof(uint32_t));
+    static_assert(kBlock * sizeof(float4) + kCoreWG * (sizeof(uint32_t) + sizeof(double2)) <=
+                      9 * kBlock * sizeof(float2),
+                  "core queue fits the table");
+    own[tid] = make_float4(hi.x, hi.y, lo.x, lo.y);
+    {
+        uint32_t m = m1, j = base;
+        while (m != 0u) {
+            const uint32_t pos = 31u - static_cast<uint32_t>(__builtin_clz(m));
+            m ^= 1u << pos;
+            const uint32_t e = cr.n1 - 1u - pos;
+            ent[j++] = (static_cast<uint32_t>(tid) << 8) | (e << 2) | static_cast<uint32_t>(ci);
+        }
+    }
+    __syncthreads();
+    const float2 *img = static_cast<const float2 *>(P.img);
+    const size_t stride = static_cast<size_t>(P.N);
+    for (uint32_t k = static_cast<uint32_t>(tid); k < total; k += kBlock) {
+        const uint32_t q = ent[k];
+        const uint32_t o = q >> 8, e = (q >> 2) & 63u, co = q & 3u;
+        const uint32_t v = P.nlist[static_cast<size_t>(e) * stride + s0 + o];
+        const uint32_t kp = (v & kListOff) >> 3;
+        const float2 qh = img[kp], ql = P.img_lo[kp];
+        const float4 ow = own[o];
+        const CoreCtx cc = make_core_from(make_float2(ow.x, ow.y), make_float2(ow.z, ow.w), g);
+        const uint32_t c = list_code(v);
+        const int oy1 = static_cast<int>((c * 11u) >> 5);
+        const int ox1 = static_cast<int>(c) - 3 * oy1;
+        double dx, dy;
+        core_disp<false>(cc, qh, ql, static_cast<double>(ox1 - 1) * cc.w, static_cast<double>(oy1 - 1) * cc.h, dx,
+                         dy);
+        double fx = 0.0, fy = 0.0;
+        const int pidx = (v & kListSame) ? static_cast<int>(co) : 2;
+        pair_core64<false>(dx, dy, CoreConst{pt.sig2[pidx], pt.eps24[pidx], 0.0, 0.0}, fx, fy, ob_unused, false);
+        res[k] = make_double2(fx, fy);
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (uint32_t j = base; j < base + cnt; ++j) {
+        const double2 r = res[j];
+        gx += r.x;
+        gy += r.y;
+    }
+    if (active) list_core_rows<false>(P, s, ci, g, pt, 0u, 0u, cr.m2, cr.n2, gx, gy, ob_unused, false);
 }
 
 // FP32 pairs of cell-ordered slot s from its neighbour list (list mode):
@@ -1585,7 +1721,7 @@ template <int ACC, bool OBS, bool UNI, int LPP>
 __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, float2 pi, int ci, const Grid &g,
                                               const PairTable &pt, const PairF32 &pc, ForceSum<ACC> &F,
                                               ObsSum &ob, bool want_obs, float2 (*shift)[kBlock],
-                                              uint32_t *stash) {
+                                              uint32_t *stash, CoreRows *defer) {
     constexpr int U = 4;
     float fxc = 0.0f, fyc = 0.0f;
     double gx = 0.0, gy = 0.0;
@@ -1612,9 +1748,15 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, fl
                 stash[threadIdx.x] = m1;
                 list_rows_f32<ACC, OBS, UNI, 2>(P, s, 32u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy,
                                                 m2);
-                m1 = static_cast<const volatile uint32_t *>(stash)[threadIdx.x];
+                uint32_t t = threadIdx.x;
+                __asm__ __volatile__("" : "+v"(t));   // a fresh LDS read, not m1 kept in a register
+                m1 = stash[t];
             }
-            list_core_rows<OBS>(P, s, ci, g, pt, m1, n1, m2, n > 32u ? n - 32u : 0u, gx, gy, ob, want_obs);
+            if (defer) {   // the caller evaluates them with the whole workgroup
+                *defer = CoreRows{m1, n1, m2, n > 32u ? n - 32u : 0u};
+            } else {
+                list_core_rows<OBS>(P, s, ci, g, pt, m1, n1, m2, n > 32u ? n - 32u : 0u, gx, gy, ob, want_obs);
+            }
         } else {
             if (nr & kListGuard)
                 list_rows_f32<ACC, OBS, UNI, 1>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy, m1);
@@ -1624,6 +1766,7 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, fl
     } else {
         (void)shift;
         (void)stash;
+        (void)defer;
         const char *imgb = static_cast<const char *>(P.img);
         const size_t stride = static_cast<size_t>(P.N);
         const uint32_t n = P.ncount[s] & ~kListGuard;
@@ -1772,6 +1915,11 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
     // images, or the cell pass's queue of core pairs (kCoreQ uint2 per lane)
     __shared__ float2 lds_tab[(LPP == 1 && !kFp64) ? 9 : 1][kBlock];
     __shared__ uint32_t lds_stash[(LPP == 1 && !kFp64) ? kBlock : 1];
+    __shared__ uint32_t lds_wsum[kWaves];
+    // the list pass of the compensated form (one lane per particle, no energy
+    // sums) leaves its core rows to the whole workgroup (wg_core_rows)
+    constexpr bool kWgCore = !kFp64 && ACC == ACC_COMP32 && LPP == 1 && !OBS;
+    CoreRows cr{0u, 0u, 0u, 0u};
     static_assert(9 * sizeof(float2) >= kCoreQ * sizeof(uint2), "core queue fits the shift table");
     uint2 *cq = reinterpret_cast<uint2 *>(&lds_tab[0][0]);
     int qn = 0;
@@ -1833,7 +1981,8 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
                 }
                 by_list = true;
             } else if (sc.list_on && !sc.list_overflow) {
-                pair_list_f32<ACC, OBS, UNI, LPP>(P, s, pi, ci, g, pt, pc, F, ob, want_obs, lds_tab, lds_stash);
+                pair_list_f32<ACC, OBS, UNI, LPP>(P, s, pi, ci, g, pt, pc, F, ob, want_obs, lds_tab, lds_stash,
+                                                  kWgCore ? &cr : nullptr);
                 by_list = true;
             }
         }
@@ -1878,6 +2027,17 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
         if constexpr (!kFp64 && ACC == ACC_COMP32 && LPP == 1)
             cell_core_queue<OBS>(P, s, ci, g, pt, cq, qn, F.gx, F.gy, ob, want_obs);
         if constexpr (LPP > 1) combine_lanes<LPP, kFp64 ? ACC_FP64 : ACC>(F, ob);
+    }
+    if constexpr (kWgCore) {
+        // the same condition as the list branch above, for every thread of
+        // the workgroup (a workgroup belongs to one slot)
+        const bool wg_list =
+            !(LEAN && (sc.fallback[parity] || !sc.list_on || !sc.list_built || sc.list_overflow)) && sc.list_on &&
+            !sc.list_overflow;
+        if (wg_list) {
+            const uint32_t s0 = P.cs[sp.coff] + static_cast<uint32_t>((gb - sp.boff) * kBlock);
+            wg_core_rows(P, s0, li < sp.n, ci, g, pt, cr, F.gx, F.gy, lds_tab, lds_wsum);
+        }
     }
     if (li < sp.n && lead) {
         const bool kicks = pmode == PM_NVE || pmode == PM_NPH || pmode == PM_PISTON || pmode == PM_NVT ||
