@@ -198,6 +198,16 @@ int mdx_sync(mdx_sim *sim);
  * 2 n_bins values. */
 int mdx_N_profile(mdx_sim *sim, int32_t n_bins, int32_t *counts);
 
+/* No counterpart in the reference.  mdx_N_profile for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving or
+ * sampling loops, harnesses that forbid hipMalloc): the same counts, kept in a
+ * fixed device array of 2 x 8192 int32 counters per GPU instead of a buffer
+ * allocated per call, so the call allocates nothing.  n_bins in [1, 8192]
+ * (MDX_ERR_CAPACITY beyond).  Every call on one GPU shares that array, so
+ * calls that overlap on the same GPU (from several host threads) race; the
+ * call synchronises the handle's stream before returning. */
+int mdx_N_profile_noalloc(mdx_sim *sim, int32_t n_bins, int32_t *counts);
+
 int mdx_cell_list(mdx_sim *sim, int32_t periodic_y, int32_t *ncx, int32_t *ncy,
                   uint32_t *keys, uint32_t *cell_start, uint32_t *sorted,
                   int64_t cap_n, int64_t cap_cells);

@@ -203,6 +203,7 @@ _SIGNATURES = {
     "mdx_batch_timing_read": (C.c_int, [_P, C.POINTER(Timing)]),
     "mdx_sync": (C.c_int, [_P]),
     "mdx_N_profile": (C.c_int, [_P, C.c_int32, _P]),
+    "mdx_N_profile_noalloc": (C.c_int, [_P, C.c_int32, _P]),
     "mdx_cell_list": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P, _P, _P,
                                 C.c_int64, C.c_int64]),
     "mdx_timing_enable": (C.c_int, [_P, C.c_int32]),

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace mdx {
 
@@ -1131,6 +1132,69 @@ int Batch::n_profile(int r, int nb, int32_t *out) {
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     (void)hipFree(d);
     MDX_HIP(e);
+    return MDX_OK;
+}
+
+// ---- allocation-free N profile ------------------------------------------
+// The counters of mdx_N_profile_noalloc: one fixed device array per GPU (each
+// device that runs the library holds its own copy until the process ends),
+// shared by every call on that device.  Host code never passes the symbol
+// itself to a kernel or a copy: the device address comes from
+// hipGetSymbolAddress, once per device, and is kept in scratch_ptr.
+static __device__ __align__(256) int32_t scratch[2 * kProfileMaxBins];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch array (the device must be current).
+int scratch_ptr(int dev, int32_t **out) {
+    if (dev < 0 || dev >= kMaxDevices) return set_error(MDX_ERR_INVALID, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(MDX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<int32_t *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return MDX_OK;
+}
+
+__global__ void __launch_bounds__(kBlock) k_zero_i32(int32_t *p, int32_t n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+}  // namespace
+
+// n_profile with the counters in the device's scratch array instead of a
+// buffer allocated per call: zeroed by a kernel (no memset node), filled by
+// k_n_profile, copied to out.  Calls that overlap on one device race on the
+// array.
+int Batch::n_profile_noalloc(int r, int nb, int32_t *out) {
+    if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
+        return set_error(MDX_ERR_STATE, "slot not loaded");
+    if (nb < 1 || nb > kProfileMaxBins)
+        return set_error(MDX_ERR_CAPACITY, "n_bins must be in [1, " + std::to_string(kProfileMaxBins) +
+                                               "] (the fixed scratch array holds 2 x " +
+                                               std::to_string(kProfileMaxBins) + " counters)");
+    if (!out) return set_error(MDX_ERR_INVALID, "null output");
+    MDX_HIP(hipSetDevice(device));
+    int32_t *d = nullptr;
+    MDX_TRY(scratch_ptr(device, &d));
+    const int n2 = 2 * nb;
+    k_zero_i32<<<ceil_div(n2, kBlock), kBlock, 0, stream>>>(d, n2);
+    MDX_HIP(hipGetLastError());
+    const int64_t n = slots[r].n;
+    if (n > 0) {
+        k_n_profile<<<static_cast<int>((n + kBlock - 1) / kBlock), kBlock, static_cast<size_t>(n2) * sizeof(int32_t),
+                      stream>>>(ptrs, r, nb, d);
+        MDX_HIP(hipGetLastError());
+    }
+    MDX_HIP(hipMemcpyAsync(out, d, static_cast<size_t>(n2) * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    MDX_HIP(hipStreamSynchronize(stream));
     return MDX_OK;
 }
 

@@ -152,6 +152,7 @@ struct Batch {
     int replay(bool obs, int mode, int count, int parity = 0);
     int capture_steps(hipGraphExec_t &ge, bool obs, int mode, int parity, int steps);
     int n_profile(int r, int nb, int32_t *out);
+    int n_profile_noalloc(int r, int nb, int32_t *out);   // counters in the device's fixed scratch array
     int run_windows(int count);
     bool windows_apply();
     int pre_step();
