@@ -53,13 +53,15 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-vs-FP64 error report")
     ap.add_argument("--workload", default="nvt1m",
-                    choices=["nvt1m", "ensemble", "serve", "fire_nph", "nve4k", "long"],
+                    choices=["nvt1m", "ensemble", "serve", "fire_nph", "nve4k", "long", "dropin"],
                     help="nvt1m: N=1M Nose-Hoover (default line); ensemble: 256 NVT replicas of uneven N "
                          "in one batch vs per-replica launches; serve: Poisson job stream through the "
                          "continuous-batching server; fire_nph: FIRE from a random start at N=262144 "
                          "then NPH; nve4k: NVE N=4096 with the FP64 oracle timed beside it; long: the "
                          "N=1M NVT step timed after up to --long-steps steps, with and without the "
-                         "storage reorder")
+                         "storage reorder; dropin: the N=1M NVT workload driven through the C++ "
+                         "MDSimulation class one step_single_nose_hoover() per step, as the reference's "
+                         "drivers step, beside one mdx_step call of the same steps")
     ap.add_argument("--long-steps", type=int, default=60000)
     ap.add_argument("--tune-steps", type=int, default=40,
                     help="steps per autotuner candidate before the timed run (0: default tuning)")
@@ -680,6 +682,67 @@ def run_nve4k(args):
     print(json.dumps(out), flush=True)
 
 
+def run_dropin(args):
+    """The N=1M NVT workload through the C++ MDSimulation class
+    (tests/cpp/dropin_nvt.cpp): one step_single_nose_hoover() per loop
+    iteration, the reference's driver loop (run/series_partial_U_lambda_test.cpp
+    :207-219), without and with the record-interval getters every 1000 steps;
+    beside it the same steps as one mdx_step call with the same tuning."""
+    import subprocess
+    from md_simulation_cuda_amd import abi
+
+    exe = ROOT / "md_simulation_cuda_amd" / "_build" / "cpp_tests" / "dropin_nvt"
+    if not exe.exists():
+        raise RuntimeError(f"{exe} not built; run python -m md_simulation_cuda_amd.build")
+    n, steps = args.n, args.steps
+    warm = max(args.warmup, 64)
+    env = dict(os.environ)
+    if args.tuning:
+        env["MDX_TUNING"] = args.tuning
+    runs = {}
+    for record in (0, 1000):
+        r = subprocess.run([str(exe), str(n), str(warm), str(steps), str(record)], capture_output=True, text=True,
+                           env=env, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"dropin_nvt failed ({r.returncode}): {r.stderr[-2000:]}")
+        runs[record] = json.loads(r.stdout.strip().splitlines()[-1])
+    tu = runs[0]["tuning"]
+    frame, Lx, Ly, kw = make_workload(n)
+    sim = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), frame)
+    sim.set_tuning(tu["cell_scale"], tu["lanes_per_particle"], tu["skin"], tu["window"])
+    sim.step(abi.NVT, warm)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(abi.NVT, steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    sim.close()
+    one_call = n * steps / wall
+    value = runs[0]["atom_steps_per_s"]
+    out = {
+        "metric": "atom-steps/s @N=1M NVT, drop-in MDSimulation loop (one step_single_nose_hoover per step)",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": runs[0]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair + fp32 two-sum accumulation, fp64 state",
+        "data": "synthetic (reference init_particles, phase-separated, seeded)",
+        "config": {"workload": "NVT Nose-Hoover, 2D binary LJ, N=1048576, 2:1 box rho=0.75, T=0.6, Q=10, "
+                               "dt=0.005, via the C++ MDSimulation class", "N": n},
+        "tuning": tu,
+        "with_records_every_1000_steps": runs[1000],
+        "one_mdx_step_call": {"value": one_call, "ms_per_step": wall * 1e3 / steps},
+        "dropin_over_one_call": value / one_call,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -703,6 +766,9 @@ def main():
         return
     if args.workload == "long":
         run_long(args)
+        return
+    if args.workload == "dropin":
+        run_dropin(args)
         return
     from md_simulation_cuda_amd import abi
 

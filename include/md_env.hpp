@@ -46,6 +46,7 @@ public:
     double get_Lx() const { return cfg_manager.config.box_w_global; }
     double get_Ly() const { return cfg_manager.config.box_h_global; }
 
+    void middle_reflect_LG();   // src/md_env.cu:1457-1508
     void step_single();   // README name of step_single_NVE
     void step_single_NVE();
     void step_single_nose_hoover(bool do_middle_wrap = false);
@@ -62,6 +63,13 @@ public:
     // Extensions of this library.
     void step_many(int kind, int nsteps, double pressure_ext = 0.0, bool do_middle_wrap = false);
     void set_precision(int precision);   // mdx_precision
+    // Tuning of the pair pass (mdx_tuning).  By default (environment variable
+    // MDX_TUNING unset or "auto") the first NVE or Nose-Hoover step runs the
+    // autotuner on a copy of the system and keeps its choice; MDX_TUNING="off"
+    // keeps the cell list every step; "cell_scale,lanes,skin,window" sets a
+    // tuning.  Results differ between tunings only by FP32 summation order.
+    void set_tuning(double cell_scale, int lanes_per_particle, double skin, int window);
+    void autotune(int kind, int steps_per_candidate = 20);
     mdx_sim *handle() { return sim_; }
     const MDConfigManager &config() const { return cfg_manager; }
 
@@ -79,4 +87,5 @@ private:
     double nph_piston_velocity = 0.0;
     double nph_piston_force = 0.0;
     int precision_ = MDX_PREC_COMP32;
+    bool autotune_pending_ = false;   // MDX_TUNING auto: tune at the first NVE / NVT step
 };

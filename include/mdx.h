@@ -147,8 +147,19 @@ int mdx_set_state(mdx_sim *sim, const mdx_particle64 *frame, int64_t n);
 /* ---- stepping ---------------------------------------------------------- */
 
 /* nsteps steps of one ensemble (mdx_ensemble); p_ext is the external
- * pressure for MDX_NPH / MDX_PISTON. */
+ * pressure for MDX_NPH / MDX_PISTON.  With rebuild windows (mdx_tuning) a call
+ * continues the window the previous call of the same kind left open, so the
+ * split into calls does not change the trajectory, and a call does not wait
+ * for the GPU; a window violation is then reported by the call itself when it
+ * ran at least 64 steps, else by the next mdx_sync, mdx_observables or
+ * mdx_collect.  Passes at the current positions (observables, forces) and
+ * changes of kind, tuning or state start a new window. */
 int mdx_step(mdx_sim *sim, int32_t kind, int32_t nsteps, double p_ext, uint32_t flags);
+
+/* MDSimulation::middle_reflect_LG (include/md_env.hpp:78, src/md_env.cu:
+ * 1457-1508): y wrapped, x reflected into the central slab of width
+ * devide_p * Lx + 2 sigma_max around Lx / 2; no force pass. */
+int mdx_middle_reflect(mdx_sim *sim);
 
 /* FIRE minimisation of the handle's current frame
  * (EnergyMinimizer::minimize_frame, src/md_energy_minimizer.cu:122-272).

@@ -204,6 +204,7 @@ _SIGNATURES = {
     "mdx_sync": (C.c_int, [_P]),
     "mdx_N_profile": (C.c_int, [_P, C.c_int32, _P]),
     "mdx_N_profile_noalloc": (C.c_int, [_P, C.c_int32, _P]),
+    "mdx_middle_reflect": (C.c_int, [_P]),
     "mdx_cell_list": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P, _P, _P,
                                 C.c_int64, C.c_int64]),
     "mdx_timing_enable": (C.c_int, [_P, C.c_int32]),
@@ -367,6 +368,9 @@ class Simulation:
         out = np.zeros(2 * n_bins, np.int32)
         check(self._lib.mdx_N_profile(self._h, n_bins, _ptr(out)))
         return out
+
+    def middle_reflect(self) -> None:
+        check(self._lib.mdx_middle_reflect(self._h))
 
     def cell_list(self, periodic_y: bool = True):
         ncx, ncy = C.c_int32(), C.c_int32()

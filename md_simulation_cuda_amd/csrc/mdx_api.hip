@@ -247,6 +247,11 @@ int mdx_N_profile(mdx_sim *sim, int32_t n_bins, int32_t *counts) {
     return sim->b.n_profile(0, n_bins, counts);
 }
 
+int mdx_middle_reflect(mdx_sim *sim) {
+    CHECK_HANDLE(sim);
+    return sim->b.middle_reflect(0);
+}
+
 int mdx_N_profile_noalloc(mdx_sim *sim, int32_t n_bins, int32_t *counts) {
     CHECK_HANDLE(sim);
     return sim->b.n_profile_noalloc(0, n_bins, counts);

@@ -243,6 +243,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     if (!cb2s.empty()) MDX_HIP(hipMemcpy(d_cblk2sys, cb2s.data(), cb2s.size() * 4, hipMemcpyHostToDevice));
     if (!lb2s.empty()) MDX_HIP(hipMemcpy(d_lb2sys, lb2s.data(), lb2s.size() * 4, hipMemcpyHostToDevice));
     MDX_HIP(hipMemcpy(d_sp, h_sp.data(), nslots * sizeof(SysParams), hipMemcpyHostToDevice));
+    sp_dev = h_sp;
     ptrs.x = x;
     ptrs.v = v;
     ptrs.a = a;
@@ -280,30 +281,71 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     return MDX_OK;
 }
 
+// SysParams go to the device only when they changed (a run of short step
+// calls uploads nothing); pending pair partials are combined first, under
+// the pair mode they were produced with.
 int Batch::upload_sp(int r) {
+    if (sp_dev.size() == h_sp.size() && std::memcmp(&sp_dev[r], &h_sp[r], sizeof(SysParams)) == 0) return MDX_OK;
+    MDX_TRY(flush());
     MDX_HIP(hipMemcpyAsync(d_sp + r, &h_sp[r], sizeof(SysParams), hipMemcpyHostToDevice, stream));
+    if (sp_dev.size() == h_sp.size()) sp_dev[r] = h_sp[r];
     return MDX_OK;
 }
 
 int Batch::upload_all_sp() {
+    if (sp_dev.size() == h_sp.size() &&
+        std::memcmp(sp_dev.data(), h_sp.data(), h_sp.size() * sizeof(SysParams)) == 0)
+        return MDX_OK;
+    MDX_TRY(flush());
     MDX_HIP(hipMemcpyAsync(d_sp, h_sp.data(), h_sp.size() * sizeof(SysParams), hipMemcpyHostToDevice, stream));
+    sp_dev = h_sp;
+    return MDX_OK;
+}
+
+int Batch::flush() {
+    if (!fin_pending) return MDX_OK;
+    fin_pending = false;
+    k_finalize<<<static_cast<int>(slots.size()), kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
+    MDX_HIP(hipGetLastError());
     return MDX_OK;
 }
 
 int Batch::read_scalars(int r, DevScalars &h) {
+    MDX_TRY(flush());
     MDX_HIP(hipMemcpyAsync(&h, d_sc + r, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
     MDX_HIP(hipStreamSynchronize(stream));
     return MDX_OK;
 }
 
 int Batch::write_scalar_block(int r, const DevScalars &h) {
+    MDX_TRY(flush());
     MDX_HIP(hipMemcpyAsync(d_sc + r, &h, sizeof(DevScalars), hipMemcpyHostToDevice, stream));
     MDX_HIP(hipStreamSynchronize(stream));
     return MDX_OK;
 }
 
-int Batch::sync() {
+// Violation flags of the lean steps since the last check (one device-to-host
+// read of every slot's scalars); the flags are cleared.
+int Batch::check_violations() {
+    viol_unchecked = false;
+    const int S = static_cast<int>(slots.size());
+    std::vector<DevScalars> cur(S);
+    MDX_HIP(hipMemcpyAsync(cur.data(), d_sc, S * sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
     MDX_HIP(hipStreamSynchronize(stream));
+    bool bad = false;
+    for (int r = 0; r < S; ++r) bad = bad || (slots[r].loaded && cur[r].violated != 0);
+    if (!bad) return MDX_OK;
+    k_clear_violations<<<ceil_div(S, kBlock), kBlock, 0, stream>>>(ptrs);
+    MDX_HIP(hipGetLastError());
+    return set_error(MDX_ERR_STATE,
+                     "rebuild window: a particle moved beyond the reach of the lean-step fallback "
+                     "(or the grid has fewer than 5 cells per side); use a shorter window");
+}
+
+int Batch::sync() {
+    MDX_TRY(flush());
+    MDX_HIP(hipStreamSynchronize(stream));
+    if (viol_unchecked) return check_violations();
     return MDX_OK;
 }
 
@@ -507,9 +549,11 @@ int Batch::run_passes(int count_, bool obs) {
     if (!obs && windows_apply()) {
         MDX_TRY(run_windows(count_));
     } else {
+        win_pos = 0;
         MDX_TRY(replay(obs, STEP_NORMAL, count_));
         k_finalize<<<static_cast<int>(slots.size()), kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
         MDX_HIP(hipGetLastError());
+        fin_pending = false;
     }
     if (s1) {
         (void)hipEventRecord(s1, stream);
@@ -523,11 +567,21 @@ int Batch::run_passes(int count_, bool obs) {
 // window - 1 lean steps (no cell-list kernels; a slot with a particle beyond
 // skin / 2 anyway, or without a usable list, takes the 5x5 pass over its last
 // cell list, which is exact while every particle stays within (2 w - rc_max)
-// / 2 of that build).  No host synchronisation inside the run; at its end the
-// slots are checked for a lean step that pass could not cover.
+// / 2 of that build).  Windows run on across calls: a call first finishes
+// the window the last call left open (win_pos), so the rebuild points and
+// results do not depend on how the steps are split into calls.  No host
+// synchronisation: the last pair pass's partials are left for the next
+// kernel that combines them (fin_pending), and the slots' violation flags are
+// read at the end of a long call (at least kWindowCheckSteps steps) or at the
+// next synchronising call (Batch::sync, observables, collect).
 int Batch::run_windows(int count_) {
-    const int S = static_cast<int>(slots.size());
-    for (int left = count_; left > 0;) {
+    int left = count_;
+    while (win_pos > 0 && left > 0) {   // the open window's remaining lean steps
+        MDX_TRY(replay(false, STEP_LEAN, 1, win_pos & 1));
+        win_pos = (win_pos + 1) % window;
+        --left;
+    }
+    while (left > 0) {
         const int w = std::min(window, left);
         if (w == window && use_graph && !timing) {
             // a whole window as one graph: one launch per window instead of
@@ -553,21 +607,11 @@ int Batch::run_windows(int count_) {
             for (int j = 1; j < w; ++j) MDX_TRY(replay(false, STEP_LEAN, 1, j & 1));
         }
         left -= w;
+        win_pos = w % window;
     }
-    k_finalize<<<S, kBlock, 0, stream>>>(ptrs, FIN_PAIR, d_want);
-    MDX_HIP(hipGetLastError());
-    std::vector<DevScalars> cur(S);
-    MDX_HIP(hipMemcpyAsync(cur.data(), d_sc, S * sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
-    MDX_HIP(hipStreamSynchronize(stream));
-    bool bad = false;
-    for (int r = 0; r < S; ++r) bad = bad || (slots[r].loaded && cur[r].violated != 0);
-    if (bad) {
-        k_clear_violations<<<ceil_div(S, kBlock), kBlock, 0, stream>>>(ptrs);
-        MDX_HIP(hipGetLastError());
-        return set_error(MDX_ERR_STATE,
-                         "rebuild window: a particle moved beyond the reach of the lean-step fallback "
-                         "(or the grid has fewer than 5 cells per side); use a shorter window");
-    }
+    fin_pending = true;
+    viol_unchecked = true;
+    if (count_ >= kWindowCheckSteps) return check_violations();
     return MDX_OK;
 }
 
@@ -578,6 +622,7 @@ int Batch::load(int r, const mdx_params &p, const mdx_particle64 *frame, int64_t
     Slot &s = slots[r];
     if (n > s.n_cap) return set_error(MDX_ERR_CAPACITY, "frame larger than the slot capacity");
     MDX_TRY(validate_params(p));
+    win_pos = 0;
     MDX_HIP(hipSetDevice(device));
     SysParams &sp = h_sp[r];
     double rc_max = 0.0;
@@ -643,6 +688,7 @@ int Batch::load(int r, const mdx_params &p, const mdx_particle64 *frame, int64_t
 int Batch::unload(int r) {
     if (r < 0 || r >= static_cast<int>(slots.size())) return set_error(MDX_ERR_INVALID, "slot out of range");
     Slot &s = slots[r];
+    win_pos = 0;
     s.loaded = false;
     s.n = 0;
     h_sp[r].n = 0;
@@ -658,6 +704,8 @@ int Batch::set_state(int r, const mdx_particle64 *frame, int64_t n) {
         return set_error(MDX_ERR_INVALID, "loaded particle count " + std::to_string(n) + " doesn't match n " +
                                               std::to_string(s.n));
     const int64_t off = h_sp[r].off;
+    win_pos = 0;
+    MDX_TRY(flush());
     std::vector<double2> hx(n), hv(n);
     std::vector<uint8_t> hc(n);
     for (int64_t i = 0; i < n; ++i) {
@@ -688,6 +736,9 @@ int Batch::set_state(int r, const mdx_particle64 *frame, int64_t n) {
 int Batch::set_kind(int r, int kind, double p_ext, uint32_t flags) {
     if (r < 0 || r >= static_cast<int>(slots.size())) return set_error(MDX_ERR_INVALID, "slot out of range");
     if (kind != SK_NONE && !slots[r].loaded) return set_error(MDX_ERR_STATE, "slot not loaded");
+    // another kind of step ends the open rebuild window (the same kind, from
+    // the next call, continues it)
+    if (slots[r].kind != kind || slots[r].p_ext != p_ext || slots[r].flags != flags) win_pos = 0;
     slots[r].kind = kind;
     slots[r].p_ext = p_ext;
     slots[r].flags = flags;
@@ -697,6 +748,8 @@ int Batch::set_kind(int r, int kind, double p_ext, uint32_t flags) {
 
 int Batch::kinetic(const std::vector<int> &which) {
     if (which.empty()) return MDX_OK;
+    MDX_TRY(flush());   // k_kinetic writes the partials (a lean step would read them)
+    win_pos = 0;
     std::fill(h_want.begin(), h_want.end(), 0);
     for (int r : which) h_want[r] = 1;
     MDX_HIP(hipMemcpyAsync(d_want, h_want.data(), h_want.size() * 4, hipMemcpyHostToDevice, stream));
@@ -711,6 +764,8 @@ int Batch::kinetic(const std::vector<int> &which) {
 
 int Batch::walls(const std::vector<int> &which) {
     if (which.empty()) return MDX_OK;
+    MDX_TRY(flush());
+    win_pos = 0;
     std::fill(h_want.begin(), h_want.end(), 0);
     for (int r : which) h_want[r] = 1;
     MDX_HIP(hipMemcpyAsync(d_want, h_want.data(), h_want.size() * 4, hipMemcpyHostToDevice, stream));
@@ -744,6 +799,7 @@ static int one_pass(Batch &b, const std::vector<int> &which, int periodic_y, int
     MDX_TRY(b.upload_all_sp());
     const bool obs = pmode == PM_OBS;
     MDX_TRY(b.run_passes(1, obs));
+    b.win_pos = 0;   // the next step starts a window (a head step) from this pass's cell list
     for (int r : which) {
         b.slots[r].cells_valid = true;
         b.slots[r].cells_py = periodic_y;
@@ -878,7 +934,11 @@ int Batch::step(int nsteps) {
             for (const Slot &s : slots)
                 if (s.loaded && s.kind != SK_NONE && !s.cells_valid && s.since_reorder >= reorder_every) chunk = 1;
         }
-        if (chunk < left && windows_apply()) chunk = std::min(left, std::max(window, chunk - chunk % window));
+        if (chunk < left && windows_apply()) {
+            // chunks end at window boundaries, where a reorder may run
+            chunk = win_pos > 0 ? std::min(left, window - win_pos)
+                                : std::min(left, std::max(window, chunk - chunk % window));
+        }
         MDX_TRY(run_passes(chunk, obs));
         left -= chunk;
         for (Slot &s : slots)
@@ -918,6 +978,7 @@ int Batch::observables(int r, uint32_t mask, mdx_obs *out) {
     if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
         return set_error(MDX_ERR_STATE, "slot not loaded");
     MDX_HIP(hipSetDevice(device));
+    if (viol_unchecked) MDX_TRY(check_violations());   // a windowed run's deferred check
     Slot &s = slots[r];
     if ((mask & (MDX_OBS_K | MDX_OBS_P)) && !s.K_valid) MDX_TRY(kinetic({r}));
     if (s.piston_active) {
@@ -979,6 +1040,7 @@ int Batch::collect(int r, mdx_particle64 *out, int64_t cap) {
     const Slot &s = slots[r];
     if (cap < s.n) return set_error(MDX_ERR_CAPACITY, "output capacity below n");
     MDX_HIP(hipSetDevice(device));
+    if (viol_unchecked) MDX_TRY(check_violations());   // a windowed run's deferred check
     const int64_t n = s.n, off = h_sp[r].off;
     std::vector<double2> hx(n), hv(n);
     std::vector<unsigned char> ha(n * 16);
@@ -1135,6 +1197,30 @@ int Batch::n_profile(int r, int nb, int32_t *out) {
     return MDX_OK;
 }
 
+// middle_reflect_LG on slot r's current positions (no force pass, as in the
+// reference); the next step rebuilds the cell list from the moved particles.
+int Batch::middle_reflect(int r) {
+    if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
+        return set_error(MDX_ERR_STATE, "slot not loaded");
+    MDX_HIP(hipSetDevice(device));
+    Slot &s = slots[r];
+    DevScalars h;
+    MDX_TRY(read_scalars(r, h));
+    const double Lx = h.g.Lx, Ly = h.g.Ly, dp = s.p.devide_p;
+    if (Lx <= 0.0 || Ly <= 0.0 || dp <= 0.0) return MDX_OK;   // src/md_env.cu:1462-1464
+    const double smax = std::max(s.p.sigma_aa, std::max(s.p.sigma_bb, s.p.sigma_ab));
+    double slab = dp * Lx + 2.0 * smax;
+    if (slab > Lx) slab = Lx;
+    if (s.n > 0) {
+        k_middle_reflect<<<ceil_div(s.n, kBlock), kBlock, 0, stream>>>(ptrs, r, 0.5 * Lx - 0.5 * slab, slab);
+        MDX_HIP(hipGetLastError());
+    }
+    win_pos = 0;
+    s.W_valid = false;
+    MDX_HIP(hipStreamSynchronize(stream));
+    return MDX_OK;
+}
+
 // ---- allocation-free N profile ------------------------------------------
 // The counters of mdx_N_profile_noalloc: one fixed device array per GPU (each
 // device that runs the library holds its own copy until the process ends),
@@ -1199,6 +1285,7 @@ int Batch::n_profile_noalloc(int r, int nb, int32_t *out) {
 }
 
 int Batch::reorder_due() {
+    if (win_pos != 0) return MDX_OK;   // not inside an open rebuild window
     std::vector<int> due;
     for (size_t r = 0; r < slots.size(); ++r) {
         const Slot &s = slots[r];
@@ -1379,9 +1466,14 @@ int Batch::autotune(int steps, mdx_tune_report *rep, int kind_override) {
         if (rc == MDX_ERR_CAPACITY) continue;   // too many cells for a slot: not a candidate
         if (rc != MDX_OK) break;
         // warm-up: every graph the timed steps launch is captured here (the
-        // per-step graphs, a whole window's, eight steps'), and caches
-        if ((rc = c.step(3)) != MDX_OK) break;
-        if ((rc = c.step(std::max(kGraphSteps, t.window))) != MDX_OK) break;
+        // per-step graphs, a whole window's, eight steps'), and caches; with
+        // windows, whole windows, so the timed steps start at a head step
+        if (t.window >= 2) {
+            if ((rc = c.step(2 * t.window)) != MDX_OK) break;
+        } else {
+            if ((rc = c.step(3)) != MDX_OK) break;
+            if ((rc = c.step(kGraphSteps)) != MDX_OK) break;
+        }
         // whole windows only, as in a long run
         const int ts = t.window >= 2 ? ((steps + t.window - 1) / t.window) * t.window : steps;
         (void)hipEventRecord(e0, c.stream);
@@ -1488,6 +1580,7 @@ int Batch::observables_many(const std::vector<int> &which, uint32_t mask, mdx_ob
         MDX_TRY(kinetic(need));
     }
     if (mask & (MDX_OBS_U | MDX_OBS_P | MDX_OBS_VIRIAL)) MDX_TRY(one_pass(*this, which, 1, PM_OBS));
+    MDX_TRY(flush());
     std::vector<DevScalars> h(slots.size());
     MDX_HIP(hipMemcpyAsync(h.data(), d_sc, h.size() * sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
     MDX_HIP(hipStreamSynchronize(stream));

@@ -50,6 +50,9 @@ struct Slot {
 
 // steps per graph launch when a call runs many steps of one mode
 constexpr int kGraphSteps = 8;
+// a windowed call of at least this many steps reads the violation flags at
+// its end; shorter calls leave them for the next synchronising call
+constexpr int kWindowCheckSteps = 64;
 // most bins of an N profile (two int counters each in LDS)
 constexpr int kProfileMaxBins = 8192;
 // since_reorder of a slot whose storage is in frame order: due at once
@@ -99,6 +102,16 @@ struct Batch {
     double2 *x0 = nullptr;
     float2 *img0 = nullptr;
     float2 *img_lo = nullptr;   // rounding remainder of the FP32 images (compensated form)
+    // rebuild windows persist across step calls: the step position within the
+    // current window (0: the next step is a head step)
+    int win_pos = 0;
+    // the last pair pass's partials are not yet combined (run_windows leaves
+    // them for k_box, k_lean or flush; the partials are consumed with the pair
+    // mode they were produced under, so flush runs before SysParams change)
+    bool fin_pending = false;
+    // a windowed run ended without checking the slots' violation flags
+    bool viol_unchecked = false;
+    std::vector<SysParams> sp_dev;   // the SysParams the device holds
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> force_ev, step_ev;
@@ -153,6 +166,7 @@ struct Batch {
     int capture_steps(hipGraphExec_t &ge, bool obs, int mode, int parity, int steps);
     int n_profile(int r, int nb, int32_t *out);
     int n_profile_noalloc(int r, int nb, int32_t *out);   // counters in the device's fixed scratch array
+    int middle_reflect(int r);
     int run_windows(int count);
     bool windows_apply();
     int pre_step();
@@ -173,6 +187,8 @@ struct Batch {
     int reorder(const std::vector<int> &which);
     int reorder_due();                        // the slots whose kReorderSteps have run
     int reset_ids(int r);                     // storage of slot r in load order
+    int flush();                              // combine pending pair partials (k_finalize)
+    int check_violations();                   // a lean step left exact forces' reach: MDX_ERR_STATE
 };
 
 }  // namespace mdx

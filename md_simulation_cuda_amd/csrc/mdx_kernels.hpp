@@ -2211,6 +2211,25 @@ __global__ void __launch_bounds__(kBlock) k_walls(BatchPtrs P, const int32_t *wa
     block_partials<2>(vals, 0u, P.partials, P.TB, gb);
 }
 
+// MDSimulation::middle_reflect_LG (src/md_env.cu:1457-1508, kernel
+// src/md_cuda_common.cu:26-65) on slot r: y wrapped, x reflected into the
+// central slab [xmin, xmin + width] with mirrored edges (period 2 width).
+__global__ void __launch_bounds__(kBlock) k_middle_reflect(BatchPtrs P, int32_t r, double xmin, double width) {
+    const SysParams &sp = P.sp[r];
+    const int li = blockIdx.x * kBlock + threadIdx.x;
+    if (li >= sp.n) return;
+    const Grid g = P.sc[r].g;
+    const int i = sp.off + li;
+    double2 p = P.x[i];
+    p.y = pbc_wrap(p.y, g.Ly);
+    const double period = 2.0 * width;
+    double ym = fmod(p.x - xmin, period);
+    if (ym < 0.0) ym += period;
+    if (ym > width) ym = period - ym;
+    p.x = xmin + ym;
+    P.x[i] = p;
+}
+
 // EnergyMinimizer::initialize_from_frame (src/md_energy_minimizer.cu:343-380):
 // wrap positions, zero velocities and accelerations (the acceleration buffer
 // is viewed as 16 bytes per particle whatever its precision).

@@ -72,6 +72,19 @@ void MDSimulation::create_from_frame(const std::vector<Particle> &frame) {
     check(mdx_create(&p, reinterpret_cast<const mdx_particle64 *>(frame.data()),
                      static_cast<int64_t>(frame.size()), &sim_),
           "MDSimulation");
+    // tuning of the pair pass: MDX_TUNING unset or "auto" (autotune at the
+    // first NVE / NVT step), "off", or "cell_scale,lanes,skin,window"
+    const char *tv = std::getenv("MDX_TUNING");
+    const std::string spec = tv ? tv : "auto";
+    if (spec == "auto" || spec.empty()) {
+        autotune_pending_ = true;
+    } else if (spec != "off") {
+        double cs = 1.0, skin = 0.0;
+        int lanes = 1, window = 0;
+        if (std::sscanf(spec.c_str(), "%lf,%d,%lf,%d", &cs, &lanes, &skin, &window) < 3)
+            abort_run("[MDSimulation] MDX_TUNING must be auto, off or cell_scale,lanes,skin[,window]: " + spec);
+        set_tuning(cs, lanes, skin, window);
+    }
     MDConfig &c = cfg_manager.config;
     c.rank_idx = 0;
     c.n_local = static_cast<int>(frame.size());
@@ -153,8 +166,27 @@ void MDSimulation::sync_scalars() {
     cfg_manager.config.x_max = o.Lx;
 }
 
+void MDSimulation::set_tuning(double cell_scale, int lanes_per_particle, double skin, int window) {
+    mdx_tuning tu{};
+    tu.cell_scale = cell_scale;
+    tu.lanes_per_particle = lanes_per_particle;
+    tu.skin = skin;
+    tu.window = window;
+    check(mdx_set_tuning(sim_, &tu), "set_tuning", true);
+    autotune_pending_ = false;
+}
+
+void MDSimulation::autotune(int kind, int steps_per_candidate) {
+    mdx_tune_report rep{};
+    check(mdx_autotune(sim_, kind, steps_per_candidate, &rep), "autotune", true);
+    autotune_pending_ = false;
+}
+
+void MDSimulation::middle_reflect_LG() { check(mdx_middle_reflect(sim_), "middle_reflect_LG"); }
+
 void MDSimulation::step_many(int kind, int nsteps, double pressure_ext, bool do_middle_wrap) {
     if (nsteps <= 0) return;
+    if (autotune_pending_ && (kind == MDX_NVE || kind == MDX_NVT)) autotune(kind);
     check(mdx_step(sim_, kind, nsteps, pressure_ext, do_middle_wrap ? MDX_STEP_MIDDLE_WRAP : 0u), "step");
     t += cfg_manager.config.dt * nsteps;
     if (kind == MDX_NPH || kind == MDX_PISTON) sync_scalars();

@@ -224,3 +224,27 @@ def test_windows_same_alone_and_in_batch_with_a_falling_back_neighbour():
         got = b.collect(r)
         assert np.array_equal(got["pos"], alone[r]["pos"]) and np.array_equal(got["vel"], alone[r]["vel"]), r
     b.close()
+
+
+@pytest.mark.parametrize("kind", [abi.NVT, abi.NVE])
+def test_windows_run_on_across_step_calls(kind):
+    """With rebuild windows, 100 calls of one step (the drop-in's
+    step_single_nose_hoover loop) give the trajectory of one 100-step call
+    bit for bit: each call continues the window the last one left open."""
+    f, Lx, Ly = lattice_frame(5000, 0.75, seed=61, jitter=0.05, T=0.9, aspect=2.0)
+    kw = dict(eps_ab=0.5, dt=0.005, T_target=0.9, Q=10.0)
+    say(f"windows across calls: kind={kind}")
+    res = []
+    for calls in ([100], [1] * 100, [7, 1, 1, 30, 61]):
+        s = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+        s.set_tuning(1.0, 1, 0.5, 12)
+        for c in calls:
+            s.step(kind, c)
+        s.sync()
+        ob = s.observables(abi.OBS_K)
+        res.append((s.collect(), ob.K, ob.xi))
+        s.close()
+    for fr, K, xi in res[1:]:
+        for k in ("pos", "vel", "acc"):
+            assert np.array_equal(fr[k], res[0][0][k]), k
+        assert K == res[0][1] and xi == res[0][2]

@@ -70,11 +70,9 @@ def test_results_do_not_depend_on_reordering(name, kind, tuning, prec):
     a = run(f, Lx, Ly, kw, kind, [120, 57, 63], 0, tuning, prec, p_ext)
     b = run(f, Lx, Ly, kw, kind, [120, 57, 63], 3, tuning, prec, p_ext)
     same(a, b)
+    # one call of 240 steps: rebuild windows run on across calls, so the
+    # split into calls changes nothing either
     c = run(f, Lx, Ly, kw, kind, [240], 40, tuning, prec, p_ext)
-    if tuning and tuning[3] > 0:
-        # rebuild windows start at each step call, so the rebuild steps (and
-        # the list order they fix) follow the calls: compare like calls
-        a = run(f, Lx, Ly, kw, kind, [240], 0, tuning, prec, p_ext)
     same(a, c)
 
 
