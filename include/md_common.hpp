@@ -36,6 +36,44 @@ bool create_folder(const std::filesystem::path &path, int rank_idx);
 bool append_latest_line(const std::filesystem::path &src, const std::filesystem::path &dst,
                         int rank_idx, const std::string &tag);
 
+// Rank 0 only: replace filepath with "bin,rho" and one "bin,value" line per
+// entry, values in the default ostream format (6 significant digits); an
+// open failure is reported on stderr under [tag].  Reference declaration:
+// include/md_common.hpp:52-56.
+void write_density_profile_csv(const std::filesystem::path &filepath, const std::vector<double> &density,
+                               int rank_idx, const std::string &tag);
+
+#if __has_include(<fmt/core.h>)
+// Drivers built with {fmt}, as the reference's are, also get its two
+// formatting helpers (include/md_common.hpp:33-40 and 63-82 there).
+#include <fmt/core.h>
+#include <fstream>
+#include <utility>
+
+template <typename... Args>
+void RankZeroPrint(int rank_idx, fmt::format_string<Args...> format_str, Args &&...args) {
+    if (rank_idx != 0) return;
+    fmt::print(format_str, std::forward<Args>(args)...);
+    std::fflush(stdout);
+}
+
+// Rank 0 only: append the formatted text (the caller ends it with '\n').
+template <typename... Args>
+bool append_csv(const std::filesystem::path &filepath, int rank_idx, const std::string &tag,
+                fmt::format_string<Args...> fmt_str, Args &&...args) {
+    if (rank_idx != 0) return true;
+    const std::string text = fmt::format(fmt_str, std::forward<Args>(args)...);
+    std::ofstream out(filepath, std::ios::out | std::ios::app);
+    if (!out) {
+        fmt::print(stderr, "[{}] Failed to open {} for appending.\n", tag, filepath.string());
+        return false;
+    }
+    out << text;
+    out.flush();
+    return true;
+}
+#endif
+
 // Write/read exactly this many bytes (write throws on a short write).
 void write_exact(std::FILE *fp, const void *buf, std::size_t bytes);
 bool read_exact(std::FILE *fp, void *buf, std::size_t bytes);

@@ -11,94 +11,104 @@
 
 namespace {
 
-// Restates MDSimulation::init_particles, src/md_env.cu:1296-1441 (rank 0 part).
+// The constructor's starting frame (behaviour of src/md_env.cu:1296-1441, the
+// rank-0 part), built from this specification:
+//  - the box splits at x_split = devide_p * Lx into an A slab [0, x_split)
+//    holding particles 0 .. n_A-1 (type 0) and a B slab [x_split, Lx) holding
+//    n_A .. N-1 (type 1); afterwards every x moves by
+//    Lx (1 - devide_p) / 2 and wraps into [0, Lx), which centres the A slab;
+//  - each slab gets its own square lattice (Lattice below); particle k of a
+//    slab sits at column k % cols, row k / cols;
+//  - one std::mt19937 seeded with 12345 feeds every draw, in this order:
+//    for each placed particle (A, then B) a U(-1e-5, 1e-5) jitter of x, then
+//    of y; then for particle 0 .. N-1 vx, then vy from N(0, sqrt(T / m));
+//    then theta ~ U(0, 2 pi) for particle 0 .. N-1.  A site outside its slab
+//    is left unplaced and draws nothing (it cannot happen for a lattice from
+//    Lattice, but the rule fixes the stream);
+//  - the velocities lose their centre-of-mass velocity (total mass from the
+//    two counts) and are scaled to K = (2N - 2) T / 2.
+// The libstdc++ distributions define the stream, so they are used as such.
+
+// Square lattice for `count` sites in a w x h slab: the pitch starts at
+// sqrt(w h / count) and is multiplied by 0.99 until floor(w / pitch) columns
+// times floor(h / pitch) rows (each at least 1) hold `count` sites.
+struct Lattice {
+    double pitch;
+    int64_t cols, rows;
+};
+
+Lattice fit_lattice(double w, double h, int64_t count) {
+    Lattice L{std::sqrt(w * h / static_cast<double>(count)), 1, 1};
+    for (;;) {
+        L.cols = std::max<int64_t>(1, static_cast<int64_t>(w / L.pitch));
+        L.rows = std::max<int64_t>(1, static_cast<int64_t>(h / L.pitch));
+        if (L.cols * L.rows >= count) return L;
+        L.pitch *= 0.99;
+    }
+}
+
+struct Slab {
+    double x0, x1;    // x range before the centring shift
+    int64_t first;    // index of the slab's first particle
+    int64_t count;
+    int32_t type;
+};
+
+void place_slab(const Slab &s, double Lx, double Ly, double shift, std::mt19937 &rng, mdx_particle64 *P) {
+    if (s.count <= 0) return;
+    const Lattice L = fit_lattice(s.x1 - s.x0, Ly, s.count);
+    std::uniform_real_distribution<double> jitter(-1.0e-5, 1.0e-5);
+    for (int64_t k = 0; k < s.count; ++k) {
+        const double col = static_cast<double>(k % L.cols), row = static_cast<double>(k / L.cols);
+        const double x = (col + 0.5) * L.pitch + s.x0;
+        const double y = (row + 0.5) * L.pitch;
+        if (x >= s.x1 || y >= Ly) continue;
+        mdx_particle64 &p = P[s.first + k];
+        p.pos[0] = std::fmod(x + shift + Lx, Lx) + jitter(rng);
+        p.pos[1] = y + jitter(rng);
+        p.type = s.type;
+    }
+}
+
 void init_particles(const mdx_init_params &ip, mdx_particle64 *P) {
-    const double devide_p = ip.devide_p;
-    const int n_particles_global = static_cast<int>(ip.n_particles_global);
-    const int n_particles_type0 = static_cast<int>(ip.n_particles_type0);
-    const int n_particles_type1 = n_particles_global - n_particles_type0;
-    const double box_w = ip.box_w, box_h = ip.box_h;
-    const double unshifted_x_devide = box_w * devide_p;
-    const double shift_dx = box_w * (0.5 - devide_p / 2.0);
-    const double T_init = ip.T_init;
-    const double mass_type0 = ip.mass_a, mass_type1 = ip.mass_b;
+    const int64_t N = ip.n_particles_global, nA = ip.n_particles_type0, nB = N - nA;
+    const double Lx = ip.box_w, Ly = ip.box_h, T = ip.T_init;
+    const double x_split = Lx * ip.devide_p;
+    const double shift = Lx * (0.5 - ip.devide_p / 2.0);
     std::mt19937 rng(12345);
-    std::uniform_real_distribution<double> pos_perturb(-1.0e-5, 1.0e-5);
-    double spacing_type0 = std::sqrt(unshifted_x_devide * box_h / n_particles_type0);
-    double spacing_type1 = std::sqrt((box_w - unshifted_x_devide) * box_h / n_particles_type1);
-    auto adjust_grid = [](double width, double height, int n, double &spacing, int &n_cols, int &n_rows) {
-        n_rows = static_cast<int>(height / spacing);
-        n_cols = static_cast<int>(width / spacing);
-        if (n_rows <= 0) n_rows = 1;
-        if (n_cols <= 0) n_cols = 1;
-        while (n_rows * n_cols < n) {
-            spacing *= 0.99;
-            n_rows = static_cast<int>(height / spacing);
-            n_cols = static_cast<int>(width / spacing);
-            if (n_rows <= 0) n_rows = 1;
-            if (n_cols <= 0) n_cols = 1;
-        }
-    };
-    int n_rows_type0 = 0, n_cols_type0 = 0, n_rows_type1 = 0, n_cols_type1 = 0;
-    adjust_grid(unshifted_x_devide, box_h, n_particles_type0, spacing_type0, n_cols_type0, n_rows_type0);
-    adjust_grid(box_w - unshifted_x_devide, box_h, n_particles_type1, spacing_type1, n_cols_type1, n_rows_type1);
-    for (int j = 0; j < n_rows_type0; ++j) {
-        for (int i = 0; i < n_cols_type0; ++i) {
-            const int idx = j * n_cols_type0 + i;
-            if (idx >= n_particles_type0) break;
-            const double x_local = (i + 0.5) * spacing_type0;
-            const double y = (j + 0.5) * spacing_type0;
-            if (x_local >= unshifted_x_devide) continue;
-            if (y >= box_h) continue;
-            const double x = std::fmod(x_local + shift_dx + box_w, box_w);
-            P[idx].pos[0] = x + pos_perturb(rng);
-            P[idx].pos[1] = y + pos_perturb(rng);
-            P[idx].type = 0;
-        }
+    // (the slab starting at 0 adds x0 = 0, which leaves every x unchanged)
+    place_slab({0.0, x_split, 0, nA, 0}, Lx, Ly, shift, rng, P);
+    place_slab({x_split, Lx, nA, nB, 1}, Lx, Ly, shift, rng, P);
+
+    auto mass = [&](const mdx_particle64 &p) { return p.type == 0 ? ip.mass_a : ip.mass_b; };
+    // Two draws per particle use up exactly the pair that each polar-method
+    // step of normal_distribution makes, so one standard normal scaled by
+    // sqrt(T / m) gives the same values as a fresh N(0, sqrt(T / m)) per
+    // particle.
+    std::normal_distribution<double> z;
+    double px = 0.0, py = 0.0;
+    for (int64_t i = 0; i < N; ++i) {
+        const double m = mass(P[i]), s = std::sqrt(T / m);
+        P[i].vel[0] = z(rng) * s;
+        P[i].vel[1] = z(rng) * s;
+        px += m * P[i].vel[0];
+        py += m * P[i].vel[1];
     }
-    for (int j = 0; j < n_rows_type1; ++j) {
-        for (int i = 0; i < n_cols_type1; ++i) {
-            const int idx = j * n_cols_type1 + i + n_particles_type0;
-            if (idx >= n_particles_global) break;
-            const double x_local = (i + 0.5) * spacing_type1 + unshifted_x_devide;
-            const double y = (j + 0.5) * spacing_type1;
-            if (x_local >= box_w) continue;
-            if (y >= box_h) continue;
-            const double x = std::fmod(x_local + shift_dx + box_w, box_w);
-            P[idx].pos[0] = x + pos_perturb(rng);
-            P[idx].pos[1] = y + pos_perturb(rng);
-            P[idx].type = 1;
-        }
+    const double M = static_cast<double>(nA) * ip.mass_a + static_cast<double>(nB) * ip.mass_b;
+    const double ux = px / M, uy = py / M;
+    double K = 0.0;
+    for (int64_t i = 0; i < N; ++i) {
+        P[i].vel[0] -= ux;
+        P[i].vel[1] -= uy;
+        K += 0.5 * mass(P[i]) * (P[i].vel[0] * P[i].vel[0] + P[i].vel[1] * P[i].vel[1]);
     }
-    const double total_mass = n_particles_type0 * mass_type0 + n_particles_type1 * mass_type1;
-    double sum_px = 0.0, sum_py = 0.0;
-    for (int i = 0; i < n_particles_global; ++i) {
-        const double mass = (P[i].type == 0) ? mass_type0 : mass_type1;
-        const double stddev = std::sqrt(T_init / mass);
-        std::normal_distribution<double> dist(0.0, stddev);
-        P[i].vel[0] = dist(rng);
-        P[i].vel[1] = dist(rng);
-        sum_px += mass * P[i].vel[0];
-        sum_py += mass * P[i].vel[1];
+    const double scale = std::sqrt((2.0 * static_cast<double>(N) - 2.0) * 0.5 * T / K);
+    std::uniform_real_distribution<double> angle(0.0, 2.0 * M_PI);
+    for (int64_t i = 0; i < N; ++i) {
+        P[i].vel[0] *= scale;
+        P[i].vel[1] *= scale;
     }
-    const double vcm_x = sum_px / total_mass, vcm_y = sum_py / total_mass;
-    for (int i = 0; i < n_particles_global; ++i) {
-        P[i].vel[0] -= vcm_x;
-        P[i].vel[1] -= vcm_y;
-    }
-    double current_K = 0.0;
-    const double target_K = (2 * static_cast<double>(n_particles_global) - 2) * 0.5 * T_init;
-    for (int i = 0; i < n_particles_global; ++i) {
-        const double mass = (P[i].type == 0) ? mass_type0 : mass_type1;
-        current_K += 0.5 * mass * (P[i].vel[0] * P[i].vel[0] + P[i].vel[1] * P[i].vel[1]);
-    }
-    const double rescale = std::sqrt(target_K / current_K);
-    for (int i = 0; i < n_particles_global; ++i) {
-        P[i].vel[0] *= rescale;
-        P[i].vel[1] *= rescale;
-    }
-    std::uniform_real_distribution<double> theta_dist(0.0, 2.0 * M_PI);
-    for (int i = 0; i < n_particles_global; ++i) P[i].theta = theta_dist(rng);
+    for (int64_t i = 0; i < N; ++i) P[i].theta = angle(rng);
 }
 
 // Portable seeded streams: 53-bit uniforms from mt19937_64, Box-Muller normals.

@@ -597,6 +597,8 @@ void orc_step_nvt(orc_sim *s, int32_t nsteps, int32_t middle_wrap) {
     for (int k = 0; k < nsteps; ++k) step_nvt(*S(s), middle_wrap != 0);
 }
 
+void orc_middle_reflect(orc_sim *s) { middle_reflect(*S(s)); }
+
 void orc_step_nph(orc_sim *s, int32_t nsteps, double p_ext) {
     for (int k = 0; k < nsteps; ++k)
         if (!step_nph(*S(s), p_ext)) return;

@@ -69,6 +69,7 @@ _SIG = {
     "orc_get_obs": (None, [_P, C.POINTER(OObs)]),
     "orc_step_nve": (None, [_P, C.c_int32]),
     "orc_step_nvt": (None, [_P, C.c_int32, C.c_int32]),
+    "orc_middle_reflect": (None, [_P]),
     "orc_step_nph": (None, [_P, C.c_int32, C.c_double]),
     "orc_step_piston": (None, [_P, C.c_int32, C.c_double]),
     "orc_fire": (None, [_P, C.POINTER(OFireParams), C.c_double, C.c_int32, C.POINTER(OFireResult),
@@ -183,6 +184,9 @@ class OracleSim:
 
     def step_nvt(self, n: int = 1, middle_wrap: bool = False) -> None:
         self._lib.orc_step_nvt(self._h, n, 1 if middle_wrap else 0)
+
+    def middle_reflect(self) -> None:
+        self._lib.orc_middle_reflect(self._h)
 
     def step_nph(self, n: int, p_ext: float) -> None:
         self._lib.orc_step_nph(self._h, n, p_ext)

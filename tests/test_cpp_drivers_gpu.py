@@ -45,3 +45,12 @@ def test_profiles_through_the_cpp_class():
     r = run(CPP / "profile_smoke", GOLDEN / "nph_config.json")
     assert r.returncode == 0
     assert "profile ok n=512" in r.stdout
+
+
+def test_save_env_and_resume_constructor(tmp_path):
+    # save_env after sample_collect, then MDSimulation(cfg, comm, file, step):
+    # the resumed frame is the saved one bit for bit, through the device too,
+    # forces, U and K at load agree, and a resume is reproducible
+    r = run(CPP / "save_resume", "run", tmp_path)
+    assert r.returncode == 0
+    assert "save_resume bad=0" in r.stdout

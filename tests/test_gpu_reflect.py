@@ -1,0 +1,83 @@
+"""middle_reflect_LG (src/md_env.cu:1457-1508 of the reference) on the GPU
+against the oracle: the fold itself bit for bit, and NVT continued after it
+(the neighbour-list windows must restart from the folded positions).
+
+Frame: 4096 particles on a jittered square lattice at density 0.12 (seeded),
+devide_p 0.7: the slab of width 0.7 Lx + 2 sigma leaves 1152 particles
+outside, and folding them in brings no two particles closer than 0.96 sigma
+(checked when the frame was chosen), so the run after the fold stays smooth.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from md_simulation_cuda_amd import abi
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KW = dict(eps_ab=0.5, dt=0.005, T_target=1.0, Q=10.0, devide_p=0.7)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def fold_frame():
+    return abi.gen_lattice(4096, 2048, 0.12, 0.25, 1.0, seed=14)
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_COMP32])
+def test_middle_reflect_bit_exact(prec):
+    f, Lx, Ly = fold_frame()
+    say(f"middle_reflect precision {prec}: N=4096, devide_p 0.7")
+    g = abi.Simulation(abi.make_params(Lx, Ly, precision=prec, **KW), f)
+    g.middle_reflect()
+    fg = g.collect()
+    g.close()
+    o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, **KW), f)
+    o.middle_reflect()
+    fo = o.frame()
+    o.close()
+    moved = int((fo["pos"][:, 0] != f["pos"][:, 0]).sum())
+    say(f"  moved {moved}; max |dx| gpu-oracle {np.abs(fg['pos'] - fo['pos']).max():.3e}")
+    assert moved == 1152
+    assert np.array_equal(fg["pos"], fo["pos"])
+    assert np.array_equal(fg["vel"], f["vel"])
+
+
+@pytest.mark.parametrize("prec,tuning,tol", [
+    (abi.PREC_FP64, None, 1e-9),
+    (abi.PREC_COMP32, (1.0, 1, 0.65, 16), 2e-5),
+])
+def test_nvt_continues_after_reflect(prec, tuning, tol):
+    """10 NVT steps (mid-window for the list tuning), the fold, 20 more
+    steps, against the oracle doing the same; xi and K relative to their
+    scales, positions absolute, all within `tol`."""
+    f, Lx, Ly = fold_frame()
+    say(f"NVT / reflect / NVT precision {prec} tuning {tuning}")
+    g = abi.Simulation(abi.make_params(Lx, Ly, precision=prec, **KW), f)
+    if tuning:
+        g.set_tuning(*tuning)
+    o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, **KW), f)
+    o.compute_forces()
+    g.step(abi.NVT, 10)
+    o.step_nvt(10)
+    g.middle_reflect()
+    o.middle_reflect()
+    g.step(abi.NVT, 20)
+    o.step_nvt(20)
+    og = g.observables(abi.OBS_K)
+    fg = g.collect()
+    g.close()
+    oo, Ko, fo = o.obs(), o.K(), o.frame()
+    o.close()
+    d = fg["pos"] - fo["pos"]
+    d -= np.round(d / [Lx, Ly]) * [Lx, Ly]
+    dxi = abs(og.xi - oo.xi) / max(abs(oo.xi), 1e-3)
+    dk = abs(og.K - Ko) / Ko
+    say(f"  xi {og.xi:.9e} vs {oo.xi:.9e}; K {og.K:.9e} vs {Ko:.9e}; max |dx| {np.abs(d).max():.3e}")
+    assert dxi <= tol and dk <= tol
+    assert np.abs(d).max() <= tol

@@ -1,6 +1,7 @@
 """CPU tests of the host side: C ABI exports, the config JSON against the
 reference's golden file, PFRAME v2 bytes checked independently with Python's
 struct and zlib, and the initial-frame generators."""
+import hashlib
 import json
 import re
 import struct
@@ -115,6 +116,15 @@ def test_init_particles_invariants():
     # deterministic: the reference seeds mt19937 with 12345
     g = abi.init_particles(n, n0, Lx, Ly, T, mass_a=1.0, mass_b=2.0)
     assert f.tobytes() == g.tobytes()
+    # the frame is fixed byte for byte: SHA-256 of this frame and of the 2:1
+    # slab of 3000 particles at density 0.75 as first generated, before the
+    # generator was restructured (a change of draw order, lattice fit or
+    # summation order changes them)
+    assert hashlib.sha256(f.tobytes()).hexdigest() == (
+        "365218bd77da631095dd0d4eb9452f2a149547d03249d88ceba383a451ab695a")
+    s = abi.init_particles(3000, 1500, np.sqrt(3000 / 0.75 * 2), np.sqrt(3000 / 0.75 / 2), 0.6, devide_p=0.5)
+    assert hashlib.sha256(s.tobytes()).hexdigest() == (
+        "a4b563333b735f9b02372200c9908c7bf3356d36e774eec921999086bddcf912")
 
 
 def test_synthetic_generators():
@@ -157,3 +167,38 @@ def test_particle_csv_and_xyz_text(tmp_path):
     assert lines[2:5] == ["A 1.5000000000 2.2500000000 0.0", "B -0.1250000000 7.0000000000 0.0",
                           "X 3.0000000000 4.0000000000 0.0"]
     assert lines[5:7] == ["3", "second"] and len(lines) == 10
+
+
+def test_density_csv_folders_and_latest_line(tmp_path):
+    # write_density_profile_csv (src/md_common.cpp:64-82 of the reference):
+    # rank 0 replaces the file with "bin,rho" and "bin,value" lines in the
+    # default ostream format, which is printf's %g (6 significant digits)
+    r = run("common_text_io", tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rho = [0.5, -1.0 / 3.0, 1e-7, 123456789.0, 0.0, -0.25, 1.0, 2.0 / 3.0, 1e300]
+    want = "bin,rho\n" + "".join("%d,%g\n" % (i, v) for i, v in enumerate(rho))
+    assert (tmp_path / "density.csv").read_bytes() == want.encode()
+    assert want.splitlines()[1:4] == ["0,0.5", "1,-0.333333", "2,1e-07"]
+    assert (tmp_path / "empty.csv").read_bytes() == b"bin,rho\n"
+    assert not (tmp_path / "density_rank1.csv").exists()
+    # create_folder: nested tree on rank 0, nothing on rank 1
+    assert (tmp_path / "made" / "a" / "b").is_dir()
+    assert not (tmp_path / "not_made").exists()
+    # append_latest_line: last non-empty line, with or without a final newline
+    assert (tmp_path / "latest.csv").read_bytes() == b"b,2\nd,4\n"
+    assert not (tmp_path / "latest_rank1.csv").exists()
+
+
+@pytest.mark.parametrize("mode,message", [
+    ("missing", "Error: Frame index 7 not found in"),
+    ("count", "[Error] Loaded particle count 16 doesn't match n_particles_global 20."),
+    ("badfile", "Not a valid PFRAME file"),
+])
+def test_resume_constructor_failures(tmp_path, mode, message):
+    # MDSimulation(cfg, comm, file, step) (src/md_env.cu:81-140 of the
+    # reference) ends the run when the frame cannot be loaded; the checks come
+    # before any device work, so they run without a GPU
+    r = run("save_resume", mode, tmp_path)
+    assert r.returncode == 1, r.stdout + r.stderr
+    assert message in r.stderr
+    assert "resumed" not in r.stdout

@@ -222,3 +222,55 @@ def test_nph_conserved_energy_drift_is_second_order():
     r1, r2 = d[0] / d[1], d[1] / d[2]
     assert 3.5 < r1 < 4.6 and 3.5 < r2 < 4.6, d
     assert d[2] < 1e-4, d
+
+
+# ---------------------------------------------------------------------------
+# middle_reflect_LG
+# ---------------------------------------------------------------------------
+
+def test_middle_reflect_hand_cases():
+    """src/md_env.cu:1457-1508: x is mirrored at the edges of the central slab
+    of width devide_p Lx + 2 sigma_max (at most Lx) until it lies inside, y is
+    wrapped.  Lx = 20, Ly = 10, devide_p 0.3, sigma 1: slab [6, 14]."""
+    x = [10.0, 6.0, 14.0, 15.0, 19.0, 5.5, 0.5, 13.75]
+    y = [1.0, -0.5, 10.25, 3.0, 9.0, 4.0, 5.0, 2.0]
+    want_x = [10.0, 6.0, 14.0, 13.0, 9.0, 6.5, 11.5, 13.75]   # 15 -> 14 - 1; 19 -> 14 - 5; 0.5 -> 6 + 5.5
+    want_y = [1.0, 9.5, 0.25, 3.0, 9.0, 4.0, 5.0, 2.0]
+    f = np.zeros(len(x), O.PARTICLE_DTYPE)
+    f["pos"] = np.stack([x, y], 1)
+
+    def fold(devide_p, Lx=20.0):
+        o = O.OracleSim(O.make_params(Lx, 10.0, force_mode=0, devide_p=devide_p), f)
+        o.middle_reflect()
+        fr = o.frame()
+        o.close()
+        return fr["pos"]
+
+    p = fold(0.3)
+    assert p[:, 0].tolist() == want_x and p[:, 1].tolist() == want_y
+    # a slab wider than the box is the box: x in [0, Lx) stays, y still wraps
+    p = fold(0.95)
+    assert p[:, 0].tolist() == x and p[:, 1].tolist() == want_y
+    # devide_p <= 0 switches the fold off entirely
+    p = fold(0.0)
+    assert p[:, 0].tolist() == x and p[:, 1].tolist() == y
+
+
+def test_middle_reflect_is_a_triangle_wave():
+    """On random x over several periods the fold equals the triangle wave
+    x_min + w - | ((x - x_min) mod 2w) - w | of period 2w."""
+    rng = np.random.default_rng(7)
+    Lx, Ly, dp = 50.0, 10.0, 0.4
+    w = dp * Lx + 2.0
+    x_min = 0.5 * Lx - 0.5 * w
+    x = rng.uniform(-3 * Lx, 3 * Lx, 5000)
+    f = np.zeros(len(x), O.PARTICLE_DTYPE)
+    f["pos"][:, 0] = x
+    f["pos"][:, 1] = rng.uniform(0, Ly, len(x))
+    o = O.OracleSim(O.make_params(Lx, Ly, force_mode=0, devide_p=dp), f)
+    o.middle_reflect()
+    got = o.frame()["pos"][:, 0]
+    o.close()
+    tri = x_min + w - np.abs(np.mod(x - x_min, 2 * w) - w)
+    assert np.abs(got - tri).max() < 1e-12
+    assert got.min() >= x_min and got.max() <= x_min + w
