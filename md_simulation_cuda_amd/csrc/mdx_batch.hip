@@ -114,7 +114,7 @@ Batch::~Batch() {
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
                      d_sp, d_sc, d_blk2sys, d_cblk2sys, d_lb2sys, d_want, sorted_old, cs_old, movers, mover_off,
-                     n_movers, arrivals, key_list, x0, img0, img_lo, nlist, ncount, pid, sid, reorder_tmp};
+                     n_movers, arrivals, key_list, x0, img0, nlist, ncount, pid, sid, reorder_tmp};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
     for (auto *vec : {&force_ev, &step_ev})
@@ -204,8 +204,6 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     MDX_HIP(hipMalloc(&v, n16));
     MDX_HIP(hipMalloc(&a, n16));
     MDX_HIP(hipMalloc(&img, n16));
-    MDX_HIP(hipMalloc(&img_lo, n16 / 2));
-    MDX_HIP(hipMemset(img_lo, 0, n16 / 2));
     MDX_HIP(hipMalloc(&cls, n1));
     MDX_HIP(hipMalloc(&icls, n1));
     MDX_HIP(hipMalloc(&key, n4));
@@ -254,7 +252,6 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     ptrs.sorted = sorted;
     ptrs.scell = scell;
     ptrs.img = img;
-    ptrs.img_lo = prec == MDX_PREC_COMP32 ? img_lo : nullptr;
     ptrs.icls = icls;
     ptrs.partials = partials;
     ptrs.sorted_old = sorted_old;
@@ -366,16 +363,16 @@ static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &
     if constexpr (!OBS) {   // lean steps take no energy sums
         if (lean) {
             switch (lpp) {
-            case 2: k_pair<float2, ACC, OBS, UNI, 2, true><<<grid, kBlock * 2, 0, st>>>(p, parity); return;
-            case 4: k_pair<float2, ACC, OBS, UNI, 4, true><<<grid, kBlock * 4, 0, st>>>(p, parity); return;
-            default: k_pair<float2, ACC, OBS, UNI, 1, true><<<grid, kBlock, 0, st>>>(p, parity); return;
+            case 2: k_pair<int2, ACC, OBS, UNI, 2, true><<<grid, kBlock * 2, 0, st>>>(p, parity); return;
+            case 4: k_pair<int2, ACC, OBS, UNI, 4, true><<<grid, kBlock * 4, 0, st>>>(p, parity); return;
+            default: k_pair<int2, ACC, OBS, UNI, 1, true><<<grid, kBlock, 0, st>>>(p, parity); return;
             }
         }
     }
     switch (lpp) {
-    case 2: k_pair<float2, ACC, OBS, UNI, 2, false><<<grid, kBlock * 2, 0, st>>>(p, 0); break;
-    case 4: k_pair<float2, ACC, OBS, UNI, 4, false><<<grid, kBlock * 4, 0, st>>>(p, 0); break;
-    default: k_pair<float2, ACC, OBS, UNI, 1, false><<<grid, kBlock, 0, st>>>(p, 0); break;
+    case 2: k_pair<int2, ACC, OBS, UNI, 2, false><<<grid, kBlock * 2, 0, st>>>(p, 0); break;
+    case 4: k_pair<int2, ACC, OBS, UNI, 4, false><<<grid, kBlock * 4, 0, st>>>(p, 0); break;
+    default: k_pair<int2, ACC, OBS, UNI, 1, false><<<grid, kBlock, 0, st>>>(p, 0); break;
     }
 }
 
@@ -413,7 +410,7 @@ int Batch::launch_step_kernels(bool obs, int mode, int parity) {
         if (prec == MDX_PREC_FP64)
             k_slot_image<double2><<<TB, kBlock, 0, stream>>>(ptrs);
         else
-            k_slot_image<float2><<<TB, kBlock, 0, stream>>>(ptrs);
+            k_slot_image<int2><<<TB, kBlock, 0, stream>>>(ptrs);
         if (nlist && prec != MDX_PREC_FP64) k_list_build<<<TB, kBlock, 0, stream>>>(ptrs);
     }
     if (TB > 0) {
@@ -1302,7 +1299,6 @@ int Batch::set_precision(int precision) {
     MDX_HIP(hipSetDevice(device));
     MDX_HIP(hipStreamSynchronize(stream));
     prec = precision;
-    ptrs.img_lo = prec == MDX_PREC_COMP32 ? img_lo : nullptr;
     invalidate_graphs();
     // accelerations and the cell image change representation: recompute
     std::vector<int> per, pis;
@@ -1338,7 +1334,7 @@ int Batch::ensure_list_arrays() {
     const size_t n = static_cast<size_t>(std::max<int64_t>(N, 1));
     MDX_HIP(hipMalloc(&key_list, n * 4));
     MDX_HIP(hipMalloc(&x0, n * sizeof(double2)));
-    MDX_HIP(hipMalloc(&img0, n * sizeof(float2)));
+    MDX_HIP(hipMalloc(&img0, n * sizeof(int2)));
     MDX_HIP(hipMalloc(&ncount, n * 4));
     MDX_HIP(hipMalloc(&nlist, n * static_cast<size_t>(list_cap) * 4));
     MDX_HIP(hipMemset(ncount, 0, n * 4));

@@ -100,8 +100,7 @@ struct Batch {
     int reorder_every = kReorderSteps;         // 0: storage stays in load order
     void *reorder_tmp = nullptr;              // 16 B per particle, allocated by the first reorder
     double2 *x0 = nullptr;
-    float2 *img0 = nullptr;
-    float2 *img_lo = nullptr;   // rounding remainder of the FP32 images (compensated form)
+    int2 *img0 = nullptr;
     // rebuild windows persist across step calls: the step position within the
     // current window (0: the next step is a head step)
     int win_pos = 0;

@@ -31,17 +31,19 @@ struct PairTable {
     float r2_guard;        // FP32 pairs closer than this are evaluated in FP64
     float pad;
     // compensated FP32 form: pairs with r^2 below (kCoreRatio sigma_ij)^2,
-    // the repulsive core, are evaluated in FP64 from an FP64 displacement
+    // deep in the repulsive core, are evaluated in FP64 from their exact
+    // fixed-point displacement
     float rcore2f[3];
     float pad2;
 };
 
-// Radius of the repulsive core, in units of sigma_ij, inside which the
-// compensated FP32 form evaluates a pair in FP64.  Inside it |dF/dr| grows as
-// r^-14, so one FP32 rounding of the displacement (~6e-8 sigma) moves a pair
-// force by up to ~1e-4 epsilon/sigma; about 1.4 of the ~15 in-cutoff partners
-// of a particle lie inside it in the liquid at density 0.75, T 0.6.
-constexpr double kCoreRatio = 1.1;
+// Radius, in units of sigma_ij, inside which the compensated FP32 form
+// evaluates a pair in FP64.  The displacement is exact (fixed point) up to
+// its rounding to FP32 (half an ulp, ~3e-8 sigma at r ~ sigma); inside r ~
+// sigma the force grows as r^-13 and that rounding moves it by up to ~1e-4
+// epsilon/sigma, so those pairs -- about 0.1 of the ~15 in-cutoff partners of
+// a particle in the liquid at density 0.75, T 0.6 -- take FP64 arithmetic.
+constexpr double kCoreRatio = 1.0;
 
 // Periodic wrap of include/md_cuda_common.hpp:20-29: fmod(x, L), plus L when
 // negative.  The common cases return exactly what fmod returns (x in [0, L):
@@ -90,6 +92,15 @@ __host__ __device__ inline int cell_coord(double x, double L, int n) {
 }
 
 // Geometry of one cell grid.
+//
+// The FP32 forms keep each particle's position relative to its cell's centre
+// as fixed-point int32, coordinate * 2^qk.  A pair's displacement is then the
+// difference of two such images less the neighbour cell's offset (wq, hq),
+// formed exactly in integer arithmetic; only its conversion to FP32 rounds.
+// qk is the largest exponent for which 5 max(w, h) * 2^qk < 2^31: images lie
+// within w / 2 (plus the drift since the last build) of the centre, and the
+// widest displacement formed, towards a cell two columns away, stays below
+// 5 max(w, h), so no difference wraps.
 struct Grid {
     double Lx, Ly;
     double w, h;       // cell sides
@@ -98,8 +109,22 @@ struct Grid {
     int periodic_y;
     int small_x;       // cell side < rc_max in x: per-pair minimum image
     int small_y;
-    int pad;
+    int qk;            // fixed-point exponent of the FP32 forms' images
+    int wq, hq;        // cell sides in fixed point (rounded)
+    int Lxq, Lyq;      // box sides in fixed point, for the minimum image on small grids
+                       // (there Lx = 3 w fits; elsewhere clamped and unused)
+    float q2f;         // 2^(2 qk): r^2 from fixed-point units to FP32 units of sigma^2
+    float qinvf;       // 2^-qk
+    double qs, qinv;   // 2^qk, 2^-qk
 };
+
+// Fixed-point image coordinate: d * 2^qk rounded to the nearest integer,
+// clamped to the int32 range (a particle far outside a non-periodic box).
+__host__ __device__ inline int to_fixed(double d, double qs) {
+    double v = rint(d * qs);
+    v = v > 2147483520.0 ? 2147483520.0 : (v < -2147483520.0 ? -2147483520.0 : v);
+    return static_cast<int>(v);
+}
 
 // Error-free transformation a + b = s + e (Knuth TwoSum), used to carry the
 // rounding error of the FP32 force sums.

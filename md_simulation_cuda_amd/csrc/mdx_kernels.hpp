@@ -10,8 +10,9 @@
 //   arrivals  uint32[C]     per-cell movers arriving this step (zero at rest)
 //   cs        uint32[C+1]   cell start: exclusive prefix of count over all cells
 //   sorted    uint32        particle ids in cell order, ascending id per cell
-//   img       float2/double2, in cell order: cell-centre-relative FP32
-//                           position, or the wrapped FP64 position
+//   img       int2/double2, in cell order: cell-centre-relative position in
+//                           fixed point (FP32 forms, Grid), or the wrapped
+//                           FP64 position (FP64 form)
 //   icls, scell             class and slot-local cell of each sorted slot
 //
 // One step = k_box (one workgroup per slot: the previous pair pass's
@@ -646,7 +647,7 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
         if (g.periodic_y) p.y = pbc_wrap(p.y, g.Ly);
     }
     IT *img = static_cast<IT *>(P.img);
-    if constexpr (std::is_same_v<IT, float2>) {
+    if constexpr (std::is_same_v<IT, int2>) {
         const uint32_t cy = lc / static_cast<uint32_t>(g.ncx);
         const uint32_t cx = lc - cy * static_cast<uint32_t>(g.ncx);
         double dx = p.x - (static_cast<double>(cx) + 0.5) * g.w;
@@ -657,11 +658,8 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
             if (dy > 0.5 * g.Ly) dy -= g.Ly;
             else if (dy < -0.5 * g.Ly) dy += g.Ly;
         }
-        const float2 im = make_float2(static_cast<float>(dx), static_cast<float>(dy));
+        const int2 im = make_int2(to_fixed(dx, g.qs), to_fixed(dy, g.qs));
         img[k] = im;
-        if (P.img_lo)   // the rounding remainder (dx - im is exact in FP64)
-            P.img_lo[k] = make_float2(static_cast<float>(dx - static_cast<double>(im.x)),
-                                      static_cast<float>(dy - static_cast<double>(im.y)));
         if (reb && sp.list_mode) P.img0[k] = im;   // build-time image, for the lean steps' check
     } else {
         img[k] = p;
@@ -738,7 +736,7 @@ __global__ void __launch_bounds__(kLeanThreads) k_lean(BatchPtrs P, int32_t pari
     // compiler could not move these loads ahead of them)
     const Grid g = sc.g;
     const bool check = moves && sc.list_valid;
-    float2 i0s[kLeanPer];
+    int2 i0s[kLeanPer];
 #pragma unroll
     for (int j = 0; j < kLeanPer; ++j)
         if (act[j] && check) i0s[j] = P.img0[k0 + static_cast<uint32_t>(base + j * kLeanThreads)];
@@ -762,9 +760,10 @@ __global__ void __launch_bounds__(kLeanThreads) k_lean(BatchPtrs P, int32_t pari
         }
     }
     if (first) sc.fallback[parity ^ 1] = 0;
-    const float thr = static_cast<float>(sp.skin_half2) * (1.0f - 1.0e-5f);
+    // thresholds in fixed-point units squared (q2f is a power of two)
+    const float thr = static_cast<float>(sp.skin_half2) * (1.0f - 1.0e-5f) * g.q2f;
     const float reach = 0.5f * static_cast<float>(2.0 * fmin(g.w, g.h) - sp.rc_max);
-    const float thr_v = reach * reach * (1.0f - 1.0e-5f);
+    const float thr_v = reach * reach * (1.0f - 1.0e-5f) * g.q2f;
     bool fb = false, bad = false;
 #pragma unroll
     for (int j = 0; j < kLeanPer; ++j) {
@@ -806,17 +805,15 @@ __global__ void __launch_bounds__(kLeanThreads) k_lean(BatchPtrs P, int32_t pari
         else if (dx < -0.5 * g.Lx) dx += g.Lx;
         if (dy > 0.5 * g.Ly) dy -= g.Ly;
         else if (dy < -0.5 * g.Ly) dy += g.Ly;
-        const float2 im = make_float2(static_cast<float>(dx), static_cast<float>(dy));
-        static_cast<float2 *>(P.img)[k] = im;
-        if (P.img_lo)
-            P.img_lo[k] = make_float2(static_cast<float>(dx - static_cast<double>(im.x)),
-                                      static_cast<float>(dy - static_cast<double>(im.y)));
+        const int2 im = make_int2(to_fixed(dx, g.qs), to_fixed(dy, g.qs));
+        static_cast<int2 *>(P.img)[k] = im;
         if (check) {
             // displacement since the build: the two images share the build
-            // cell's centre (FP32; the thresholds are lowered by 1e-5 relative
-            // so that a rounding near them errs towards the fallback)
-            const float2 i0 = i0s[j];
-            const float ex = im.x - i0.x, ey = im.y - i0.y;
+            // cell's centre (exact difference, squared in FP32; the
+            // thresholds are lowered by 1e-5 relative so that a rounding near
+            // them errs towards the fallback)
+            const int2 i0 = i0s[j];
+            const float ex = static_cast<float>(im.x - i0.x), ey = static_cast<float>(im.y - i0.y);
             const float d2 = __builtin_fmaf(ex, ex, ey * ey);
             fb = fb || d2 > thr;
             bad = bad || d2 > thr_v;
@@ -827,12 +824,12 @@ __global__ void __launch_bounds__(kLeanThreads) k_lean(BatchPtrs P, int32_t pari
 }
 
 // Neighbour-list entry (list mode): (partner slot << 3), so that entry &
-// kListOff is the byte offset of the partner's FP32 image; bit 2 set when the
+// kListOff is the byte offset of the partner's fixed-point image; bit 2 set when the
 // partner has the particle's own type class; the neighbour cell's offset
 // (ox, oy), -1..1 each, as the code c = (ox + 1) + 3 (oy + 1) in 0..8, whose
 // bits 0-1 sit in entry bits 30-31 and bits 2-3 in entry bits 0-1: rotated
 // left by 13 the entry holds c in bits 11-14, the byte offset of row c of a
-// [9][kBlock] table of float2 (list_shift_off).  Partner slots are below 2^27.
+// [9][kBlock] table of int2 (list_shift_off).  Partner slots are below 2^27.
 constexpr uint32_t kListOff = 0x3FFFFFF8u;
 constexpr uint32_t kListSame = 4u;
 // list rows a lane stages in LDS per pass of the list build
@@ -843,17 +840,15 @@ __device__ inline uint32_t list_entry(uint32_t k, int ox, int oy, bool same) {
     return (k << 3) | (same ? kListSame : 0u) | (c >> 2) | ((c & 3u) << 30);
 }
 __device__ inline uint32_t list_code(uint32_t v) { return (v >> 30) | ((v & 3u) << 2); }
-// byte offset of the entry's row in a [9][kBlock] float2 table
+// byte offset of the entry's row in a [9][kBlock] int2 table
 __device__ inline uint32_t list_shift_off(uint32_t v) { return __builtin_amdgcn_alignbit(v, v, 19) & 0x7800u; }
-static_assert(kBlock * sizeof(float2) == 0x800u, "list_shift_off assumes 2048-byte table rows");
+static_assert(kBlock * sizeof(int2) == 0x800u, "list_shift_off assumes 2048-byte table rows");
 // the offset's components from the code: oy + 1 = c / 3 (as (11 c) >> 5 for
 // c <= 8), ox + 1 = c - 3 (oy + 1)
-__device__ inline float list_oy(uint32_t v) {
-    return static_cast<float>(static_cast<int>((list_code(v) * 11u) >> 5) - 1);
-}
-__device__ inline float list_ox(uint32_t v) {
+__device__ inline int list_oyi(uint32_t v) { return static_cast<int>((list_code(v) * 11u) >> 5) - 1; }
+__device__ inline int list_oxi(uint32_t v) {
     const uint32_t c = list_code(v);
-    return static_cast<float>(static_cast<int>(c - 3u * ((c * 11u) >> 5)) - 1);
+    return static_cast<int>(c - 3u * ((c * 11u) >> 5)) - 1;
 }
 // Padding entry of slot s: the particle itself one cell to the side, at
 // distance w >= r_list > r_c, so it is never inside the cutoff.
@@ -885,13 +880,14 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
     const int li = (gb - sp.boff) * kBlock + tid;
     const bool active = li < sp.n;
     const Grid g = sc.g;
-    const float2 *img = static_cast<const float2 *>(P.img);
-    const float rl2 = static_cast<float>(sp.r_list * sp.r_list);
+    const int2 *img = static_cast<const int2 *>(P.img);
+    // r^2 thresholds in fixed-point units squared
+    const float rl2 = static_cast<float>(sp.r_list * sp.r_list) * g.q2f;
     const uint32_t cap = static_cast<uint32_t>(sp.list_cap);   // a multiple of 4
     const size_t stride = static_cast<size_t>(P.N);
     uint32_t s = 0;
     int cx = 0, cy = 0, ci = 0;
-    float2 pi = make_float2(0.0f, 0.0f);
+    int2 pi = make_int2(0, 0);
     if (active) {
         s = P.cs[sp.coff] + static_cast<uint32_t>(li);
         const uint32_t lc = P.scell[s];
@@ -903,7 +899,7 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
     // pairs closer than this may reach the guard distance before a rebuild
     // (with a margin for the FP32 difference between this and the pair form)
     const float rgd = sqrtf(sp.pt.r2_guard) + 2.0f * sqrtf(static_cast<float>(sp.skin_half2)) + 1.0e-3f;
-    const float rg2 = rgd * rgd;
+    const float rg2 = rgd * rgd * g.q2f;
     bool near = false;
     uint32_t rows = 0, cnt = 0;
     for (uint32_t base = 0;; base += kListStage) {
@@ -914,7 +910,7 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
                 if (yy < 0) yy += g.ncy;
                 else if (yy >= g.ncy) yy -= g.ncy;
                 const int rowbase = sp.coff + yy * g.ncx;
-                const float piy = pi.y - static_cast<float>(oy) * g.hf;
+                const int piy = pi.y - oy * g.hq;
                 // the row's three cells (periodic in x) as one run of candidates:
                 // cell -1 [a0, a0 + na), cell 0 [b0, ...), cell +1 [c0, ...)
                 const int xl = cx == 0 ? g.ncx - 1 : cx - 1;
@@ -928,13 +924,13 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
                 // image moved by minus the cell's offset and the entry's cell
                 // code bits (selects, no branches)
                 const uint32_t off1 = a0, off2 = b0 - na, off3 = c0 - nab;
-                const float pxa = pi.x + g.wf, pxc = pi.x - g.wf;
+                const int pxa = pi.x + g.wq, pxc = pi.x - g.wq;
                 const uint32_t cba = list_entry(0u, -1, oy, false), cbb = list_entry(0u, 0, oy, false),
                                cbc = list_entry(0u, 1, oy, false);
                 // four candidates per pass, their loads in flight together
                 for (uint32_t t = 0; t < nt; t += 4) {
                     uint32_t k[4], cb[4];
-                    float px[4];
+                    int px[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const uint32_t tt = t + static_cast<uint32_t>(u);
@@ -944,7 +940,7 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
                         px[u] = g2 ? pxc : (g1 ? pi.x : pxa);
                         cb[u] = g2 ? cbc : (g1 ? cbb : cba);
                     }
-                    float2 pj[4];
+                    int2 pj[4];
                     int cj[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
@@ -953,7 +949,8 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        const float dx = px[u] - pj[u].x, dy = piy - pj[u].y;
+                        const float dx = static_cast<float>(px[u] - pj[u].x);
+                        const float dy = static_cast<float>(piy - pj[u].y);
                         const float rr = __builtin_fmaf(dx, dx, dy * dy);
                         // 0 < r^2 (itself, and coincident particles, as in the
                         // cell pass, are not partners) and r^2 < r_list^2, as
@@ -1163,47 +1160,53 @@ __device__ inline void finalize_pair(int pmode, const double (&out)[3], const Sy
 }
 
 // Per-particle constants of the FP32 pair path: "s" entries for partners of
-// the particle's own class, "o" for the AB combination.
+// the particle's own class, "o" for the AB combination.  A displacement
+// arrives in fixed-point units (times 2^qk, Grid), so squared distances
+// (rc2, r2_guard, rcore2) are compared in those units squared, 1/r^2 is
+// brought back to units of sigma^-2 by q2 = 2^(2 qk), and the force factors
+// e24, fa, fb carry 2^-qk, so that their product with the fixed-point
+// displacement is the force.  All scalings are powers of two (exact).
 struct PairF32 {
     float rc2s, rc2o, sig2s, sig2o, e24s, e24o, e4s, e4o, r2_guard;
-    float fas, fao, fbs, fbo, uas, uao, ubs, ubo;   // PairTable fa, fb, ua, ub (list path)
+    float fas, fao, fbs, fbo, uas, uao, ubs, ubo;   // PairTable fa, fb (scaled), ua, ub (list path)
     float rcore2s, rcore2o;                         // PairTable rcore2f (compensated form)
+    float q2, qinv;                                 // 2^(2 qk), 2^-qk
 };
 
-// Repulsive-core pairs of the compensated FP32 form.  A pair whose FP32 r^2
-// is below rcore2f (r < kCoreRatio sigma_ij) is left out of the FP32 sums and
-// evaluated in FP64 instead, from the FP64 displacement that the image and
-// its rounding remainder (img + img_lo) give.  Such pairs are deferred: the
-// pass notes them (list rows in a 64-bit row mask, cell candidates in a
-// per-lane LDS queue of kCoreQ entries) and evaluates them after its FP32
+__device__ inline PairF32 make_pair_f32(const PairTable &pt, int ci, const Grid &g) {
+    PairF32 pc;
+    pc.rc2s = pt.rc2f[ci] * g.q2f;
+    pc.rc2o = pt.rc2f[2] * g.q2f;
+    pc.sig2s = pt.sig2f[ci];
+    pc.sig2o = pt.sig2f[2];
+    pc.e24s = pt.eps24f[ci] * g.qinvf;
+    pc.e24o = pt.eps24f[2] * g.qinvf;
+    pc.e4s = pt.eps4f[ci];
+    pc.e4o = pt.eps4f[2];
+    pc.r2_guard = pt.r2_guard * g.q2f;
+    pc.fas = pt.fa[ci] * g.qinvf;
+    pc.fao = pt.fa[2] * g.qinvf;
+    pc.fbs = pt.fb[ci] * g.qinvf;
+    pc.fbo = pt.fb[2] * g.qinvf;
+    pc.uas = pt.ua[ci];
+    pc.uao = pt.ua[2];
+    pc.ubs = pt.ub[ci];
+    pc.ubo = pt.ub[2];
+    pc.rcore2s = pt.rcore2f[ci] * g.q2f;
+    pc.rcore2o = pt.rcore2f[2] * g.q2f;
+    pc.q2 = g.q2f;
+    pc.qinv = g.qinvf;
+    return pc;
+}
+
+// Repulsive-core pairs of the compensated FP32 form.  A pair whose r^2 is
+// below rcore2f (r < kCoreRatio sigma_ij) is left out of the FP32 sums and
+// evaluated in FP64 from its exact fixed-point displacement.  Such pairs are
+// deferred: the pass notes them (list rows in a row mask, cell candidates in
+// a per-lane LDS queue of kCoreQ entries) and evaluates them after its FP32
 // loop, in the order noted, so the FP32 loop stays branch-free and the FP64
 // work runs once per noted pair instead of masked in every iteration.
 constexpr int kCoreQ = 9;
-
-// The particle's FP64 cell-relative position and what the core evaluation
-// needs of the grid.
-struct CoreCtx {
-    double px, py;
-    double w, h, Lx, Ly;
-    bool small_x, small_y;
-};
-
-__device__ inline CoreCtx make_core_from(float2 hi, float2 lo, const Grid &g) {
-    CoreCtx c;
-    c.px = static_cast<double>(hi.x) + static_cast<double>(lo.x);   // exact in FP64
-    c.py = static_cast<double>(hi.y) + static_cast<double>(lo.y);
-    c.w = g.w;
-    c.h = g.h;
-    c.Lx = g.Lx;
-    c.Ly = g.Ly;
-    c.small_x = g.small_x != 0;
-    c.small_y = g.periodic_y && g.small_y;
-    return c;
-}
-
-__device__ inline CoreCtx make_core(const BatchPtrs &P, uint32_t s, const Grid &g) {
-    return make_core_from(static_cast<const float2 *>(P.img)[s], P.img_lo[s], g);
-}
 
 // FP64 pair constants of one type pair (PairTable index).
 struct CoreConst {
@@ -1224,13 +1227,15 @@ __device__ inline CoreConst core_select(bool same, const CoreConst &a, const Cor
                      same ? a.vc : b.vc};
 }
 
-// FP64 force (and energy sums) of one pair from its FP64 displacement, added
-// to (gx, gy).  1/r^2 from the hardware FP64 reciprocal refined by two Newton
-// steps (relative error at the FP64 rounding level).  Only core pairs come
-// here, all well inside the cutoff.
+// FP64 force (and energy sums) of one pair from its fixed-point displacement
+// (dxi, dyi) * qinv, which converts exactly; added to (gx, gy).  1/r^2 from
+// the hardware FP64 reciprocal refined by two Newton steps (relative error at
+// the FP64 rounding level).  Only core pairs come here, well inside the
+// cutoff.
 template <bool OBS>
-__device__ inline void pair_core64(double dx, double dy, const CoreConst &k, double &gx, double &gy, ObsSum &ob,
-                                   bool want_obs) {
+__device__ inline void pair_core64(int dxi, int dyi, double qinv, const CoreConst &k, double &gx, double &gy,
+                                   ObsSum &ob, bool want_obs) {
+    const double dx = static_cast<double>(dxi) * qinv, dy = static_cast<double>(dyi) * qinv;
     const double r2 = __builtin_fma(dx, dx, dy * dy);
     if (!(r2 > 0.0)) return;   // coincident particles: no pair, as in the FP32 pass
     double y = __builtin_amdgcn_rcp(r2);
@@ -1252,64 +1257,46 @@ __device__ inline void pair_core64(double dx, double dy, const CoreConst &k, dou
     }
 }
 
-// FP64 displacement of a core pair: the particle's FP64 position moved by
-// minus the partner cell's offset (sx, sy) = (ox w, oy h), minus the partner's
-// FP64 position (image q plus remainder ql), as the FP32 pass forms it in FP32.
-// MIC: grids with cells narrower than the cutoff take the per-pair minimum
-// image there too (never in list mode, whose cells are at least r_list wide).
-template <bool MIC = true>
-__device__ inline void core_disp(const CoreCtx &c, float2 q, float2 ql, double sx, double sy, double &dx,
-                                 double &dy) {
-    dx = (c.px - sx) - (static_cast<double>(q.x) + static_cast<double>(ql.x));
-    dy = (c.py - sy) - (static_cast<double>(q.y) + static_cast<double>(ql.y));
-    if constexpr (MIC) {
-        if (c.small_x) dx -= c.Lx * rint(dx / c.Lx);
-        if (c.small_y) dy -= c.Ly * rint(dy / c.Ly);
-    }
+// Fixed-point displacement of the particle, whose image moved by minus the
+// partner cell's offset is (pix, piy), from the partner image pj; on small
+// grids (cells narrower than the cutoff, never in list mode) reduced to the
+// minimum image along that axis.
+__device__ inline void disp_fixed(int pix, int piy, int2 pj, const Grid &g, int &dxi, int &dyi) {
+    dxi = pix - pj.x;
+    dyi = piy - pj.y;
+    if (g.small_x) dxi -= g.Lxq * static_cast<int>(rintf(static_cast<float>(dxi) / static_cast<float>(g.Lxq)));
+    if (g.periodic_y && g.small_y)
+        dyi -= g.Lyq * static_cast<int>(rintf(static_cast<float>(dyi) / static_cast<float>(g.Lyq)));
 }
 
-// Core pair of the particle with the partner at cell-ordered slot k, whose
-// cell is (ox, oy) cells away ((sx, sy) = (ox w, oy h)).
-template <bool OBS>
-__device__ inline void core_pair(const BatchPtrs &P, const CoreCtx &c, uint32_t k, double sx, double sy,
-                                 const CoreConst &kc, double &gx, double &gy, ObsSum &ob, bool want_obs) {
-    const float2 q = static_cast<const float2 *>(P.img)[k];
-    const float2 ql = P.img_lo[k];
-    double dx, dy;
-    core_disp(c, q, ql, sx, sy, dx, dy);
-    pair_core64<OBS>(dx, dy, kc, gx, gy, ob, want_obs);
-}
-
-// FP32 pairs of particle (pix, piy) -- its cell-relative image already moved
-// by minus this neighbour cell's offset (ox, oy) -- with the candidates
-// [k0, k1) of one neighbour cell, in ascending slot order.  The cell's pair
-// forces are summed in FP32 and the cell total enters the particle's
-// accumulator once: a compensated (two-sum) add in the COMP32 form, an FP64
-// add in the ACC64 form (which adds every pair in FP64, as the reference
-// does).  Self is excluded by r^2 > 0.  UNI: all type pairs share sigma (rc
-// and sigma^2 need no select); the arithmetic is identical to the general
-// form.  COMP32: core pairs go to the caller's queue cq (one lane's column,
-// entry j at cq[j * kBlock + lane]: slot k and the offset code (ox + 2) +
-// 5 (oy + 2)) while it has room, else (and with LPP > 1) they are evaluated
-// at once.  ACC64: pairs closer than r2_guard, whose FP32 (sigma/r)^12 could
-// overflow, are evaluated in FP64 from the FP32 displacement.
+// FP32 pairs of a particle, whose fixed-point image moved by minus this
+// neighbour cell's offset (ox, oy) is (pix, piy), with the candidates [k0, k1)
+// of that cell, in ascending slot order.  The cell's pair forces are summed in
+// FP32 and the cell total enters the particle's accumulator once: a
+// compensated (two-sum) add in the COMP32 form, an FP64 add in the ACC64 form
+// (which adds every pair in FP64, as the reference does).  Self is excluded by
+// r^2 > 0.  UNI: all type pairs share sigma (rc and sigma^2 need no select);
+// the arithmetic is identical to the general form.  COMP32: core pairs go to
+// the caller's queue cq (one lane's column, entry j at cq[j * kBlock + lane]:
+// slot k and the offset code (ox + 2) + 5 (oy + 2)) while it has room, else
+// (and with LPP > 1) they are evaluated at once.  ACC64: pairs closer than
+// r2_guard, whose FP32 (sigma/r)^12 could overflow, are evaluated in FP64.
 template <int ACC, bool OBS, bool UNI, int LPP>
-__device__ __forceinline__ void pair_cell_f32(const BatchPtrs &P, uint32_t k0, uint32_t k1, float pix, float piy,
+__device__ __forceinline__ void pair_cell_f32(const BatchPtrs &P, uint32_t k0, uint32_t k1, int pix, int piy,
                                               int ci, int ox, int oy, const Grid &g, const PairTable &pt,
                                               const PairF32 &pc, ForceSum<ACC> &F, ObsSum &ob, bool want_obs,
-                                              const CoreCtx &cc, uint2 *cq, int &qn) {
-    const char *__restrict__ imgb = static_cast<const char *>(P.img);
+                                              uint2 *cq, int &qn) {
+    const int2 *__restrict__ img = static_cast<const int2 *>(P.img);
     const uint8_t *__restrict__ clsb = P.icls;
     float fxc = 0.0f, fyc = 0.0f;
     // with LPP lanes per particle, lane q of the group takes candidates
     // k0 + q, k0 + q + LPP, ...
     for (uint32_t k = k0 + (LPP > 1 ? threadIdx.x % LPP : 0u); k < k1; k += LPP) {
-        const float2 pj = *reinterpret_cast<const float2 *>(imgb + (k << 3));
+        const int2 pj = img[k];
         const int cj = clsb[k];
-        float dx = pix - pj.x;
-        float dy = piy - pj.y;
-        if (g.small_x) dx -= g.Lxf * rintf(dx / g.Lxf);
-        if (g.periodic_y && g.small_y) dy -= g.Lyf * rintf(dy / g.Lyf);
+        int dxi, dyi;
+        disp_fixed(pix, piy, pj, g, dxi, dyi);
+        const float dx = static_cast<float>(dxi), dy = static_cast<float>(dyi);
         const float rr = __builtin_fmaf(dx, dx, dy * dy);
         const bool same = cj == ci;
         const float rc2 = UNI ? pc.rc2s : (same ? pc.rc2s : pc.rc2o);
@@ -1319,36 +1306,27 @@ __device__ __forceinline__ void pair_cell_f32(const BatchPtrs &P, uint32_t k0, u
                 const float rco = UNI ? pc.rcore2s : (same ? pc.rcore2s : pc.rcore2o);
                 if (rr < rco) {
                     fp32 = false;
-                    if constexpr (LPP == 1) {
-                        if (qn < kCoreQ) {
-                            cq[qn * kBlock + threadIdx.x] =
-                                make_uint2(k, static_cast<uint32_t>((ox + 2) + 5 * (oy + 2)));
-                            ++qn;
-                        } else {   // more than kCoreQ core partners (overlapping starts only): FP64
-                                   // arithmetic on the FP32 displacement
-                            double fx, fy;
-                            pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), pair_index(ci, cj), pt,
-                                           fx, fy, ob, want_obs);
-                            F.addd(fx, fy);
-                        }
-                    } else {
-                        core_pair<OBS>(P, cc, k, ox * cc.w, oy * cc.h, core_const_sel(pt, pair_index(ci, cj)), F.gx,
-                                       F.gy, ob, want_obs);
+                    if (LPP == 1 && qn < kCoreQ) {
+                        cq[qn * kBlock + threadIdx.x] = make_uint2(k, static_cast<uint32_t>((ox + 2) + 5 * (oy + 2)));
+                        ++qn;
+                    } else {   // several lanes per particle, or more than kCoreQ core partners
+                        pair_core64<OBS>(dxi, dyi, g.qinv, core_const_sel(pt, pair_index(ci, cj)), F.gx, F.gy, ob,
+                                         want_obs);
                     }
                 }
             } else {
                 if (rr < pc.r2_guard) {
                     fp32 = false;
                     double fx, fy;
-                    pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), pair_index(ci, cj), pt, fx, fy,
-                                   ob, want_obs);
+                    pair_fp64<OBS>(static_cast<double>(dxi) * g.qinv, static_cast<double>(dyi) * g.qinv,
+                                   pair_index(ci, cj), pt, fx, fy, ob, want_obs);
                     F.addd(fx, fy);
                 }
             }
             if (fp32) {
                 const float sig2 = UNI ? pc.sig2s : (same ? pc.sig2s : pc.sig2o);
                 const float e24 = same ? pc.e24s : pc.e24o;
-                const float r2i = fast_rcp(rr);
+                const float r2i = fast_rcp(rr) * pc.q2;
                 const float sr2 = sig2 * r2i;
                 const float sr6 = sr2 * sr2 * sr2;
                 const float sr12 = sr6 * sr6;
@@ -1364,7 +1342,8 @@ __device__ __forceinline__ void pair_cell_f32(const BatchPtrs &P, uint32_t k0, u
                     if (want_obs) {
                         const float e4 = same ? pc.e4s : pc.e4o;
                         const float u = e4 * (sr12 - sr6);
-                        const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
+                        const double ddx = static_cast<double>(dx) * g.qinv, ddy = static_cast<double>(dy) * g.qinv;
+                        const double dfx = fx, dfy = fy;
                         ob.u += 0.5 * (static_cast<double>(u) - pt.vc[pair_index(ci, cj)]);
                         ob.w += 0.5 * (ddx * dfx + ddy * dfy);
                         ob.lam += 0.5 * (dfx * ddx - dfy * ddy);
@@ -1376,26 +1355,27 @@ __device__ __forceinline__ void pair_cell_f32(const BatchPtrs &P, uint32_t k0, u
     if constexpr (ACC == ACC_COMP32) F.add(fxc, fyc);
 }
 
-// The queued core pairs of a cell pass (pair_cell_f32), in queue order.
+// The queued core pairs of a cell pass (pair_cell_f32), in queue order; pi is
+// the particle's own fixed-point image.
 template <bool OBS>
-__device__ inline void cell_core_queue(const BatchPtrs &P, uint32_t s, int ci, const Grid &g, const PairTable &pt,
+__device__ inline void cell_core_queue(const BatchPtrs &P, int2 pi, int ci, const Grid &g, const PairTable &pt,
                                        const uint2 *cq, int qn, double &gx, double &gy, ObsSum &ob, bool want_obs) {
-    if (qn <= 0) return;
-    const CoreCtx cc = make_core(P, s, g);
+    const int2 *img = static_cast<const int2 *>(P.img);
     for (int j = 0; j < qn; ++j) {
         const uint2 e = cq[j * kBlock + threadIdx.x];
         const int ox = static_cast<int>(e.y % 5u) - 2, oy = static_cast<int>(e.y / 5u) - 2;
-        core_pair<OBS>(P, cc, e.x, ox * cc.w, oy * cc.h, core_const_sel(pt, pair_index(ci, P.icls[e.x])), gx, gy,
-                       ob, want_obs);
+        int dxi, dyi;
+        disp_fixed(pi.x - ox * g.wq, pi.y - oy * g.hq, img[e.x], g, dxi, dyi);
+        pair_core64<OBS>(dxi, dyi, g.qinv, core_const_sel(pt, pair_index(ci, P.icls[e.x])), gx, gy, ob, want_obs);
     }
 }
 
-// One neighbour-list entry of a particle whose image, moved by minus the
-// entry's neighbour-cell offset, is (psx, psy), with the partner image q: the
-// FP32 pair of pair_cell_f32 (the same displacement), its force evaluated in
-// powers of 1/r^2 as F/r = r^-2 r^-6 (fa r^-6 - fb).  The force is computed
-// unconditionally and kept only when the pair is inside the cutoff, so no
-// branch is taken per entry.  CHK selects what else is checked:
+// One neighbour-list entry of a particle whose fixed-point image, moved by
+// minus the entry's neighbour-cell offset, is (psx, psy), with the partner
+// image q: the FP32 pair of pair_cell_f32 (the same displacement), its force
+// evaluated in powers of 1/r^2 as F/r = r^-2 r^-6 (fa r^-6 - fb).  The force
+// is computed unconditionally and kept only when the pair is inside the
+// cutoff, so no branch is taken per entry.  CHK selects what else is checked:
 //   0  nothing;
 //   1  (ACC64) pairs closer than the FP64 guard are left out here; `flag`
 //      collects, as a bit per lane, whether one of them is also farther than
@@ -1404,17 +1384,17 @@ __device__ inline void cell_core_queue(const BatchPtrs &P, uint32_t s, int ci, c
 //   2  (COMP32) core pairs (r^2 < rcore2) are left out here; `flag` is set
 //      to whether this entry is one.
 template <int ACC, bool OBS, bool UNI, int CHK>
-__device__ __forceinline__ void list_pair_f32(uint32_t v, float2 q, float psx, float psy, int ci,
-                                              const PairTable &pt, const PairF32 &pc, float &fxc, float &fyc,
-                                              ForceSum<ACC> &F, ObsSum &ob, bool want_obs, bool &flag) {
-    const float dx = psx - q.x;
-    const float dy = psy - q.y;
+__device__ __forceinline__ void list_pair_f32(uint32_t v, int2 q, int psx, int psy, int ci, const PairTable &pt,
+                                              const PairF32 &pc, float &fxc, float &fyc, ForceSum<ACC> &F,
+                                              ObsSum &ob, bool want_obs, bool &flag) {
+    const float dx = static_cast<float>(psx - q.x);
+    const float dy = static_cast<float>(psy - q.y);
     const float rr = __builtin_fmaf(dx, dx, dy * dy);
     const bool same = (v & kListSame) != 0u;
     const float rc2 = UNI ? pc.rc2s : (same ? pc.rc2s : pc.rc2o);
     const float fa = same ? pc.fas : pc.fao;
     const float fb = same ? pc.fbs : pc.fbo;
-    const float r2i = fast_rcp(rr);
+    const float r2i = fast_rcp(rr) * pc.q2;
     const float r6i = r2i * r2i * r2i;
     const float t = __builtin_fmaf(fa, r6i, -fb) * r6i * r2i;
     bool in;
@@ -1439,7 +1419,9 @@ __device__ __forceinline__ void list_pair_f32(uint32_t v, float2 q, float psx, f
         if (want_obs && in) {
             const float fx = tmp * dx, fy = tmp * dy;
             const float uu = __builtin_fmaf(same ? pc.uas : pc.uao, r6i, -(same ? pc.ubs : pc.ubo)) * r6i;
-            const double ddx = dx, ddy = dy, dfx = fx, dfy = fy;
+            const double qi = pc.qinv;
+            const double ddx = static_cast<double>(dx) * qi, ddy = static_cast<double>(dy) * qi;
+            const double dfx = fx, dfy = fy;
             ob.u += 0.5 * (static_cast<double>(uu) - pt.vc[same ? ci : 2]);
             ob.w += 0.5 * (ddx * dfx + ddy * dfy);
             ob.lam += 0.5 * (dfx * ddx - dfy * ddy);
@@ -1451,19 +1433,19 @@ __device__ __forceinline__ void list_pair_f32(uint32_t v, float2 q, float psx, f
 // r2_guard), in entry order; ps[u] is the particle's image moved by minus
 // entry u's cell offset.
 template <int U, bool OBS>
-__device__ __forceinline__ void list_pairs_guarded(const uint32_t (&v)[U], const float2 (&q)[U],
-                                                   const float2 (&ps)[U], int ci, const PairTable &pt,
-                                                   const PairF32 &pc, double &gx, double &gy, ObsSum &ob,
-                                                   bool want_obs) {
+__device__ __forceinline__ void list_pairs_guarded(const uint32_t (&v)[U], const int2 (&q)[U], const int2 (&ps)[U],
+                                                   int ci, const PairTable &pt, const PairF32 &pc, double &gx,
+                                                   double &gy, ObsSum &ob, bool want_obs) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const float dx = ps[u].x - q[u].x;
-        const float dy = ps[u].y - q[u].y;
+        const int dxi = ps[u].x - q[u].x, dyi = ps[u].y - q[u].y;
+        const float dx = static_cast<float>(dxi), dy = static_cast<float>(dyi);
         const float rr = __builtin_fmaf(dx, dx, dy * dy);
         if (rr < pc.r2_guard && rr > 0.0f) {
+            const double qi = pc.qinv;
             double fx, fy;
-            pair_fp64<OBS>(static_cast<double>(dx), static_cast<double>(dy), (v[u] & kListSame) ? ci : 2, pt, fx, fy,
-                           ob, want_obs);
+            pair_fp64<OBS>(static_cast<double>(dxi) * qi, static_cast<double>(dyi) * qi,
+                           (v[u] & kListSame) ? ci : 2, pt, fx, fy, ob, want_obs);
             gx += fx;
             gy += fy;
         }
@@ -1498,11 +1480,11 @@ __device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, ui
         for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const uint32_t *>(row + u * rowb + soff);
     }
     for (uint32_t e = e0; e < e1; e += U) {
-        float2 q[U], ps[U];
+        int2 q[U], ps[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            q[u] = *reinterpret_cast<const float2 *>(imgb + (v[u] & kListOff));
-            ps[u] = *reinterpret_cast<const float2 *>(shb + (list_shift_off(v[u]) | lane8));
+            q[u] = *reinterpret_cast<const int2 *>(imgb + (v[u] & kListOff));
+            ps[u] = *reinterpret_cast<const int2 *>(shb + (list_shift_off(v[u]) | lane8));
         }
         const uint32_t en = e + U < e1 ? e + U : e;
         const char *row = lb + static_cast<size_t>(en) * rowb;
@@ -1547,165 +1529,29 @@ __device__ inline bool core_take(uint32_t &m, uint32_t base, uint32_t cnt, uint3
 }
 
 // The core rows of slot s's list marked by list_rows_f32 in m1 (rows [0, n1))
-// and m2 (rows [32, 32 + n2)), in ascending row order, evaluated in FP64.
-// Each pass takes up to four rows per lane and issues their loads together
-// (entry, then partner image and remainder), so a wave waits on one chain of
-// loads per pass; a wave runs as many passes as its lane with the most core
-// rows needs (usually one).
+// and m2 (rows [32, 32 + n2)), in ascending row order, evaluated in FP64 from
+// the exact fixed-point displacement (the shifted own image from the table
+// shb, the partner's image reloaded).  A wave runs as many passes as its lane
+// with the most core rows needs (usually none or one).
 template <bool OBS>
 __device__ inline void list_core_rows(const BatchPtrs &P, uint32_t s, int ci, const Grid &g, const PairTable &pt,
-                                      uint32_t m1, uint32_t n1, uint32_t m2, uint32_t n2, double &gx, double &gy,
-                                      ObsSum &ob, bool want_obs) {
+                                      const char *shb, uint32_t m1, uint32_t n1, uint32_t m2, uint32_t n2,
+                                      double &gx, double &gy, ObsSum &ob, bool want_obs) {
     if (!__any((m1 | m2) != 0u)) return;
-    constexpr int B = 1;
-    // the particle's image is loaded again here: the index passes through an
-    // empty asm, so the compiler cannot reuse the pass's first load of it and
-    // keep the image in registers through the list loop
-    uint32_t so = s;
-    __asm__ __volatile__("" : "+v"(so));
-    const CoreCtx cc = make_core_from(static_cast<const float2 *>(P.img)[so], P.img_lo[so], g);
     const CoreConst ks = core_const_sel(pt, ci), ko = core_const(pt, 2);
     const size_t stride = static_cast<size_t>(P.N);
-    const float2 *img = static_cast<const float2 *>(P.img);
+    const char *imgb = static_cast<const char *>(P.img);
+    const uint32_t lane8 = threadIdx.x * 8u;
     while (__any((m1 | m2) != 0u)) {
-        uint32_t e[B], v[B];
-        bool ok[B];
-#pragma unroll
-        for (int j = 0; j < B; ++j) {
-            e[j] = 0u;
-            ok[j] = core_take(m1, 0u, n1, e[j]) || core_take(m2, 32u, n2, e[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < B; ++j) v[j] = ok[j] ? P.nlist[static_cast<size_t>(e[j]) * stride + s] : 0u;
-        float2 q[B], ql[B];
-#pragma unroll
-        for (int j = 0; j < B; ++j) {
-            if (ok[j]) {
-                const uint32_t k = (v[j] & kListOff) >> 3;
-                q[j] = img[k];
-                ql[j] = P.img_lo[k];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < B; ++j) {
-            if (ok[j]) {
-                double dx, dy;
-                const uint32_t c = list_code(v[j]);
-                const int oy1 = static_cast<int>((c * 11u) >> 5);
-                const int ox1 = static_cast<int>(c) - 3 * oy1;
-                core_disp<false>(cc, q[j], ql[j], static_cast<double>(ox1 - 1) * cc.w,
-                                 static_cast<double>(oy1 - 1) * cc.h, dx, dy);
-                pair_core64<OBS>(dx, dy, core_select((v[j] & kListSame) != 0u, ks, ko), gx, gy, ob, want_obs);
-            }
+        uint32_t e = 0u;
+        if (core_take(m1, 0u, n1, e) || core_take(m2, 32u, n2, e)) {
+            const uint32_t v = P.nlist[static_cast<size_t>(e) * stride + s];
+            const int2 q = *reinterpret_cast<const int2 *>(imgb + (v & kListOff));
+            const int2 ps = *reinterpret_cast<const int2 *>(shb + (list_shift_off(v) | lane8));
+            pair_core64<OBS>(ps.x - q.x, ps.y - q.y, g.qinv, core_select((v & kListSame) != 0u, ks, ko), gx, gy, ob,
+                             want_obs);
         }
     }
-}
-
-// Core rows of one particle's list left for the caller (list_rows_f32 masks).
-struct CoreRows {
-    uint32_t m1, n1, m2, n2;
-};
-
-// Rows of a workgroup's core queue (wg_core_rows).
-constexpr int kCoreWG = 704;
-
-// Core rows of the workgroup's lists, shared out over all its lanes (one lane
-// per particle, no energy sums).  Each lane's list pass marked its core rows
-// below 32 in m1 (rows at and past 32, long lists only, in m2); a lane with
-// many of them would otherwise keep its wave, and the workgroup, waiting while
-// the other lanes idle.  After the list loops (a barrier) the workgroup's
-// [9][kBlock] LDS table, no longer needed, holds every lane's FP64 position
-// (image and remainder), a queue of (lane, row, class) entries in lane and
-// then row order, and one FP64 force per entry: lane t evaluates entries t,
-// t + kBlock, ..., and each lane then adds its own entries' forces in row
-// order, so the sums are those the lane alone would form.  A queue longer than
-// kCoreWG entries (dense overlapping starts) falls back to each lane's own
-// evaluation.  Every thread of the workgroup must call this (barriers); s0 is
-// the workgroup's first cell-ordered slot, active whether this thread has a
-// particle.
-__device__ inline void wg_core_rows(const BatchPtrs &P, uint32_t s0, bool active, int ci, const Grid &g,
-                                    const PairTable &pt, const CoreRows &cr, double &gx, double &gy,
-                                    float2 (*tab)[kBlock], uint32_t *wsum) {
-    const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, wid = tid >> 6;
-    const uint32_t s = s0 + static_cast<uint32_t>(tid);
-    const uint32_t m1 = active ? cr.m1 : 0u;
-    // the own image is read from the table (offset (0, 0)) before the barrier
-    // after which the table is overwritten
-    const float2 hi = active ? tab[4][tid] : make_float2(0.0f, 0.0f);
-    const float2 lo = active ? P.img_lo[s] : make_float2(0.0f, 0.0f);
-    ObsSum ob_unused;
-    if (!__syncthreads_or(m1 != 0u)) {
-        if (active) list_core_rows<false>(P, s, ci, g, pt, 0u, 0u, cr.m2, cr.n2, gx, gy, ob_unused, false);
-        return;
-    }
-    // exclusive scan of the lanes' entry counts over the workgroup
-    const uint32_t cnt = static_cast<uint32_t>(__popc(m1));
-    uint32_t inc = cnt;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
-        if (lane >= o) inc += t;
-    }
-    if (lane == 63) wsum[wid] = inc;
-    __syncthreads();
-    uint32_t base = inc - cnt, total = 0;
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-        const uint32_t t = wsum[w];
-        base += w < wid ? t : 0u;
-        total += t;
-    }
-    if (total > static_cast<uint32_t>(kCoreWG)) {   // the same decision in every lane
-        if (active) list_core_rows<false>(P, s, ci, g, pt, m1, cr.n1, cr.m2, cr.n2, gx, gy, ob_unused, false);
-        return;
-    }
-    char *lds = reinterpret_cast<char *>(&tab[0][0]);
-    float4 *own = reinterpret_cast<float4 *>(lds);
-    uint32_t *ent = reinterpret_cast<uint32_t *>(lds + kBlock * sizeof(float4));
-    double2 *res = reinterpret_cast<double2 *>(lds + kBlock * sizeof(float4) + kCoreWG * sizeof(uint32_t));
-    static_assert(kBlock * sizeof(float4) + kCoreWG * (sizeof(uint32_t) + sizeof(double2)) <=
-                      9 * kBlock * sizeof(float2),
-                  "core queue fits the table");
-    own[tid] = make_float4(hi.x, hi.y, lo.x, lo.y);
-    {
-        uint32_t m = m1, j = base;
-        while (m != 0u) {
-            const uint32_t pos = 31u - static_cast<uint32_t>(__builtin_clz(m));
-            m ^= 1u << pos;
-            const uint32_t e = cr.n1 - 1u - pos;
-            ent[j++] = (static_cast<uint32_t>(tid) << 8) | (e << 2) | static_cast<uint32_t>(ci);
-        }
-    }
-    __syncthreads();
-    const float2 *img = static_cast<const float2 *>(P.img);
-    const size_t stride = static_cast<size_t>(P.N);
-    for (uint32_t k = static_cast<uint32_t>(tid); k < total; k += kBlock) {
-        const uint32_t q = ent[k];
-        const uint32_t o = q >> 8, e = (q >> 2) & 63u, co = q & 3u;
-        const uint32_t v = P.nlist[static_cast<size_t>(e) * stride + s0 + o];
-        const uint32_t kp = (v & kListOff) >> 3;
-        const float2 qh = img[kp], ql = P.img_lo[kp];
-        const float4 ow = own[o];
-        const CoreCtx cc = make_core_from(make_float2(ow.x, ow.y), make_float2(ow.z, ow.w), g);
-        const uint32_t c = list_code(v);
-        const int oy1 = static_cast<int>((c * 11u) >> 5);
-        const int ox1 = static_cast<int>(c) - 3 * oy1;
-        double dx, dy;
-        core_disp<false>(cc, qh, ql, static_cast<double>(ox1 - 1) * cc.w, static_cast<double>(oy1 - 1) * cc.h, dx,
-                         dy);
-        double fx = 0.0, fy = 0.0;
-        const int pidx = (v & kListSame) ? static_cast<int>(co) : 2;
-        pair_core64<false>(dx, dy, CoreConst{pt.sig2[pidx], pt.eps24[pidx], 0.0, 0.0}, fx, fy, ob_unused, false);
-        res[k] = make_double2(fx, fy);
-    }
-    __syncthreads();
-#pragma unroll 2
-    for (uint32_t j = base; j < base + cnt; ++j) {
-        const double2 r = res[j];
-        gx += r.x;
-        gy += r.y;
-    }
-    if (active) list_core_rows<false>(P, s, ci, g, pt, 0u, 0u, cr.m2, cr.n2, gx, gy, ob_unused, false);
 }
 
 // FP32 pairs of cell-ordered slot s from its neighbour list (list mode):
@@ -1718,10 +1564,9 @@ __device__ inline void wg_core_rows(const BatchPtrs &P, uint32_t s0, bool active
 // the rows with one lane per particle, at once with several).  shift: the
 // workgroup's [9][kBlock] LDS table (LPP == 1).
 template <int ACC, bool OBS, bool UNI, int LPP>
-__device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, float2 pi, int ci, const Grid &g,
+__device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, int2 pi, int ci, const Grid &g,
                                               const PairTable &pt, const PairF32 &pc, ForceSum<ACC> &F,
-                                              ObsSum &ob, bool want_obs, float2 (*shift)[kBlock],
-                                              uint32_t *stash, CoreRows *defer) {
+                                              ObsSum &ob, bool want_obs, int2 (*shift)[kBlock], uint32_t *stash) {
     constexpr int U = 4;
     float fxc = 0.0f, fyc = 0.0f;
     double gx = 0.0, gy = 0.0;
@@ -1732,9 +1577,7 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, fl
         for (int oy = -1; oy <= 1; ++oy)
 #pragma unroll
             for (int ox = -1; ox <= 1; ++ox)
-                shift[(ox + 1) + 3 * (oy + 1)][threadIdx.x] =
-                    make_float2(__builtin_fmaf(-static_cast<float>(ox), g.wf, pi.x),
-                                __builtin_fmaf(-static_cast<float>(oy), g.hf, pi.y));
+                shift[(ox + 1) + 3 * (oy + 1)][threadIdx.x] = make_int2(pi.x - ox * g.wq, pi.y - oy * g.hq);
         const char *shb = reinterpret_cast<const char *>(&shift[0][0]);
         const uint32_t nr = __builtin_amdgcn_readfirstlane(P.ncount[s]);
         const uint32_t n = nr & ~kListGuard;
@@ -1752,11 +1595,7 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, fl
                 __asm__ __volatile__("" : "+v"(t));   // a fresh LDS read, not m1 kept in a register
                 m1 = stash[t];
             }
-            if (defer) {   // the caller evaluates them with the whole workgroup
-                *defer = CoreRows{m1, n1, m2, n > 32u ? n - 32u : 0u};
-            } else {
-                list_core_rows<OBS>(P, s, ci, g, pt, m1, n1, m2, n > 32u ? n - 32u : 0u, gx, gy, ob, want_obs);
-            }
+            list_core_rows<OBS>(P, s, ci, g, pt, shb, m1, n1, m2, n > 32u ? n - 32u : 0u, gx, gy, ob, want_obs);
         } else {
             if (nr & kListGuard)
                 list_rows_f32<ACC, OBS, UNI, 1>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy, m1);
@@ -1766,13 +1605,10 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, fl
     } else {
         (void)shift;
         (void)stash;
-        (void)defer;
         const char *imgb = static_cast<const char *>(P.img);
         const size_t stride = static_cast<size_t>(P.N);
         const uint32_t n = P.ncount[s] & ~kListGuard;
         const uint32_t pad = list_pad(s);
-        CoreCtx cc{};
-        if constexpr (ACC == ACC_COMP32) cc = make_core(P, s, g);
         uint32_t run = 0;
         for (uint32_t e = threadIdx.x % LPP; e < n; e += U * LPP) {
             uint32_t v[U];
@@ -1781,12 +1617,11 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, fl
                 const uint32_t ee = e + u * LPP;
                 v[u] = ee < n ? P.nlist[ee * stride + s] : pad;
             }
-            float2 q[U], ps[U];
+            int2 q[U], ps[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                q[u] = *reinterpret_cast<const float2 *>(imgb + (v[u] & kListOff));
-                ps[u] = make_float2(__builtin_fmaf(-list_ox(v[u]), g.wf, pi.x),
-                                    __builtin_fmaf(-list_oy(v[u]), g.hf, pi.y));
+                q[u] = *reinterpret_cast<const int2 *>(imgb + (v[u] & kListOff));
+                ps[u] = make_int2(pi.x - list_oxi(v[u]) * g.wq, pi.y - list_oyi(v[u]) * g.hq);
             }
             bool slow = false;
 #pragma unroll
@@ -1796,9 +1631,8 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, fl
                     list_pair_f32<ACC, OBS, UNI, 2>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
                                                     want_obs, core);
                     if (core)
-                        core_pair<OBS>(P, cc, (v[u] & kListOff) >> 3, static_cast<double>(list_ox(v[u])) * cc.w,
-                                       static_cast<double>(list_oy(v[u])) * cc.h,
-                                       core_const(pt, (v[u] & kListSame) ? ci : 2), gx, gy, ob, want_obs);
+                        pair_core64<OBS>(ps[u].x - q[u].x, ps[u].y - q[u].y, g.qinv,
+                                         core_const(pt, (v[u] & kListSame) ? ci : 2), gx, gy, ob, want_obs);
                     if (++run == 8u) {
                         F.add(fxc, fyc);
                         fxc = fyc = 0.0f;
@@ -1913,14 +1747,9 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
     int ci = 0;
     // one lane per particle, FP32 forms: the list pass's table of shifted
     // images, or the cell pass's queue of core pairs (kCoreQ uint2 per lane)
-    __shared__ float2 lds_tab[(LPP == 1 && !kFp64) ? 9 : 1][kBlock];
+    __shared__ int2 lds_tab[(LPP == 1 && !kFp64) ? 9 : 1][kBlock];
     __shared__ uint32_t lds_stash[(LPP == 1 && !kFp64) ? kBlock : 1];
-    __shared__ uint32_t lds_wsum[kWaves];
-    // the list pass of the compensated form (one lane per particle, no energy
-    // sums) leaves its core rows to the whole workgroup (wg_core_rows)
-    constexpr bool kWgCore = !kFp64 && ACC == ACC_COMP32 && LPP == 1 && !OBS;
-    CoreRows cr{0u, 0u, 0u, 0u};
-    static_assert(9 * sizeof(float2) >= kCoreQ * sizeof(uint2), "core queue fits the shift table");
+    static_assert(9 * sizeof(int2) >= kCoreQ * sizeof(uint2), "core queue fits the shift table");
     uint2 *cq = reinterpret_cast<uint2 *>(&lds_tab[0][0]);
     int qn = 0;
     if (li < sp.n) {
@@ -1932,30 +1761,7 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
         ci = P.icls[s];
         const IT pi = static_cast<const IT *>(P.img)[s];
         PairF32 pc{};
-        if constexpr (!kFp64) {
-            pc.rc2s = pt.rc2f[ci];
-            pc.rc2o = pt.rc2f[2];
-            pc.sig2s = pt.sig2f[ci];
-            pc.sig2o = pt.sig2f[2];
-            pc.e24s = pt.eps24f[ci];
-            pc.e24o = pt.eps24f[2];
-            pc.e4s = pt.eps4f[ci];
-            pc.e4o = pt.eps4f[2];
-            pc.r2_guard = pt.r2_guard;
-            pc.fas = pt.fa[ci];
-            pc.fao = pt.fa[2];
-            pc.fbs = pt.fb[ci];
-            pc.fbo = pt.fb[2];
-            pc.uas = pt.ua[ci];
-            pc.uao = pt.ua[2];
-            pc.ubs = pt.ub[ci];
-            pc.ubo = pt.ub[2];
-            pc.rcore2s = pt.rcore2f[ci];
-            pc.rcore2o = pt.rcore2f[2];
-        }
-        // own FP64 position for core pairs evaluated at once (LPP > 1)
-        CoreCtx cc{};
-        if constexpr (!kFp64 && ACC == ACC_COMP32 && LPP > 1) cc = make_core(P, s, g);
+        if constexpr (!kFp64) pc = make_pair_f32(pt, ci, g);
         bool by_list = false;
         if constexpr (!kFp64) {
             if (LEAN && (sc.fallback[parity] || !sc.list_on || !sc.list_built || sc.list_overflow)) {
@@ -1963,26 +1769,24 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
                 // (5 distinct cells per row and column, periodic; else the
                 // forces are not exact and the run reports it)
                 if ((g.ncx < 5 || g.ncy < 5 || !g.periodic_y) && li == 0) sc.violated = 1;
-                const char *imgb = static_cast<const char *>(P.img);
                 for (int oy = -2; oy <= 2; ++oy) {
                     int yy = cy + oy;
                     if (yy < 0) yy += g.ncy;
                     else if (yy >= g.ncy) yy -= g.ncy;
                     const int rowbase = sp.coff + yy * g.ncx;
-                    const float piy = pi.y - static_cast<float>(oy) * g.hf;
+                    const int piy = pi.y - oy * g.hq;
                     for (int ox = -2; ox <= 2; ++ox) {
                         int xx = cx + ox;
                         if (xx < 0) xx += g.ncx;
                         else if (xx >= g.ncx) xx -= g.ncx;
                         const uint32_t k0 = P.cs[rowbase + xx], k1 = P.cs[rowbase + xx + 1];
-                        pair_cell_f32<ACC, OBS, UNI, LPP>(P, k0, k1, pi.x - static_cast<float>(ox) * g.wf, piy, ci,
-                                                          ox, oy, g, pt, pc, F, ob, want_obs, cc, cq, qn);
+                        pair_cell_f32<ACC, OBS, UNI, LPP>(P, k0, k1, pi.x - ox * g.wq, piy, ci, ox, oy, g, pt, pc, F,
+                                                          ob, want_obs, cq, qn);
                     }
                 }
                 by_list = true;
             } else if (sc.list_on && !sc.list_overflow) {
-                pair_list_f32<ACC, OBS, UNI, LPP>(P, s, pi, ci, g, pt, pc, F, ob, want_obs, lds_tab, lds_stash,
-                                                  kWgCore ? &cr : nullptr);
+                pair_list_f32<ACC, OBS, UNI, LPP>(P, s, pi, ci, g, pt, pc, F, ob, want_obs, lds_tab, lds_stash);
                 by_list = true;
             }
         }
@@ -2015,29 +1819,17 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
                 pair_cell_f64<OBS>(img, P.icls, b0, b1, s, pi, ci, 0.0, sy, g, pt, F, ob, want_obs);
                 pair_cell_f64<OBS>(img, P.icls, c0, c1, s, pi, ci, sxr, sy, g, pt, F, ob, want_obs);
             } else {
-                const float piy = pi.y - static_cast<float>(oy) * g.hf;
-                pair_cell_f32<ACC, OBS, UNI, LPP>(P, a0, a1, pi.x + g.wf, piy, ci, -1, oy, g, pt, pc, F, ob, want_obs,
-                                                  cc, cq, qn);
-                pair_cell_f32<ACC, OBS, UNI, LPP>(P, b0, b1, pi.x, piy, ci, 0, oy, g, pt, pc, F, ob, want_obs, cc, cq,
-                                                  qn);
-                pair_cell_f32<ACC, OBS, UNI, LPP>(P, c0, c1, pi.x - g.wf, piy, ci, 1, oy, g, pt, pc, F, ob, want_obs,
-                                                  cc, cq, qn);
+                const int piy = pi.y - oy * g.hq;
+                pair_cell_f32<ACC, OBS, UNI, LPP>(P, a0, a1, pi.x + g.wq, piy, ci, -1, oy, g, pt, pc, F, ob, want_obs,
+                                                  cq, qn);
+                pair_cell_f32<ACC, OBS, UNI, LPP>(P, b0, b1, pi.x, piy, ci, 0, oy, g, pt, pc, F, ob, want_obs, cq, qn);
+                pair_cell_f32<ACC, OBS, UNI, LPP>(P, c0, c1, pi.x - g.wq, piy, ci, 1, oy, g, pt, pc, F, ob, want_obs,
+                                                  cq, qn);
             }
         }
         if constexpr (!kFp64 && ACC == ACC_COMP32 && LPP == 1)
-            cell_core_queue<OBS>(P, s, ci, g, pt, cq, qn, F.gx, F.gy, ob, want_obs);
+            cell_core_queue<OBS>(P, pi, ci, g, pt, cq, qn, F.gx, F.gy, ob, want_obs);
         if constexpr (LPP > 1) combine_lanes<LPP, kFp64 ? ACC_FP64 : ACC>(F, ob);
-    }
-    if constexpr (kWgCore) {
-        // the same condition as the list branch above, for every thread of
-        // the workgroup (a workgroup belongs to one slot)
-        const bool wg_list =
-            !(LEAN && (sc.fallback[parity] || !sc.list_on || !sc.list_built || sc.list_overflow)) && sc.list_on &&
-            !sc.list_overflow;
-        if (wg_list) {
-            const uint32_t s0 = P.cs[sp.coff] + static_cast<uint32_t>((gb - sp.boff) * kBlock);
-            wg_core_rows(P, s0, li < sp.n, ci, g, pt, cr, F.gx, F.gy, lds_tab, lds_wsum);
-        }
     }
     if (li < sp.n && lead) {
         const bool kicks = pmode == PM_NVE || pmode == PM_NPH || pmode == PM_PISTON || pmode == PM_NVT ||

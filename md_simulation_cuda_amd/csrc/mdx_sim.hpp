@@ -147,12 +147,7 @@ struct BatchPtrs {
     // same-class bit) and its row count ncount[s]
     uint32_t *key_list;
     double2 *x0;
-    float2 *img0;     // per cell-ordered slot: its FP32 image at the last list-mode build
-    // per cell-ordered slot, compensated FP32 form only (else null): the
-    // rounding remainder of the FP32 image, lo = fl32(d - img) with d the FP64
-    // cell-relative position, so img + lo holds d to about 2^-48 relative;
-    // the pair pass forms FP64 displacements of close pairs from it
-    float2 *img_lo;
+    int2 *img0;       // per cell-ordered slot: its fixed-point image at the last list-mode build
     uint32_t *nlist, *ncount;
     // storage order: particle data (x, v, a, cls, key, key_list, x0) sits at
     // storage index i; pid[i] is its original (load-order) global index and
@@ -195,6 +190,17 @@ __host__ __device__ inline Grid make_grid(double Lx, double Ly, double side, dou
     g.periodic_y = periodic_y;
     g.small_x = g.w < rc_max ? 1 : 0;
     g.small_y = g.h < rc_max ? 1 : 0;
+    int e = 0;
+    frexp(5.0 * (g.w > g.h ? g.w : g.h), &e);   // 5 max(w, h) < 2^e
+    g.qk = 31 - e;
+    g.qs = ldexp(1.0, g.qk);
+    g.qinv = ldexp(1.0, -g.qk);
+    g.q2f = static_cast<float>(ldexp(1.0, 2 * g.qk));
+    g.qinvf = static_cast<float>(g.qinv);
+    g.wq = to_fixed(g.w, g.qs);
+    g.hq = to_fixed(g.h, g.qs);
+    g.Lxq = to_fixed(Lx, g.qs);
+    g.Lyq = to_fixed(Ly, g.qs);
     return g;
 }
 

@@ -11,8 +11,9 @@ same frame and compare:
 - per-particle forces at the bench configuration (N = 1,048,576, 2:1 box,
   density 0.75) after 100 windowed steps: the accelerations the list pass
   stored, and a fresh cell-path pass, against the oracle's FP64 cell-list
-  forces at the same positions, max_i |dF_i| / rms |F| <= 1e-5 over the
-  particles without a pair in the cutoff's FP32 rounding band;
+  forces at the same positions, max_i |dF_i| / rms |F| <= 1e-5 and the rms
+  of |dF_i| / rms |F| <= 2e-6, both over the particles without a pair in the
+  cutoff's FP32 rounding band;
 - over 1000 steps at N = 4096 (square lattice, density 0.8) and on a 2:1
   slab of 3000 particles: xi(t) and K(t) while the trajectories coincide, then
   the drift of the Nose-Hoover conserved energy H = K + U + Q xi^2 / 2 +
@@ -72,9 +73,14 @@ def cutoff_band(frame, Lx, Ly, rc=2.5, band=1e-6):
 
 
 def force_errors(acc, acc_o, band):
+    """max and rms of |dF_i| / rms |F| over the particles outside the cutoff
+    band: a band pair that FP32 and FP64 put on opposite sides of the
+    (unshifted) cutoff moves its particles' forces by F(rc) / rms |F| ~ 3e-3,
+    which is the cutoff's discontinuity, not a rounding error, and a few such
+    flips would dominate the rms."""
     rms = float(np.sqrt((acc_o ** 2).sum(1).mean()))
-    err = np.sqrt(((acc - acc_o) ** 2).sum(1))
-    return float(err[~band].max() / rms), float(np.sqrt((err ** 2).mean()) / rms)
+    err = np.sqrt(((acc - acc_o) ** 2).sum(1))[~band]
+    return float(err.max() / rms), float(np.sqrt((err ** 2).mean()) / rms)
 
 
 def test_bench_config_forces_after_windowed_steps():
