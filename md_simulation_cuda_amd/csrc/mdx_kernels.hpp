@@ -854,8 +854,13 @@ __device__ inline int list_oxi(uint32_t v) {
 // distance w >= r_list > r_c, so it is never inside the cutoff.
 __device__ inline uint32_t list_pad(uint32_t s) { return list_entry(s, 1, 0, true); }
 // ncount[s] = rows | kListGuard when some pair of the wave's lists may come
-// closer than the FP64 guard distance before the next rebuild
+// closer than the FP64 guard distance before the next rebuild, | kListMixed
+// when some entry of the wave's lists pairs two type classes (else every
+// lane's partners have the lane's own class, and the pair pass takes the
+// lane's own-class constants without a per-entry select)
 constexpr uint32_t kListGuard = 0x80000000u;
+constexpr uint32_t kListMixed = 0x40000000u;
+constexpr uint32_t kListRows = ~(kListGuard | kListMixed);
 
 // Neighbour list of every cell-ordered slot of the slots that rebuild in list
 // mode: the partners within r_list in the 3x3 neighbour cells, in the pair
@@ -900,7 +905,7 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
     // (with a margin for the FP32 difference between this and the pair form)
     const float rgd = sqrtf(sp.pt.r2_guard) + 2.0f * sqrtf(static_cast<float>(sp.skin_half2)) + 1.0e-3f;
     const float rg2 = rgd * rgd * g.q2f;
-    bool near = false;
+    bool near = false, mixed = false;
     uint32_t rows = 0, cnt = 0;
     for (uint32_t base = 0;; base += kListStage) {
         cnt = 0;
@@ -959,6 +964,7 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
                         const bool in_run = t + static_cast<uint32_t>(u) < nt;
                         const bool acc = in_run && rb < __float_as_uint(rl2) - 1u;
                         near = near || (in_run && rb < __float_as_uint(rg2) - 1u);
+                        mixed = mixed || (acc && cj[u] != ci);
                         // unconditional store: a rejected candidate, or an entry
                         // outside this pass's rows, goes to the spare row
                         const uint32_t rel = cnt - base;
@@ -989,9 +995,9 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
         }
         if (base + kListStage >= rows) break;
     }
-    const bool guard = __any(near);
+    const bool guard = __any(near), mix = __any(mixed);
     if (active) {
-        P.ncount[s] = rows | (guard ? kListGuard : 0u);
+        P.ncount[s] = rows | (guard ? kListGuard : 0u) | (mix ? kListMixed : 0u);
         if (cnt > cap) sc.list_overflow = 1;   // this step's pair pass uses the cells instead
     }
     if (li == 0) sc.list_built = 1;
@@ -1383,14 +1389,16 @@ __device__ inline void cell_core_queue(const BatchPtrs &P, int2 pi, int ci, cons
 //      the values for r^2 >= 0), and the caller evaluates those in FP64;
 //   2  (COMP32) core pairs (r^2 < rcore2) are left out here; `flag` is set
 //      to whether this entry is one.
-template <int ACC, bool OBS, bool UNI, int CHK>
+// PURE: the wave's lists pair no two type classes (kListMixed clear), so
+// every entry takes the lane's own-class constants.
+template <int ACC, bool OBS, bool UNI, int CHK, bool PURE = false>
 __device__ __forceinline__ void list_pair_f32(uint32_t v, int2 q, int psx, int psy, int ci, const PairTable &pt,
                                               const PairF32 &pc, float &fxc, float &fyc, ForceSum<ACC> &F,
                                               ObsSum &ob, bool want_obs, bool &flag) {
     const float dx = static_cast<float>(psx - q.x);
     const float dy = static_cast<float>(psy - q.y);
     const float rr = __builtin_fmaf(dx, dx, dy * dy);
-    const bool same = (v & kListSame) != 0u;
+    const bool same = PURE || (v & kListSame) != 0u;
     const float rc2 = UNI ? pc.rc2s : (same ? pc.rc2s : pc.rc2o);
     const float fa = same ? pc.fas : pc.fao;
     const float fb = same ? pc.fbs : pc.fbo;
@@ -1461,8 +1469,8 @@ __device__ __forceinline__ void list_pairs_guarded(const uint32_t (&v)[U], const
 // neighbour-cell offset, read at the entry's code (list_shift_off).  CHK as
 // list_pair_f32; with CHK == 2 (at most 32 rows) each row shifts its core flag
 // into m from the right (m = 2 m + flag), so row e0 + j ends at bit
-// e1 - e0 - 1 - j.
-template <int ACC, bool OBS, bool UNI, int CHK>
+// e1 - e0 - 1 - j.  PURE as list_pair_f32.
+template <int ACC, bool OBS, bool UNI, int CHK, bool PURE>
 __device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, uint32_t e0, uint32_t e1,
                                               const char *shb, int ci, const PairTable &pt, const PairF32 &pc,
                                               ForceSum<ACC> &F, ObsSum &ob, bool want_obs, float &fxc, float &fyc,
@@ -1473,12 +1481,24 @@ __device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, ui
     const size_t rowb = static_cast<size_t>(P.N) * 4u;   // bytes per row
     const uint32_t soff = s * 4u;
     const uint32_t lane8 = threadIdx.x * 8u;
+    // a row's base address is the same for the whole wave: a raw buffer
+    // descriptor over the row (base read back from the first lane, so it is
+    // uniform to the compiler and lives in scalar registers; extent one row,
+    // N * 4 < 2^29 bytes), loaded at the lane's 32-bit offset, so no per-lane
+    // 64-bit address is formed.  Descriptor word 3 0x00020000: 32-bit data
+    // format, the value gfx9-family untyped buffer loads take.
+    auto row_load = [&](uint32_t e) {
+        const uint64_t a = reinterpret_cast<uint64_t>(lb) + static_cast<uint64_t>(e) * rowb;
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a));
+        const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a >> 32));
+        char *row = reinterpret_cast<char *>((static_cast<uint64_t>(hi) << 32) | lo);
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc(row, 0, static_cast<int>(rowb), 0x00020000);
+        return static_cast<uint32_t>(__builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(soff), 0, 0));
+    };
     uint32_t v[U];
-    {
-        const char *row = lb + static_cast<size_t>(e0) * rowb;
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const uint32_t *>(row + u * rowb + soff);
-    }
+    for (int u = 0; u < U; ++u) v[u] = row_load(e0 + u);
     for (uint32_t e = e0; e < e1; e += U) {
         int2 q[U], ps[U];
 #pragma unroll
@@ -1487,21 +1507,20 @@ __device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, ui
             ps[u] = *reinterpret_cast<const int2 *>(shb + (list_shift_off(v[u]) | lane8));
         }
         const uint32_t en = e + U < e1 ? e + U : e;
-        const char *row = lb + static_cast<size_t>(en) * rowb;
         uint32_t vn[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) vn[u] = *reinterpret_cast<const uint32_t *>(row + u * rowb + soff);
+        for (int u = 0; u < U; ++u) vn[u] = row_load(en + u);
         bool slow = false;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if constexpr (CHK == 2) {
                 bool cf = false;
-                list_pair_f32<ACC, OBS, UNI, CHK>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
-                                                  want_obs, cf);
+                list_pair_f32<ACC, OBS, UNI, CHK, PURE>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
+                                                        want_obs, cf);
                 m = m + m + (cf ? 1u : 0u);   // one add with carry-in
             } else {
-                list_pair_f32<ACC, OBS, UNI, CHK>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
-                                                  want_obs, slow);
+                list_pair_f32<ACC, OBS, UNI, CHK, PURE>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
+                                                        want_obs, slow);
             }
         }
         if constexpr (CHK == 1) {
@@ -1580,17 +1599,23 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, in
                 shift[(ox + 1) + 3 * (oy + 1)][threadIdx.x] = make_int2(pi.x - ox * g.wq, pi.y - oy * g.hq);
         const char *shb = reinterpret_cast<const char *>(&shift[0][0]);
         const uint32_t nr = __builtin_amdgcn_readfirstlane(P.ncount[s]);
-        const uint32_t n = nr & ~kListGuard;
+        const uint32_t n = nr & kListRows;
+        const bool pure = (nr & kListMixed) == 0u;   // uniform
         uint32_t m1 = 0u, m2 = 0u;
         if constexpr (ACC == ACC_COMP32) {
             // core flags of rows [0, 32) in m1, of rows [32, n) in m2; m1 waits
             // in LDS while the rows past 32 (long lists only) run
             const uint32_t n1 = n < 32u ? n : 32u;
-            list_rows_f32<ACC, OBS, UNI, 2>(P, s, 0u, n1, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy, m1);
+            if (pure)
+                list_rows_f32<ACC, OBS, UNI, 2, true>(P, s, 0u, n1, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy,
+                                                      m1);
+            else
+                list_rows_f32<ACC, OBS, UNI, 2, false>(P, s, 0u, n1, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx,
+                                                       gy, m1);
             if (n > 32u) {
                 stash[threadIdx.x] = m1;
-                list_rows_f32<ACC, OBS, UNI, 2>(P, s, 32u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy,
-                                                m2);
+                list_rows_f32<ACC, OBS, UNI, 2, false>(P, s, 32u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx,
+                                                       gy, m2);
                 uint32_t t = threadIdx.x;
                 __asm__ __volatile__("" : "+v"(t));   // a fresh LDS read, not m1 kept in a register
                 m1 = stash[t];
@@ -1598,16 +1623,21 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, in
             list_core_rows<OBS>(P, s, ci, g, pt, shb, m1, n1, m2, n > 32u ? n - 32u : 0u, gx, gy, ob, want_obs);
         } else {
             if (nr & kListGuard)
-                list_rows_f32<ACC, OBS, UNI, 1>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy, m1);
+                list_rows_f32<ACC, OBS, UNI, 1, false>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy,
+                                                       m1);
+            else if (pure)
+                list_rows_f32<ACC, OBS, UNI, 0, true>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy,
+                                                      m1);
             else
-                list_rows_f32<ACC, OBS, UNI, 0>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy, m1);
+                list_rows_f32<ACC, OBS, UNI, 0, false>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy,
+                                                       m1);
         }
     } else {
         (void)shift;
         (void)stash;
         const char *imgb = static_cast<const char *>(P.img);
         const size_t stride = static_cast<size_t>(P.N);
-        const uint32_t n = P.ncount[s] & ~kListGuard;
+        const uint32_t n = P.ncount[s] & kListRows;
         const uint32_t pad = list_pad(s);
         uint32_t run = 0;
         for (uint32_t e = threadIdx.x % LPP; e < n; e += U * LPP) {
