@@ -337,25 +337,52 @@ def run_ensemble(args):
     tm = b.timing_read()
     b.timing_enable(False)
     b.close()
-    # per-replica launches: each replica its own handle, stream and step graph
-    sims = [abi.Simulation(abi.make_params(Lx, Ly, precision=prec, **kw), f) for f, Lx, Ly, kw in reps]
-    if tune is not None:   # the batch's kernels; no rebuild windows (a host sync per window and handle)
+    # per-replica launches: each replica its own handle, stream and step
+    # graphs; once with the batch's tuning, rebuild windows included (windows
+    # run on across calls and a call does not wait for the GPU), once with
+    # the same lists but no windows
+    def per_replica(window):
+        sims = [abi.Simulation(abi.make_params(Lx, Ly, precision=prec, **kw), f) for f, Lx, Ly, kw in reps]
+        if tune is not None:
+            for s in sims:
+                s.set_tuning(tune["cell_scale"], tune["lanes_per_particle"], tune["skin"], window)
+        w = window if window > 1 else 12
+        k = -(-args.per_replica_steps // w) * w   # whole windows
+        for s in sims:   # two whole windows: every graph a window launches is captured
+            s.step(abi.NVT, 2 * w if window > 1 else 5)
         for s in sims:
-            s.set_tuning(tune["cell_scale"], tune["lanes_per_particle"], tune["skin"], 0)
-    for s in sims:
-        s.step(abi.NVT, 5)
-    for s in sims:
-        s.sync()
-    k = args.per_replica_steps
-    t0 = time.perf_counter()
-    for s in sims:
-        s.step(abi.NVT, k)
-    for s in sims:
-        s.sync()
-    wall_pr = time.perf_counter() - t0
-    for s in sims:
-        s.close()
-    value_pr = total_n * k / wall_pr
+            s.sync()
+        # round robin, one window per call: a call under 64 steps does not
+        # wait for the GPU, so the replicas' streams overlap
+        t0 = time.perf_counter()
+        for _ in range(k // w):
+            for s in sims:
+                s.step(abi.NVT, w)
+        for s in sims:
+            s.sync()
+        wall_pr = time.perf_counter() - t0
+        for s in sims:
+            s.close()
+        return {"value": total_n * k / wall_pr, "steps": k, "wall_s": wall_pr, "window": window}
+
+    pr_same = per_replica(tune["window"] if tune is not None else 0)
+    pr_nowin = per_replica(0) if tune is not None and tune["window"] > 1 else pr_same
+    # the FP64 oracle on the host cores: NVT steps of a sample of the replicas
+    cpu = None
+    if not args.no_cpu:
+        threads = host_threads()
+        atoms_steps, secs, used = 0.0, 0.0, 0
+        budget = args.cpu_seconds
+        for f, Lx, Ly, kw in reps:
+            v, st, el, _ = oracle_rate(f, Lx, Ly, kw, budget / 8.0, threads)
+            atoms_steps += len(f) * st
+            secs += el
+            used += 1
+            if secs >= budget:
+                break
+        cpu = {"value": atoms_steps / secs, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle (cell list, OpenMP {threads} threads), NVT steps of the first {used} "
+                         f"replicas, about {budget / 8.0:.1f} s each, {secs:.1f} s in all"}
     out = {
         "metric": "ensemble atom-steps/s (256 NVT replicas, one batch)",
         "value": value,
@@ -373,10 +400,11 @@ def run_ensemble(args):
                                "T U[0.4,1], eps_AB U[0.3,0.9], rho U[0.70,0.85], dt 0.005, Q 10",
                    "replicas": len(reps), "total_atoms": total_n},
         "pair_avg_launch_ms": tm.force_ms / max(1, tm.force_launches),
-        "per_replica_launches": {"value": value_pr, "steps": k, "wall_s": wall_pr},
-        "speedup_vs_per_replica": value / value_pr,
+        "per_replica_launches": {"same_tuning": pr_same, "no_windows": pr_nowin},
+        "speedup_vs_per_replica": value / pr_same["value"],
+        "speedup_vs_per_replica_no_windows": value / pr_nowin["value"],
         "tuning": tune,
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -517,7 +545,32 @@ def run_fire_nph(args):
     sim.sync()
     t_nph = time.perf_counter() - t0
     ob = sim.observables()
+    f_end = sim.collect()
     sim.close()
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        threads = host_threads()
+        o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, n_threads=threads, **kw), f_end)
+        o.compute_forces()
+        nsteps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < args.cpu_seconds * 0.6:
+            o.step_nph(1, 1.0)
+            nsteps += 1
+        el = time.perf_counter() - t0
+        o.close()
+        o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, n_threads=threads, **kw), frame)
+        t1 = time.perf_counter()
+        ores, _, _ = o.fire(0.005, 0.01, 0.1, 1.1, 0.5, 5, max_step=0.1, max_steps=50, force_tol=0.0)
+        el_f = time.perf_counter() - t1
+        o.close()
+        cpu = {"value": n * nsteps / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle (cell list, OpenMP {threads} threads), {nsteps} NPH steps from the "
+                         f"GPU run's final state in {el:.1f} s",
+               "fire_iterations_per_s": ores.steps / el_f,
+               "fire_sample": f"{ores.steps} FIRE iterations (dt_max 0.01) from the same random start in "
+                              f"{el_f:.1f} s"}
     out = {
         "metric": "NPH atom-steps/s after FIRE (N=262144)",
         "value": n * steps / t_nph,
@@ -541,7 +594,7 @@ def run_fire_nph(args):
                  "max_force": res.max_force, "pe_per_particle": res.potential_energy / n, "wall_s": t_fire,
                  "iterations_per_s": res.steps / t_fire},
         "nph_final": {"P": ob.P, "Lx": ob.Lx, "Ly": ob.Ly, "K_per_atom": ob.K / n, "U_per_atom": ob.U / n},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
