@@ -43,6 +43,10 @@ public:
     std::vector<int> get_N_profile(int n_bins_per_rank);
     // per slab (n_A - n_B) / (n_A + n_B), 0 for an empty slab (src/md_env.cu:4086-4104)
     std::vector<double> get_density_profile(int n_bins_per_rank);
+    // Pxx, Pyy, Pxy in n_bins_local slabs along y, each divided by the slab's
+    // area (src/md_env.cu:3821-3918 with one rank: 3 * n_bins_local values;
+    // empty for n_bins_local <= 0)
+    std::vector<double> get_pressure_profile(int n_bins_local);
     double get_Lx() const { return cfg_manager.config.box_w_global; }
     double get_Ly() const { return cfg_manager.config.box_h_global; }
 

@@ -208,6 +208,17 @@ int mdx_sync(mdx_sim *sim);
  * domain: the reference with one rank.  n_bins in [1, 8192]; counts holds
  * 2 n_bins values. */
 int mdx_N_profile(mdx_sim *sim, int32_t n_bins, int32_t *counts);
+/* Pressure-tensor profile in n_bins slabs of the box along y: out[0, n) Pxx,
+ * out[n, 2n) Pyy, out[2n, 3n) Pxy (n = n_bins), each slab's sum over its
+ * particles of m v_a v_b and, over the ordered pairs inside the cutoff whose
+ * midpoint y_i - d_y / 2 falls in it, of 1/2 d_a F_b, divided by the slab's
+ * area Lx Ly / n.  Pair forces in FP64 whatever the precision; sums in fixed
+ * point (2^-32), so the result does not depend on summation order.  Replaces
+ * get_pressure_profile (src/md_env.cu:3821-3918, kernel
+ * src/md_cuda_common.cu:879-1030) on one domain.  y is periodic unless the
+ * piston walls are active (the reference always wraps y here).  Rebuilds the
+ * cell list: the next step starts a new rebuild window.  n_bins in [1, 8192]. */
+int mdx_pressure_profile(mdx_sim *sim, int32_t n_bins, double *out);
 
 /* No counterpart in the reference.  mdx_N_profile for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving or

@@ -203,6 +203,7 @@ _SIGNATURES = {
     "mdx_batch_timing_read": (C.c_int, [_P, C.POINTER(Timing)]),
     "mdx_sync": (C.c_int, [_P]),
     "mdx_N_profile": (C.c_int, [_P, C.c_int32, _P]),
+    "mdx_pressure_profile": (C.c_int, [_P, C.c_int32, _P]),
     "mdx_N_profile_noalloc": (C.c_int, [_P, C.c_int32, _P]),
     "mdx_middle_reflect": (C.c_int, [_P]),
     "mdx_cell_list": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P, _P, _P,
@@ -367,6 +368,12 @@ class Simulation:
         """Counts per x slab: type 0 in [0, n_bins), other types after."""
         out = np.zeros(2 * n_bins, np.int32)
         check(self._lib.mdx_N_profile(self._h, n_bins, _ptr(out)))
+        return out
+
+    def pressure_profile(self, n_bins: int):
+        """Pxx, Pyy, Pxy per y slab, as one array of 3 n_bins values."""
+        out = np.zeros(3 * n_bins, np.float64)
+        check(self._lib.mdx_pressure_profile(self._h, n_bins, _ptr(out)))
         return out
 
     def middle_reflect(self) -> None:

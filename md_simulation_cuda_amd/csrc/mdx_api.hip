@@ -247,6 +247,11 @@ int mdx_N_profile(mdx_sim *sim, int32_t n_bins, int32_t *counts) {
     return sim->b.n_profile(0, n_bins, counts);
 }
 
+int mdx_pressure_profile(mdx_sim *sim, int32_t n_bins, double *out) {
+    CHECK_HANDLE(sim);
+    return sim->b.pressure_profile(0, n_bins, out);
+}
+
 int mdx_middle_reflect(mdx_sim *sim) {
     CHECK_HANDLE(sim);
     return sim->b.middle_reflect(0);

@@ -1194,6 +1194,49 @@ int Batch::n_profile(int r, int nb, int32_t *out) {
     return MDX_OK;
 }
 
+// Pressure-tensor profile of slot r in nb slabs along y (get_pressure_profile,
+// src/md_env.cu:3821-3918): a fresh cell list at the current positions (y
+// periodic unless the slot's piston walls are active; the next step starts a
+// new rebuild window), the fixed-point sums of k_pressure_profile, then each
+// slab's sum divided by its area Lx * Ly / nb.  out: 3 nb values, Pxx for the
+// nb slabs, then Pyy, then Pxy.
+int Batch::pressure_profile(int r, int nb, double *out) {
+    if (r < 0 || r >= static_cast<int>(slots.size()) || !slots[r].loaded)
+        return set_error(MDX_ERR_STATE, "slot not loaded");
+    if (nb < 1 || nb > kProfileMaxBins) return set_error(MDX_ERR_INVALID, "n_bins must be in [1, 8192]");
+    if (!out) return set_error(MDX_ERR_INVALID, "null output");
+    MDX_HIP(hipSetDevice(device));
+    if (viol_unchecked) MDX_TRY(check_violations());
+    MDX_TRY(rebuild_cells({r}, slots[r].piston_active ? 0 : 1));
+    DevScalars h;
+    MDX_TRY(read_scalars(r, h));
+    const size_t bytes = static_cast<size_t>(3 * nb) * sizeof(unsigned long long);
+    unsigned long long *d = nullptr;
+    MDX_HIP(hipMalloc(&d, bytes));
+    hipError_t e = hipMemsetAsync(d, 0, bytes, stream);
+    const int64_t n = slots[r].n;
+    constexpr int kLdsBins = 2048;   // 48 KB of LDS histogram
+    if (e == hipSuccess && n > 0) {
+        const int grid = static_cast<int>((n + kBlock - 1) / kBlock);
+        if (nb <= kLdsBins)
+            k_pressure_profile<true><<<grid, kBlock, bytes, stream>>>(ptrs, r, nb, d);
+        else
+            k_pressure_profile<false><<<grid, kBlock, 0, stream>>>(ptrs, r, nb, d);
+        e = hipGetLastError();
+    }
+    std::vector<long long> hs(static_cast<size_t>(3 * nb));
+    if (e == hipSuccess) e = hipMemcpyAsync(hs.data(), d, bytes, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(d);
+    MDX_HIP(e);
+    const double area = h.g.Lx * h.g.Ly / static_cast<double>(nb);
+    for (int k = 0; k < 3 * nb; ++k) {
+        const double v = static_cast<double>(hs[k]) / 4294967296.0;
+        out[k] = area > 0.0 ? v / area : v;
+    }
+    return MDX_OK;
+}
+
 // middle_reflect_LG on slot r's current positions (no force pass, as in the
 // reference); the next step rebuilds the cell list from the moved particles.
 int Batch::middle_reflect(int r) {

@@ -164,6 +164,7 @@ struct Batch {
     int replay(bool obs, int mode, int count, int parity = 0);
     int capture_steps(hipGraphExec_t &ge, bool obs, int mode, int parity, int steps);
     int n_profile(int r, int nb, int32_t *out);
+    int pressure_profile(int r, int nb, double *out);   // [Pxx | Pyy | Pxy] per slab along y
     int n_profile_noalloc(int r, int nb, int32_t *out);   // counters in the device's fixed scratch array
     int middle_reflect(int r);
     int run_windows(int count);

@@ -2033,6 +2033,130 @@ __global__ void __launch_bounds__(kBlock) k_walls(BatchPtrs P, const int32_t *wa
     block_partials<2>(vals, 0u, P.partials, P.TB, gb);
 }
 
+// Pressure-tensor profile in nb slabs along y of slot r (the reference's
+// local_pressure_tensor_profile_kernel, src/md_cuda_common.cu:879-1030, on one
+// periodic domain): particle i adds m v_a v_b to the slab of its y; each
+// ordered pair inside the cutoff adds 1/2 d_a F_b (d = r_i - r_j, F the FP64
+// pair force on i) to the slab of the pair's midpoint y_i - d_y / 2 (wrapped
+// when y is periodic, else clamped into [0, Ly)); slab = int(y / Ly * nb)
+// clamped to [0, nb).  The pairs come from the slot's current cell list (3x3
+// cells, displacements formed as in pair_cell_f64 from the wrapped FP64
+// positions), so the FP64 force of every pair is the oracle's.  Sums are
+// int64 fixed point, units of 2^-32 (each contribution rounded to the
+// nearest unit, saturating at 2^62), so their value does not depend on the
+// order of the atomics.  acc: [3][nb] (xx, yy, xy); LDS: [3][nb] when LDS_HIST.
+constexpr double kProfScale = 4294967296.0;   // 2^32
+
+__device__ inline unsigned long long prof_fixed(double v) {
+    double q = rint(v * kProfScale);
+    q = q > 4.6e18 ? 4.6e18 : (q < -4.6e18 ? -4.6e18 : q);
+    return static_cast<unsigned long long>(static_cast<long long>(q));
+}
+
+__device__ inline int prof_bin(double y, double Ly, int nb, bool periodic) {
+    if (periodic) {
+        y = pbc_wrap(y, Ly);
+    } else if (y < 0.0) {
+        y = 0.0;
+    } else if (y >= Ly && Ly > 0.0) {
+        y = Ly * (1.0 - 1e-12);
+    }
+    const double q = Ly > 0.0 ? y / Ly : 0.0;
+    int b = static_cast<int>(q * static_cast<double>(nb));
+    return b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+}
+
+template <bool LDS_HIST>
+__global__ void __launch_bounds__(kBlock) k_pressure_profile(BatchPtrs P, int32_t r, int32_t nb,
+                                                             unsigned long long *acc) {
+    extern __shared__ unsigned long long phist[];
+    const SysParams &sp = P.sp[r];
+    const Grid g = P.sc[r].g;
+    const PairTable &pt = sp.pt;
+    const bool py = g.periodic_y != 0;
+    unsigned long long *h = LDS_HIST ? phist : acc;
+    if constexpr (LDS_HIST) {
+        for (int j = threadIdx.x; j < 3 * nb; j += kBlock) phist[j] = 0ull;
+        __syncthreads();
+    }
+    const int li = blockIdx.x * kBlock + threadIdx.x;
+    if (li < sp.n) {
+        const uint32_t s = P.cs[sp.coff] + static_cast<uint32_t>(li);
+        const uint32_t id = P.sorted[s];
+        const int ci = P.cls[id];
+        double2 pi = P.x[id];
+        pi.x = pbc_wrap(pi.x, g.Lx);
+        if (py) pi.y = pbc_wrap(pi.y, g.Ly);
+        const double2 vi = P.v[id];
+        const double m = pt.m[ci];
+        {
+            const int b = prof_bin(pi.y, g.Ly, nb, py);
+            atomicAdd(&h[b], prof_fixed(m * vi.x * vi.x));
+            atomicAdd(&h[nb + b], prof_fixed(m * vi.y * vi.y));
+            atomicAdd(&h[2 * nb + b], prof_fixed(m * vi.x * vi.y));
+        }
+        const uint32_t lc = P.scell[s];
+        const int cx = static_cast<int>(lc % static_cast<uint32_t>(g.ncx));
+        const int cy = static_cast<int>(lc / static_cast<uint32_t>(g.ncx));
+        for (int oy = -1; oy <= 1; ++oy) {
+            int yy = cy + oy;
+            double sy = 0.0;
+            if (py) {
+                if (yy < 0) {
+                    yy += g.ncy;
+                    sy = -g.Ly;
+                } else if (yy >= g.ncy) {
+                    yy -= g.ncy;
+                    sy = g.Ly;
+                }
+            } else if (yy < 0 || yy >= g.ncy) {
+                continue;
+            }
+            for (int ox = -1; ox <= 1; ++ox) {
+                int xx = cx + ox;
+                double sx = 0.0;
+                if (xx < 0) {
+                    xx += g.ncx;
+                    sx = -g.Lx;
+                } else if (xx >= g.ncx) {
+                    xx -= g.ncx;
+                    sx = g.Lx;
+                }
+                const int c = sp.coff + yy * g.ncx + xx;
+                for (uint32_t k = P.cs[c]; k < P.cs[c + 1]; ++k) {
+                    if (k == s) continue;
+                    const uint32_t jd = P.sorted[k];
+                    double2 pj = P.x[jd];
+                    pj.x = pbc_wrap(pj.x, g.Lx);
+                    if (py) pj.y = pbc_wrap(pj.y, g.Ly);
+                    double dx = (pi.x - pj.x) - sx;
+                    double dy = (pi.y - pj.y) - sy;
+                    if (g.small_x) {
+                        const double d = pi.x - pj.x;
+                        dx = d - g.Lx * round(d / g.Lx);
+                    }
+                    if (py && g.small_y) {
+                        const double d = pi.y - pj.y;
+                        dy = d - g.Ly * round(d / g.Ly);
+                    }
+                    double fx, fy;
+                    ObsSum unused;
+                    if (!pair_fp64<false>(dx, dy, pair_index(ci, P.cls[jd]), pt, fx, fy, unused, false)) continue;
+                    const int b = prof_bin(pi.y - 0.5 * dy, g.Ly, nb, py);
+                    atomicAdd(&h[b], prof_fixed(0.5 * dx * fx));
+                    atomicAdd(&h[nb + b], prof_fixed(0.5 * dy * fy));
+                    atomicAdd(&h[2 * nb + b], prof_fixed(0.5 * dx * fy));
+                }
+            }
+        }
+    }
+    if constexpr (LDS_HIST) {
+        __syncthreads();
+        for (int j = threadIdx.x; j < 3 * nb; j += kBlock)
+            if (phist[j] != 0ull) atomicAdd(&acc[j], phist[j]);
+    }
+}
+
 // MDSimulation::middle_reflect_LG (src/md_env.cu:1457-1508, kernel
 // src/md_cuda_common.cu:26-65) on slot r: y wrapped, x reflected into the
 // central slab [xmin, xmin + width] with mirrored edges (period 2 width).

@@ -236,6 +236,14 @@ std::vector<int> MDSimulation::get_N_profile(int n_bins_per_rank) {
     return out;
 }
 
+std::vector<double> MDSimulation::get_pressure_profile(int n_bins_local) {   // src/md_env.cu:3821-3918
+    std::vector<double> out;
+    if (n_bins_local <= 0) return out;
+    out.assign(static_cast<size_t>(3 * n_bins_local), 0.0);
+    check(mdx_pressure_profile(sim_, n_bins_local, out.data()), "get_pressure_profile");
+    return out;
+}
+
 std::vector<double> MDSimulation::get_density_profile(int n_bins_per_rank) {
     const std::vector<int> n = get_N_profile(n_bins_per_rank);
     const size_t nb = n.size() / 2;

@@ -563,6 +563,99 @@ double orc_instant_pressure(orc_sim *s) {
     return sim.last_P;
 }
 
+// Pressure-tensor profile in nb slabs along y: local_pressure_tensor_profile_kernel
+// (src/md_cuda_common.cu:879-1030) driven by get_pressure_profile
+// (src/md_env.cu:3821-3918) with one rank.  Particle i adds m v_a v_b to the
+// slab of its y; each ordered pair inside the cutoff adds 1/2 d_a F_b to the
+// slab of the midpoint y_i - d_y / 2; slab = int(y / Ly * nb) clamped to
+// [0, nb) after wrapping y (periodic) or clamping it into [0, Ly).  Positions
+// are wrapped first, displacements take the minimum image.  Each contribution
+// is rounded to int64 units of 2^-32 and the units are summed as integers (the
+// library's definition, which makes the sums independent of their order; the
+// reference adds doubles with atomics).  out: Pxx[nb], Pyy[nb], Pxy[nb], each
+// slab's sum divided by its area Lx Ly / nb.
+void orc_pressure_profile(orc_sim *s, int32_t nb, int32_t periodic_y, double *out) {
+    Sim &sim = *S(s);
+    const orc_params &p = sim.p;
+    const double Lx = p.Lx, Ly = p.Ly;
+    const int n = static_cast<int>(sim.P.size());
+    const bool py = periodic_y != 0;
+    std::vector<long long> acc(static_cast<size_t>(3 * nb), 0);
+    auto fixed = [](double v) {
+        double q = std::rint(v * 4294967296.0);
+        q = q > 4.6e18 ? 4.6e18 : (q < -4.6e18 ? -4.6e18 : q);
+        return static_cast<long long>(q);
+    };
+    auto bin = [&](double y) {
+        if (py) {
+            y = pbc_wrap(y, Ly);
+        } else if (y < 0.0) {
+            y = 0.0;
+        } else if (y >= Ly && Ly > 0.0) {
+            y = Ly * (1.0 - 1e-12);
+        }
+        const double q = Ly > 0.0 ? y / Ly : 0.0;
+        int b = static_cast<int>(q * static_cast<double>(nb));
+        return b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+    };
+    std::vector<double> xw(n), yw(n);
+    for (int i = 0; i < n; ++i) {
+        xw[i] = pbc_wrap(sim.P[i].px, Lx);
+        yw[i] = py ? pbc_wrap(sim.P[i].py, Ly) : sim.P[i].py;
+        const double m = mass_of(p, sim.P[i].type);
+        const int b = bin(yw[i]);
+        acc[b] += fixed(m * sim.P[i].vx * sim.P[i].vx);
+        acc[nb + b] += fixed(m * sim.P[i].vy * sim.P[i].vy);
+        acc[2 * nb + b] += fixed(m * sim.P[i].vx * sim.P[i].vy);
+    }
+    const double vcc = vc_const(p.cutoff);
+    auto pair = [&](int i, int j) {
+        double dx = mic(xw[i] - xw[j], Lx);
+        double dy = yw[i] - yw[j];
+        if (py) dy = mic(dy, Ly);
+        const PairOut o = pair_eval(p, dx, dy, sim.P[i].type, sim.P[j].type, vcc);
+        if (!o.in) return;
+        const int b = bin(yw[i] - 0.5 * dy);
+        acc[b] += fixed(0.5 * dx * o.fx);
+        acc[nb + b] += fixed(0.5 * dy * o.fy);
+        acc[2 * nb + b] += fixed(0.5 * dx * o.fy);
+    };
+    if (p.force_mode == 0) {
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+                if (j != i) pair(i, j);
+    } else {
+        CellGrid g;
+        std::vector<uint32_t> keys, start, sorted;
+        build_cells(sim, py, g, keys, start, sorted);
+        for (int i = 0; i < n; ++i) {
+            const int cx = static_cast<int>(keys[i] % g.ncx);
+            const int cy = static_cast<int>(keys[i] / g.ncx);
+            for (int oy = -1; oy <= 1; ++oy) {
+                int yy = cy + oy;
+                if (py) {
+                    yy = (yy + g.ncy) % g.ncy;
+                } else if (yy < 0 || yy >= g.ncy) {
+                    continue;
+                }
+                for (int ox = -1; ox <= 1; ++ox) {
+                    const int xx = (cx + ox + g.ncx) % g.ncx;
+                    const uint32_t c = static_cast<uint32_t>(yy) * g.ncx + xx;
+                    for (uint32_t k = start[c]; k < start[c + 1]; ++k) {
+                        const int j = static_cast<int>(sorted[k]);
+                        if (j != i) pair(i, j);
+                    }
+                }
+            }
+        }
+    }
+    const double area = Lx * Ly / static_cast<double>(nb);
+    for (int k = 0; k < 3 * nb; ++k) {
+        const double v = static_cast<double>(acc[k]) / 4294967296.0;
+        out[k] = area > 0.0 ? v / area : v;
+    }
+}
+
 // src/md_env.cu:2362-2406: -eps * 1/2 sum_ordered (Fx dx - Fy dy).
 double orc_partial_U_lambda(orc_sim *s, double eps_lambda) {
     Sim &sim = *S(s);

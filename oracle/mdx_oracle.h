@@ -126,7 +126,9 @@ void orc_get_obs(const orc_sim *s, orc_obs *o);
 void orc_step_nve(orc_sim *s, int32_t nsteps);
 void orc_step_nvt(orc_sim *s, int32_t nsteps, int32_t middle_wrap);
 void orc_step_nph(orc_sim *s, int32_t nsteps, double p_ext);
-void orc_middle_reflect(orc_sim *s);   /* MDSimulation::middle_reflect_LG alone */
+void orc_middle_reflect(orc_sim *s);
+/* pressure-tensor profile in nb slabs along y: Pxx[nb], Pyy[nb], Pxy[nb] */
+void orc_pressure_profile(orc_sim *s, int32_t nb, int32_t periodic_y, double *out);   /* MDSimulation::middle_reflect_LG alone */
 void orc_step_piston(orc_sim *s, int32_t nsteps, double p_ext);
 
 /* FIRE on the sim's current frame (wrapped, v = a = 0 first).  energy_steps /

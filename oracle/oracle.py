@@ -70,6 +70,7 @@ _SIG = {
     "orc_step_nve": (None, [_P, C.c_int32]),
     "orc_step_nvt": (None, [_P, C.c_int32, C.c_int32]),
     "orc_middle_reflect": (None, [_P]),
+    "orc_pressure_profile": (None, [_P, C.c_int32, C.c_int32, _P]),
     "orc_step_nph": (None, [_P, C.c_int32, C.c_double]),
     "orc_step_piston": (None, [_P, C.c_int32, C.c_double]),
     "orc_fire": (None, [_P, C.POINTER(OFireParams), C.c_double, C.c_int32, C.POINTER(OFireResult),
@@ -187,6 +188,11 @@ class OracleSim:
 
     def middle_reflect(self) -> None:
         self._lib.orc_middle_reflect(self._h)
+
+    def pressure_profile(self, n_bins: int, periodic_y: bool = True) -> np.ndarray:
+        out = np.zeros(3 * n_bins)
+        self._lib.orc_pressure_profile(self._h, n_bins, 1 if periodic_y else 0, out.ctypes.data)
+        return out
 
     def step_nph(self, n: int, p_ext: float) -> None:
         self._lib.orc_step_nph(self._h, n, p_ext)

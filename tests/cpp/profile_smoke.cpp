@@ -1,8 +1,9 @@
-// get_N_profile and get_density_profile through the header-compatible
-// MDSimulation (reference src/md_env.cu:3920-3988, 4086-4104): 20 NVE steps,
-// then 16-slab profiles; the counts cover every particle and the density
-// profile is (n_A - n_B) / (n_A + n_B) per slab (0 where empty), so it lies
-// in [-1, 1].  argv: config.json
+// get_N_profile, get_density_profile and get_pressure_profile through the
+// header-compatible MDSimulation (reference src/md_env.cu:3920-3988,
+// 4086-4104, 3821-3918): 20 NVE steps, then 16-slab profiles; the counts
+// cover every particle, the density profile is (n_A - n_B) / (n_A + n_B) per
+// slab (0 where empty), so it lies in [-1, 1], and the pressure profile's
+// slabs sum to the scalar pressure.  argv: config.json
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -32,6 +33,15 @@ int main(int argc, char **argv) {
         }
     }
     if (total != cfg_mgr.config.n_particles_global) return 4;
-    std::printf("profile ok n=%ld a=%ld slab0=%d,%d\n", total, total_a, n[0], n[nb]);
+    const std::vector<double> pp = sim.get_pressure_profile(nb);
+    if (static_cast<int>(pp.size()) != 3 * nb || !sim.get_pressure_profile(0).empty()) return 5;
+    double tr = 0.0;
+    for (int i = 0; i < nb; ++i) tr += pp[i] + pp[i + nb];
+    const double P_from_profile = tr / nb / 2.0, P = sim.cal_instant_pressure();
+    if (!(std::fabs(P_from_profile - P) <= 1e-6 * std::fabs(P))) {
+        std::fprintf(stderr, "pressure profile sums to %.12g, scalar pressure %.12g\n", P_from_profile, P);
+        return 6;
+    }
+    std::printf("profile ok n=%ld a=%ld slab0=%d,%d P=%.9g\n", total, total_a, n[0], n[nb], P);
     return 0;
 }

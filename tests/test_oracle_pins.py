@@ -274,3 +274,68 @@ def test_middle_reflect_is_a_triangle_wave():
     tri = x_min + w - np.abs(np.mod(x - x_min, 2 * w) - w)
     assert np.abs(got - tri).max() < 1e-12
     assert got.min() >= x_min and got.max() <= x_min + w
+
+
+# ---------------------------------------------------------------------------
+# pressure-tensor profile
+# ---------------------------------------------------------------------------
+
+def test_pressure_profile_sums_to_the_scalar_pressure_and_lambda():
+    """Summed over the slabs (times each slab's area), Pxx + Pyy is 2 A P
+    with P the scalar pressure (src/md_env.cu:2205-2252), whatever the number
+    of slabs, and at rest Pxx - Pyy sums to -dU/dlambda at eps_lambda = 1
+    (src/md_env.cu:2362-2406) -- both pinned by finite differences."""
+    f, Lx, Ly = lattice_frame(256, 0.8, seed=5, jitter=0.08, T=0.7)
+    o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, eps_ab=0.5), f)
+    P = o.pressure()
+    area = Lx * Ly
+    for nb in (1, 7, 50):
+        pp = o.pressure_profile(nb).reshape(3, nb)
+        V = area / nb
+        P_sum = (pp[0] + pp[1]).sum() * V / (2.0 * area)
+        assert abs(P_sum - P) <= 1e-9 * abs(P), (nb, P_sum, P)
+    o.close()
+    g = f.copy()
+    g["vel"] = 0.0
+    o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, eps_ab=0.5), g)
+    lam = o.partial_U_lambda(1.0)
+    pp = o.pressure_profile(9).reshape(3, 9)
+    assert abs((pp[0] - pp[1]).sum() * area / 9 + lam) <= 1e-9 * abs(lam)
+    # Pxy = Pyx for central forces: sum over slabs of 1/2 dx Fy equals 1/2 dy Fx,
+    # checked against a brute-force sum
+    pos = np.mod(g["pos"], [Lx, Ly])
+    sxy = 0.0
+    for i in range(len(pos)):
+        d = pos[i] - pos
+        d -= np.round(d / [Lx, Ly]) * [Lx, Ly]
+        for j in range(len(pos)):
+            if j == i:
+                continue
+            eps = 1.0 if g["type"][i] == g["type"][j] else 0.5
+            F = lj_force_pair(d[j], eps=eps)
+            sxy += 0.5 * d[j][0] * F[1]
+    assert abs(pp[2].sum() * area / 9 - sxy) <= 1e-8 * max(1.0, abs(sxy))
+    o.close()
+
+
+def test_pressure_profile_hand_cases():
+    """Two particles, Ly = 10, 10 slabs: the pair's virial lands in the slab
+    of its midpoint (twice, once per ordered pair), each kinetic term in its
+    particle's slab; across the periodic edge the midpoint wraps."""
+    L, nb = 10.0, 10
+    for (y1, y2, want_bin) in ((1.8, 2.7, 2), (0.2, 9.7, 9)):
+        f = np.zeros(2, O.PARTICLE_DTYPE)
+        f["pos"] = [[5.0, y1], [5.4, y2]]
+        f["vel"] = [[0.5, -0.25], [-1.0, 0.75]]
+        o = O.OracleSim(O.make_params(L, L, force_mode=0, eps_ab=1.0), f)
+        pp = o.pressure_profile(nb).reshape(3, nb) * (L * L / nb)
+        o.close()
+        d = np.array([5.0 - 5.4, y1 - y2])
+        d -= np.round(d / L) * L
+        F = lj_force_pair(d)
+        want = np.zeros((3, nb))
+        for (y, v) in ((y1, f["vel"][0]), (y2, f["vel"][1])):
+            b = int(y / L * nb)
+            want[:, b] += [v[0] * v[0], v[1] * v[1], v[0] * v[1]]
+        want[:, want_bin] += [d[0] * F[0], d[1] * F[1], d[0] * F[1]]   # two ordered pairs, 1/2 each
+        assert np.allclose(pp, want, rtol=1e-9, atol=1e-9), (y1, y2, pp, want)
