@@ -13,6 +13,7 @@
 
 #include "md_common.hpp"
 #include "md_config.hpp"
+#include "md_equilibrium.hpp"
 #include "md_particle.hpp"
 #include "mdx.h"
 
@@ -63,6 +64,11 @@ public:
     double get_nph_piston_force() const { return nph_piston_force; }
 
     void sample_collect();
+    // Records U at every save_dt_interval boundary t has passed since the last
+    // call (cal_total_U at the current state), then runs one equilibrium test
+    // (EquilibriumTracker) at max(sensitivity, 1e-3).  The reference's name
+    // and spelling (src/md_env.cu:2478-2500).
+    bool check_eqlibrium(double sensitivity);
 
     // Extensions of this library.
     void step_many(int kind, int nsteps, double pressure_ext = 0.0, bool do_middle_wrap = false);
@@ -92,4 +98,6 @@ private:
     double nph_piston_force = 0.0;
     int precision_ = MDX_PREC_COMP32;
     bool autotune_pending_ = false;   // MDX_TUNING auto: tune at the first NVE / NVT step
+    EquilibriumTracker eq_;           // U record of check_eqlibrium (reset by the constructors)
+    double next_record_time_ = 0.0;
 };

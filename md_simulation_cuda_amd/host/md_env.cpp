@@ -68,6 +68,10 @@ void MDSimulation::check(int rc, const char *where, bool throws) const {
 }
 
 void MDSimulation::create_from_frame(const std::vector<Particle> &frame) {
+    // equilibrium record (src/md_env.cu:174-195): time 0, first record at one interval
+    t = 0.0;
+    eq_.reset(cfg_manager.config.save_dt_interval, cfg_manager.config.dt);
+    next_record_time_ = eq_.interval();
     const mdx_params p = params_from(cfg_manager.config, precision_);
     check(mdx_create(&p, reinterpret_cast<const mdx_particle64 *>(frame.data()),
                      static_cast<int64_t>(frame.size()), &sim_),
@@ -234,6 +238,15 @@ std::vector<int> MDSimulation::get_N_profile(int n_bins_per_rank) {
     if (n_bins_per_rank > 0)
         check(mdx_N_profile(sim_, n_bins_per_rank, reinterpret_cast<int32_t *>(out.data())), "get_N_profile");
     return out;
+}
+
+bool MDSimulation::check_eqlibrium(double sensitivity) {   // src/md_env.cu:2478-2500
+    const double sens = std::max(sensitivity, 1e-3);
+    while (t + 1e-12 >= next_record_time_) {   // the same U for every boundary passed
+        eq_.record(cal_total_U());
+        next_record_time_ += eq_.interval();
+    }
+    return eq_.evaluate(sens, cfg_manager.config.n_particles_global);
 }
 
 std::vector<double> MDSimulation::get_pressure_profile(int n_bins_local) {   // src/md_env.cu:3821-3918

@@ -54,3 +54,14 @@ def test_save_env_and_resume_constructor(tmp_path):
     r = run(CPP / "save_resume", "run", tmp_path)
     assert r.returncode == 0
     assert "save_resume bad=0" in r.stdout
+
+
+def test_check_eqlibrium_through_the_cpp_class():
+    # a window is 100 samples (t = 100), so no test can pass before two
+    # windows, 40000 steps; an NVT liquid from a lattice start passes within
+    # the run
+    r = run(CPP / "equilibrium_nvt", 60000, 200, timeout=300)
+    assert r.returncode == 0
+    m = re.search(r"first=(\d+) checks=(\d+) positives=(\d+)", r.stdout)
+    first, checks, positives = map(int, m.groups())
+    assert 40000 <= first <= 60000 and checks == 300 and positives >= 1

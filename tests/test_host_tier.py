@@ -202,3 +202,81 @@ def test_resume_constructor_failures(tmp_path, mode, message):
     assert r.returncode == 1, r.stdout + r.stderr
     assert message in r.stderr
     assert "resumed" not in r.stdout
+
+
+def eq_series(tmp_path, series, interval=10.0, dt=0.005, sens=1.0, n=100):
+    path = tmp_path / "u.txt"
+    path.write_text("\n".join(repr(float(u)) for u in series) + "\n")
+    r = run("equilibrium_stats", interval, dt, sens, n, path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    W = int(lines[0].split()[1])
+    rows = [tuple(int(v) for v in l.split()) for l in lines[1:]]
+    return W, [s for _, s, _ in rows], [e for _, _, e in rows]
+
+
+def eq_reference(series, W, sens, n):
+    """The equilibrium test written from its definition with the statistics
+    module: (streak, result) after each sample."""
+    import math
+    import statistics
+
+    out, streak = [], 0
+    for k in range(1, len(series) + 1):
+        hist = series[:k][-8 * W:]
+        if len(hist) < 2 * W:
+            out.append((streak, False))
+            continue
+        a, b = hist[-2 * W:-W], hist[-W:]
+
+        def rel(w):
+            den = w[0]
+            if abs(den) < 1e-12:
+                den = w[-1] if abs(w[-1]) > 1e-12 else (1.0 if w[0] >= 0 else -1.0)
+            return 0.0 if abs(den) < 1e-12 else (w[-1] - w[0]) / den
+
+        tol = sens / math.sqrt(max(1, n))
+        flat = abs(rel(a)) < tol and abs(rel(b)) < tol
+        va = statistics.variance(a) if W > 1 else 0.0
+        vb = statistics.variance(b) if W > 1 else 0.0
+        se = math.sqrt(va / W + vb / W + 1e-18)
+        z = abs(statistics.fmean(a) - statistics.fmean(b)) / se if se > 0 else 0.0
+        alpha = min(max(0.2 / sens, 1e-4), 0.5)
+        streak = streak + 1 if (flat and math.erfc(z / math.sqrt(2)) > alpha) else 0
+        out.append((streak, streak >= max(1, int(math.floor(3 / sens + 0.5)))))
+    return out
+
+
+def test_equilibrium_detector_on_known_series(tmp_path):
+    # check_eqlibrium's test (src/md_env.cu:174-313): windows of 100 time
+    # units, two of them needed, round(3 / sensitivity) passes in a row
+    W, streak, eq = eq_series(tmp_path, [-2.0] * 30)
+    assert W == 10
+    assert eq.index(1) == 21 and streak[19] == 1 and streak[21] == 3   # 2W samples, then 3 passes
+    for sens, first in ((2.0, 20), (3.0, 19), (0.5, 24)):
+        _, _, e = eq_series(tmp_path, [-2.0] * 40, sens=sens)
+        assert e.index(1) == first, sens
+    # a slow ramp keeps each window flat but moves the mean: never equilibrated
+    _, streak, eq = eq_series(tmp_path, [-2.0 + 0.01 * i for i in range(60)])
+    assert not any(eq) and not any(streak)
+    # a steep ramp fails the slope test
+    _, _, eq = eq_series(tmp_path, [-2.0 + 0.1 * i for i in range(60)])
+    assert not any(eq)
+    # a series at zero uses the fallback denominators and counts as flat
+    _, _, eq = eq_series(tmp_path, [0.0] * 25)
+    assert eq[21] == 1
+    # record intervals: 100 time units of samples, the dt fallback, at least 1
+    assert eq_series(tmp_path, [1.0], interval=0.1)[0] == 1000
+    assert eq_series(tmp_path, [1.0], interval=0.0, dt=0.5)[0] == 200
+    assert eq_series(tmp_path, [1.0], interval=500.0)[0] == 1
+
+
+def test_equilibrium_detector_matches_its_definition(tmp_path):
+    rng = np.random.default_rng(11)
+    series = list(-2.0 + 0.01 * rng.standard_normal(400))
+    series[150:170] = [-1.9] * 20   # a jump that resets the streak
+    for sens in (0.7, 1.0, 2.5):
+        W, streak, eq = eq_series(tmp_path, series, interval=5.0, sens=sens, n=1000)
+        ref = eq_reference(series, W, sens, 1000)
+        assert streak == [s for s, _ in ref] and eq == [int(e) for _, e in ref], sens
+        assert any(eq) and not all(eq)
