@@ -65,3 +65,48 @@ def test_check_eqlibrium_through_the_cpp_class():
     m = re.search(r"first=(\d+) checks=(\d+) positives=(\d+)", r.stdout)
     first, checks, positives = map(int, m.groups())
     assert 40000 <= first <= 60000 and checks == 300 and positives >= 1
+
+
+def read_csv(path):
+    import csv
+    with open(path) as f:
+        return list(csv.DictReader(f))
+
+
+def test_three_phase_nph_protocol(tmp_path):
+    """The reference's NPH driver protocol (tests/run_NPH_test/run_test_NPH.cpp)
+    on its own config (N = 16384, 6:1 box, dt 0.001, W = 1e4), phases
+    shortened: 2000 auto-target steps, a sample every 100 steps, windows of 5
+    samples, tolerance 20%, 2 stable windows, warm-up 1000..4000 steps, 1000
+    production steps, a record every 500 steps.  Checked from the logs: the
+    target is the mean of the auto phase's samples, the phases run in order,
+    the warm-up ends on the stated rule, the box moves at fixed aspect ratio."""
+    out = tmp_path / "run"
+    r = run(CPP / "nph_protocol", GOLDEN / "run_nph_test_config.json", out, 2000, 100, 5, 0.2, 2, 1000, 4000,
+            1000, 0.5, timeout=300)
+    assert r.returncode == 0
+    m = dict(kv.split("=") for kv in r.stdout.split()[1:])
+    P_target = float(m["P_target"])
+    rows = read_csv(out / "pressure_log.csv")
+    auto = [float(x["P_inst"]) for x in rows if x["phase"] == "AUTO_TARGET"]
+    assert len(auto) == 20
+    assert abs(sum(auto) / len(auto) - P_target) <= 1e-8 * abs(P_target)
+    phases = [x["phase"] for x in rows]
+    order = ["AUTO_TARGET", "NPH_WARMUP", "NPH_PRODUCTION"]
+    assert [p for i, p in enumerate(phases) if i == 0 or p != phases[i - 1]] == order
+    assert int(m["auto"]) == 2000 and int(m["nph"]) == 1000
+    warm = int(m["warm"])
+    assert 1000 <= warm <= 4000
+    if m["forced"] == "0":   # the last evaluations before production met the tolerance
+        last = [x for x in rows if x["phase"] == "NPH_WARMUP"][-2:]
+        for x in last:
+            assert float(x["rel_err"]) <= 0.2 and float(x["rolling_std"]) <= 0.2 * abs(P_target)
+    else:
+        assert warm == 4000
+    box = [(float(x["Lx"]), float(x["Ly"])) for x in rows if x["phase"] != "AUTO_TARGET"]
+    ratios = [a / b for a, b in box]
+    assert max(ratios) - min(ratios) <= 1e-9 * ratios[0]          # aspect ratio kept
+    assert max(a for a, _ in box) > min(a for a, _ in box)          # the box moved
+    thermo = read_csv(out / "thermo_log.csv")
+    assert [int(x["step"]) for x in thermo] == list(range(0, 2000 + warm + 1000, 500))
+    assert (out / "saved_env.bin").stat().st_size > 16
