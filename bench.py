@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-vs-FP64 error report")
     ap.add_argument("--workload", default="nvt1m",
-                    choices=["nvt1m", "ensemble", "serve", "fire_nph", "nve4k", "long", "dropin"],
+                    choices=["nvt1m", "ensemble", "serve", "fire_nph", "nve4k", "long", "dropin", "liquidgas"],
                     help="nvt1m: N=1M Nose-Hoover (default line); ensemble: 256 NVT replicas of uneven N "
                          "in one batch vs per-replica launches; serve: Poisson job stream through the "
                          "continuous-batching server; fire_nph: FIRE from a random start at N=262144 "
@@ -735,6 +735,71 @@ def run_nve4k(args):
     print(json.dumps(out), flush=True)
 
 
+def run_liquidgas(args):
+    """The LiquidGas NVT mode at scale (run/series_LiquidGas.cpp:209-211 of
+    the reference: step_single_nose_hoover(do_middle_wrap)): N=1M particles of
+    one species, 2:1 box at overall density 0.375, the liquid started as the
+    central slab of width Lx / 2 (init_particles, devide_p 0.5), T = 0.6,
+    Q = 10, dt = 0.005, folded into the slab every step.  Timed with rebuild
+    windows (the lean steps fold too) and with the cell list every step; beside
+    them the per-record observables the series drivers call (pressure-tensor
+    and N profiles in 100 slabs)."""
+    from md_simulation_cuda_amd import abi
+
+    n = args.n
+    Ly = math.sqrt(n / 0.375 / 2.0)
+    Lx = 2.0 * Ly
+    frame = abi.init_particles(n, n, Lx, Ly, 0.6, devide_p=0.5)
+    kw = dict(dt=0.005, T_target=0.6, Q=10.0, devide_p=0.5)
+    steps = args.steps
+    res = {}
+    for name, tu in (("windows", (1.0, 1, 0.65, 16)), ("cell_list_every_step", (1.0, 1, 0.0, 0))):
+        sim = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), frame)
+        sim.set_tuning(*tu)
+        w = max(tu[3], 1)
+        sim.step(abi.NVT, 2 * 16, flags=abi.STEP_MIDDLE_WRAP)
+        sim.sync()
+        k = -(-steps // 16) * 16
+        t0 = time.perf_counter()
+        sim.step(abi.NVT, k, flags=abi.STEP_MIDDLE_WRAP)
+        sim.sync()
+        wall = time.perf_counter() - t0
+        res[name] = {"value": n * k / wall, "ms_per_step": wall * 1e3 / k, "steps": k,
+                     "tuning": dict(zip(("cell_scale", "lanes_per_particle", "skin", "window"), tu))}
+        if name == "windows":
+            for what, fn in (("pressure_profile_100", lambda: sim.pressure_profile(100)),
+                             ("N_profile_100", lambda: sim.N_profile(100))):
+                fn()
+                t0 = time.perf_counter()
+                for _ in range(5):
+                    fn()
+                res[what + "_ms"] = (time.perf_counter() - t0) * 1e3 / 5
+            fr = sim.collect()
+            slab = 0.5 * Lx + 2.0
+            x0 = 0.5 * Lx - 0.5 * slab
+            res["all_in_slab"] = bool((fr["pos"][:, 0] >= x0).all() and (fr["pos"][:, 0] <= x0 + slab).all())
+        sim.close()
+    out = {
+        "metric": "atom-steps/s @N=1M LiquidGas NVT (do_middle_wrap)",
+        "value": res["windows"]["value"],
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": res["windows"]["steps"],
+        "warmup": 32,
+        "ms_per_step": res["windows"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair + fp32 two-sum accumulation, fp64 state",
+        "data": "synthetic (reference init_particles, one species, seeded)",
+        "config": {"workload": "NVT Nose-Hoover with the middle fold, one species, N=1048576, 2:1 box at overall "
+                               "density 0.375, liquid slab Lx/2, T=0.6, Q=10, dt=0.005", "N": n},
+        "modes": res,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_dropin(args):
     """The N=1M NVT workload through the C++ MDSimulation class
     (tests/cpp/dropin_nvt.cpp): one step_single_nose_hoover() per loop
@@ -819,6 +884,9 @@ def main():
         return
     if args.workload == "long":
         run_long(args)
+        return
+    if args.workload == "liquidgas":
+        run_liquidgas(args)
         return
     if args.workload == "dropin":
         run_dropin(args)

@@ -514,7 +514,7 @@ bool Batch::windows_apply() {
     for (size_t r = 0; r < slots.size(); ++r) {
         Slot &sl = slots[r];
         if (!sl.loaded || h_sp[r].kind == SK_NONE) continue;
-        if (!h_sp[r].list_mode || h_sp[r].middle_wrap) return false;
+        if (!h_sp[r].list_mode) return false;
         // the lean-step fallback needs 5 distinct cells per side; list-mode
         // kinds (NVE, NVT) keep the box, which is read back once after an NPH
         // or piston run may have changed it

@@ -791,6 +791,12 @@ __global__ void __launch_bounds__(kLeanThreads) k_lean(BatchPtrs P, int32_t pari
             }
             p.x = pbc_wrap(p.x, g.Lx);
             p.y = pbc_wrap(p.y, g.Ly);
+            if (sp.middle_wrap) {   // do_middle_wrap: as in k_drift (src/md_cuda_common.cu:26-65)
+                double ym = fmod(p.x - sp.slab_xmin, sp.slab_period);
+                if (ym < 0.0) ym += sp.slab_period;
+                if (ym > sp.slab_width) ym = sp.slab_period - ym;
+                p.x = sp.slab_xmin + ym;
+            }
             P.v[id] = q;
             P.x[id] = p;
         }

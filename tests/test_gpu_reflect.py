@@ -7,6 +7,7 @@ devide_p 0.7: the slab of width 0.7 Lx + 2 sigma leaves 1152 particles
 outside, and folding them in brings no two particles closer than 0.96 sigma
 (checked when the frame was chosen), so the run after the fold stays smooth.
 """
+import math
 import sys
 
 import numpy as np
@@ -81,3 +82,39 @@ def test_nvt_continues_after_reflect(prec, tuning, tol):
     say(f"  xi {og.xi:.9e} vs {oo.xi:.9e}; K {og.K:.9e} vs {Ko:.9e}; max |dx| {np.abs(d).max():.3e}")
     assert dxi <= tol and dk <= tol
     assert np.abs(d).max() <= tol
+
+
+def test_windowed_nvt_with_middle_wrap_follows_oracle():
+    """The LiquidGas NVT mode (step_single_nose_hoover(do_middle_wrap=true),
+    src/md_env.cu:1639-1728 with the fold of :1457-1508) in rebuild windows:
+    the lean steps fold too.  One species, 4096 particles in a liquid slab of
+    width 0.4 Lx (overall density 0.3, T 0.45), 60 steps (the head step and
+    lean steps of four windows) against the oracle: xi and K to FP32
+    rounding, positions to 1e-5, and every particle inside the slab."""
+    n = 4096
+    Ly = math.sqrt(n / 0.3 / 2.0)
+    Lx = 2.0 * Ly
+    f = abi.init_particles(n, n, Lx, Ly, 0.45, devide_p=0.4)
+    kw = dict(eps_ab=0.5, dt=0.005, T_target=0.45, Q=10.0, devide_p=0.4)
+    say("LiquidGas NVT with middle wrap, windows of 16, 60 steps")
+    g = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), f)
+    g.set_tuning(1.0, 1, 0.65, 16)
+    o = O.OracleSim(O.make_params(Lx, Ly, force_mode=1, **kw), f)
+    o.compute_forces()
+    for chunk in (20, 17, 23):
+        g.step(abi.NVT, chunk, flags=abi.STEP_MIDDLE_WRAP)
+        o.step_nvt(chunk, middle_wrap=True)
+    og = g.observables(abi.OBS_K)
+    fg = g.collect()
+    g.close()
+    oo, Ko, fo = o.obs(), o.K(), o.frame()
+    o.close()
+    d = fg["pos"] - fo["pos"]
+    d -= np.round(d / [Lx, Ly]) * [Lx, Ly]
+    slab = 0.4 * Lx + 2.0
+    x0 = 0.5 * Lx - 0.5 * slab
+    say(f"  xi {og.xi:.9e} vs {oo.xi:.9e}; K {og.K:.9e} vs {Ko:.9e}; max |dx| {np.abs(d).max():.2e}")
+    assert abs(og.xi - oo.xi) <= 1e-5 * max(abs(oo.xi), 1e-3)
+    assert abs(og.K - Ko) <= 3e-6 * Ko
+    assert np.abs(d).max() <= 1e-5
+    assert fg["pos"][:, 0].min() >= x0 and fg["pos"][:, 0].max() <= x0 + slab
