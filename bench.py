@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--per-replica-steps", type=int, default=100)
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-frame-interval", type=int, default=2000,
+                    help="serve: each job streams a frame about every this many steps (0: final frames only)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (scale for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -413,7 +415,9 @@ def run_serve(args):
     """Poisson job stream through the server: kinds NVE/NVT/NPH/FIRE with
     weights 0.3/0.4/0.2/0.1, N log-uniform in [1024, 262144], steps log-uniform
     in [500, 20000], the binary LJ family (T U[0.4,1], eps_AB U[0.3,0.9],
-    density U[0.70,0.85]); frames are built when a job arrives."""
+    density U[0.70,0.85]); frames are built when a job arrives.  Every job
+    streams a frame about every --serve-frame-interval steps, taken from the
+    server after each tick."""
     import numpy as np
     from md_simulation_cuda_amd import abi
 
@@ -434,6 +438,8 @@ def run_serve(args):
     srv = abi.Server()
     offered = sum(j["n"] * j["steps"] for j in jobs) / args.serve_seconds
     idx = 0
+    n_frames, frame_bytes = 0, 0
+    live = set()
     lat = {}
     done_ids = set()
     id2job = {}
@@ -448,15 +454,22 @@ def run_serve(args):
                                         T, j["seed"])
             p = abi.make_params(Lx, Ly, eps_ab=j["eab"], dt=0.005, T_target=j["T"], Q=10.0, barostat_mass=50.0)
             jid = srv.submit(abi.make_job(p, j["kind"], j["steps"], p_ext=1.0, force_tol=1e-6, fire=fp,
-                                          t_arrival=t_start + j["t"]), f)
+                                          t_arrival=t_start + j["t"], frame_interval=args.serve_frame_interval), f)
             id2job[jid] = j
+            live.add(jid)
             idx += 1
         nr, nq, fin = srv.tick()
+        if args.serve_frame_interval > 0:
+            for jid in list(live):
+                while (fr := srv.frame(jid)) is not None:
+                    n_frames += 1
+                    frame_bytes += fr[1].nbytes
         for jid in fin:
             info = srv.info(jid)
             lat[jid] = (info.t_done - info.t_arrival, id2job[jid]["kind"])
             srv.result(jid, release=True)
             done_ids.add(jid)
+            live.discard(jid)
         if nr == 0 and nq == 0 and idx < len(jobs):
             time.sleep(max(0.0, min(0.05, jobs[idx]["t"] - (srv.now() - t_start))))
         if time.perf_counter() - last_print > 30:
@@ -496,6 +509,8 @@ def run_serve(args):
         "wall_s": wall,
         "ticks": st.ticks,
         "peak_reserved_bytes": st.peak_reserved_bytes,
+        "frames": {"interval_steps": args.serve_frame_interval, "streamed": n_frames,
+                   "dropped": int(st.frames_dropped), "bytes_per_s": frame_bytes / wall},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -384,7 +384,8 @@ typedef struct mdx_server_params {
     int32_t device;               /* -1: current                                         */
     int32_t list_slots_per_class; /* NVE / NVT slots per size class, in a batch with
                                      neighbour lists (6; 0: they share the MD slots)     */
-    int32_t pad0;
+    int32_t frame_queue;          /* streamed frames a job holds until taken (8); a
+                                     further frame drops the oldest                      */
     double list_skin;             /* neighbour-list skin of those slots (0.5)           */
 } mdx_server_params;
 
@@ -396,6 +397,9 @@ typedef struct mdx_job {
     double force_tol;             /* FIRE: converged when max|a| < force_tol             */
     mdx_fire_params fire;         /* FIRE parameters (energy_interval, fp64_switch unused) */
     double t_arrival;             /* server clock seconds (mdx_server_now); < 0: now     */
+    int32_t frame_interval;       /* > 0: stream a frame about every frame_interval MD
+                                     steps or FIRE iterations (mdx_server_frame)         */
+    int32_t pad;
 } mdx_job;
 
 typedef struct mdx_energy_record {
@@ -412,13 +416,14 @@ typedef struct mdx_job_info {
     mdx_obs last;                 /* latest K/U/P (MD) or final PE in U (FIRE)         */
     mdx_fire_result fire;         /* FIRE jobs */
     int32_t n_records;
-    int32_t pad;
+    int32_t n_frames;             /* streamed frames waiting in mdx_server_frame         */
 } mdx_job_info;
 
 typedef struct mdx_server_stats {
     double atom_steps;            /* sum over ticks of steps x N of every running job   */
     double reserved_bytes, peak_reserved_bytes;
     int64_t ticks, completed, queued, slots;
+    int64_t frames_streamed, frames_dropped;
 } mdx_server_counters;
 
 typedef struct mdx_server mdx_server;
@@ -438,6 +443,16 @@ int mdx_server_records(mdx_server *s, int64_t job_id, mdx_energy_record *out, in
 /* Final frame of a finished job (original particle order); release != 0 forgets the job. */
 int mdx_server_result(mdx_server *s, int64_t job_id, mdx_particle64 *frame, int64_t cap, int32_t release);
 int mdx_server_stats(mdx_server *s, mdx_server_counters *out);
+/* Streamed frames: the oldest frame a job with frame_interval > 0 holds
+ * (original particle order, as mdx_collect gives it), taken out of its queue.
+ * *step receives the job's step (FIRE: iteration) count at the frame, *got 1,
+ * or *got 0 when none is waiting.  A frame is taken at the end of each tick in
+ * which the job's count reaches or passes a multiple of frame_interval (one
+ * frame for several multiples; ticks are not shortened for it), the last MD
+ * step included; a job holds at most frame_queue frames.  frame NULL: *got
+ * receives the number of frames waiting and none is taken. */
+int mdx_server_frame(mdx_server *s, int64_t job_id, mdx_particle64 *frame, int64_t cap, int64_t *step,
+                     int32_t *got);
 
 /* ---- initial frames (host only, no GPU) --------------------------------- */
 

@@ -118,7 +118,7 @@ class ServerParams(C.Structure):
         ("min_class_log2", C.c_int32), ("max_class_log2", C.c_int32),
         ("md_slots_per_class", C.c_int32), ("fire_slots_per_class", C.c_int32),
         ("quantum", C.c_int32), ("record_energies", C.c_int32), ("md_precision", C.c_int32),
-        ("device", C.c_int32), ("list_slots_per_class", C.c_int32), ("pad0", C.c_int32),
+        ("device", C.c_int32), ("list_slots_per_class", C.c_int32), ("frame_queue", C.c_int32),
         ("list_skin", C.c_double),
     ]
 
@@ -127,6 +127,7 @@ class Job(C.Structure):
     _fields_ = [
         ("params", Params), ("kind", C.c_int32), ("nsteps", C.c_int32), ("p_ext", C.c_double),
         ("force_tol", C.c_double), ("fire", FireParams), ("t_arrival", C.c_double),
+        ("frame_interval", C.c_int32), ("pad", C.c_int32),
     ]
 
 
@@ -139,7 +140,7 @@ class JobInfo(C.Structure):
         ("id", C.c_int64), ("state", C.c_int32), ("kind", C.c_int32), ("n", C.c_int64),
         ("steps_done", C.c_int64), ("t_arrival", C.c_double), ("t_admit", C.c_double),
         ("t_done", C.c_double), ("last", Obs), ("fire", FireResult), ("n_records", C.c_int32),
-        ("pad", C.c_int32),
+        ("n_frames", C.c_int32),
     ]
 
 
@@ -147,6 +148,7 @@ class ServerCounters(C.Structure):
     _fields_ = [
         ("atom_steps", C.c_double), ("reserved_bytes", C.c_double), ("peak_reserved_bytes", C.c_double),
         ("ticks", C.c_int64), ("completed", C.c_int64), ("queued", C.c_int64), ("slots", C.c_int64),
+        ("frames_streamed", C.c_int64), ("frames_dropped", C.c_int64),
     ]
 
 
@@ -170,6 +172,7 @@ _SIGNATURES = {
     "mdx_server_records": (C.c_int, [_P, C.c_int64, _P, C.c_int32, C.POINTER(C.c_int32)]),
     "mdx_server_result": (C.c_int, [_P, C.c_int64, _P, C.c_int64, C.c_int32]),
     "mdx_server_stats": (C.c_int, [_P, C.POINTER(ServerCounters)]),
+    "mdx_server_frame": (C.c_int, [_P, C.c_int64, _P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "mdx_create": (C.c_int, [C.POINTER(Params), _P, C.c_int64, C.POINTER(_P)]),
     "mdx_destroy": (C.c_int, [_P]),
     "mdx_set_state": (C.c_int, [_P, _P, C.c_int64]),
@@ -539,7 +542,7 @@ def server_default_params() -> ServerParams:
 
 
 def make_job(params: Params, kind: int, nsteps: int, p_ext: float = 0.0, force_tol: float = 1e-6,
-             fire: FireParams | None = None, t_arrival: float = -1.0) -> Job:
+             fire: FireParams | None = None, t_arrival: float = -1.0, frame_interval: int = 0) -> Job:
     j = Job()
     j.params = params
     j.kind = kind
@@ -549,6 +552,7 @@ def make_job(params: Params, kind: int, nsteps: int, p_ext: float = 0.0, force_t
     if fire is not None:
         j.fire = fire
     j.t_arrival = t_arrival
+    j.frame_interval = frame_interval
     return j
 
 
@@ -614,6 +618,16 @@ class Server:
         if release:
             self._n.pop(jid, None)
         return out
+
+    def frame(self, jid: int):
+        """The oldest streamed frame of the job and its step count, or None."""
+        step, got = C.c_int64(), C.c_int32()
+        check(self._lib.mdx_server_frame(self._h, jid, None, 0, None, C.byref(got)))
+        if got.value == 0:
+            return None
+        out = empty_frame(self._n[jid])
+        check(self._lib.mdx_server_frame(self._h, jid, _ptr(out), len(out), C.byref(step), C.byref(got)))
+        return (step.value, out) if got.value else None
 
     def stats(self) -> ServerCounters:
         out = ServerCounters()

@@ -15,7 +15,9 @@
 //      quantum and at most any MD job's remaining steps); the FIRE pool's
 //      launches are queued first so the pools overlap on the device;
 //   3. records K, U, P of every running MD job (one kinetic and one energy
-//      pass per pool) and retires finished jobs: final observables, frame
+//      pass per pool), copies the current frame of each job that streams
+//      frames (frame_interval f > 0) and whose step count passed a multiple of
+//      f in this tick, and retires finished jobs: final observables, frame
 //      copied to the host, slot freed.
 // Because every reduction is per slot and a slot's neighbour order depends only
 // on its own particles, a job gives the same result as the same system run
@@ -70,6 +72,11 @@ struct Job {
     mdx_obs last{};
     mdx_fire_result fire{};
     std::vector<mdx_energy_record> records;
+    struct Streamed {
+        int64_t step;
+        std::vector<mdx_particle64> frame;
+    };
+    std::deque<Streamed> frames;   // streamed frames not yet taken, oldest first
 };
 
 struct Pool {
@@ -92,6 +99,7 @@ struct mdx_server {
     double reserved = 0.0, peak_reserved = 0.0;
     double atom_steps = 0.0;
     int64_t ticks = 0, completed = 0;
+    int64_t frames_streamed = 0, frames_dropped = 0;
     std::chrono::steady_clock::time_point t0;
 
     double now() const {
@@ -148,6 +156,7 @@ struct mdx_server {
         const bool fire = j.kind == MDX_FIRE;
         if (!fire && (j.kind < MDX_NVE || j.kind > MDX_PISTON))
             return set_error(MDX_ERR_INVALID, "server: unknown job kind");
+        if (j.frame_interval < 0) return set_error(MDX_ERR_INVALID, "server: negative frame_interval");
         const Pool &pl = pools[pool_of(j.kind)];
         if (pl.cap.empty() || n > pl.cap.back())
             return set_error(MDX_ERR_CAPACITY, "server: job larger than the largest slot");
@@ -220,6 +229,26 @@ struct mdx_server {
         return MDX_OK;
     }
 
+    // the job's current frame into its stream (the oldest dropped beyond
+    // frame_queue frames)
+    int stream_frame(Job &j) {
+        Job::Streamed f{j.steps_done, std::vector<mdx_particle64>(static_cast<size_t>(j.n))};
+        int rc = pools[j.pool].b.collect(j.slot, f.frame.data(), j.n);
+        if (rc != MDX_OK) return rc;
+        j.frames.push_back(std::move(f));
+        ++frames_streamed;
+        while (static_cast<int64_t>(j.frames.size()) > std::max(1, p.frame_queue)) {
+            j.frames.pop_front();
+            ++frames_dropped;
+        }
+        return MDX_OK;
+    }
+    // the job's count passed a multiple of its frame interval in the last q steps
+    static bool frame_due(const Job &j, int64_t q) {
+        const int64_t f = j.spec.frame_interval;
+        return f > 0 && j.steps_done / f > (j.steps_done - q) / f;
+    }
+
     int retire(Job &j) {
         Pool &pl = pools[j.pool];
         j.frame.resize(j.n);
@@ -289,6 +318,10 @@ struct mdx_server {
                 for (int s : running) {
                     Job &j = jobs.at(md.job[s]);
                     if (p.record_energies) j.records.push_back({j.steps_done, j.last.K, j.last.U, j.last.P});
+                    if (frame_due(j, q)) {
+                        rc = stream_frame(j);
+                        if (rc != MDX_OK) return rc;
+                    }
                     if (j.steps_done >= j.spec.nsteps) {
                         rc = retire(j);
                         if (rc != MDX_OK) return rc;
@@ -305,6 +338,10 @@ struct mdx_server {
                 int done = 0;
                 rc = fp.b.fire_poll(static_cast<int>(s), &done, &j.fire);
                 if (rc != MDX_OK) return rc;
+                if (!done && frame_due(j, q)) {
+                    rc = stream_frame(j);
+                    if (rc != MDX_OK) return rc;
+                }
                 if (done) {
                     j.last.U = j.fire.potential_energy;
                     rc = retire(j);
@@ -344,6 +381,7 @@ int mdx_server_default_params(mdx_server_params *out) {
     out->device = -1;
     out->list_slots_per_class = 6;
     out->list_skin = 0.5;
+    out->frame_queue = 8;
     return MDX_OK;
 }
 
@@ -407,6 +445,7 @@ int mdx_server_info(mdx_server *s, int64_t id, mdx_job_info *out) {
     out->last = j.last;
     out->fire = j.fire;
     out->n_records = static_cast<int32_t>(j.records.size());
+    out->n_frames = static_cast<int32_t>(j.frames.size());
     return MDX_OK;
 }
 
@@ -447,6 +486,27 @@ int mdx_server_stats(mdx_server *s, mdx_server_counters *out) {
     int64_t slots = 0;
     for (auto &pl : s->pools) slots += static_cast<int64_t>(pl.cap.size());
     out->slots = slots;
+    out->frames_streamed = s->frames_streamed;
+    out->frames_dropped = s->frames_dropped;
+    return MDX_OK;
+}
+
+int mdx_server_frame(mdx_server *s, int64_t id, mdx_particle64 *frame, int64_t cap, int64_t *step, int32_t *got) {
+    if (!s || !got) return set_error(MDX_ERR_INVALID, "null argument");
+    *got = 0;
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return set_error(MDX_ERR_INVALID, "server: unknown job id");
+    Job &j = it->second;
+    if (!frame) {   // peek: the number waiting
+        *got = static_cast<int32_t>(j.frames.size());
+        return MDX_OK;
+    }
+    if (j.frames.empty()) return MDX_OK;
+    if (cap < j.n) return set_error(MDX_ERR_CAPACITY, "output capacity below n");
+    std::memcpy(frame, j.frames.front().frame.data(), j.n * sizeof(mdx_particle64));
+    if (step) *step = j.frames.front().step;
+    j.frames.pop_front();
+    *got = 1;
     return MDX_OK;
 }
 

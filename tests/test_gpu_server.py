@@ -144,3 +144,68 @@ def test_server_rejects_oversized_and_bad_jobs():
     with pytest.raises(abi.MdxError):
         srv.submit(abi.make_job(abi.make_params(Lx, Ly), abi.NVT, 0), f)
     srv.close()
+
+
+def test_streamed_frames_equal_alone_runs_at_their_steps():
+    """Jobs with a frame interval stream a frame at the end of every tick in
+    which their step count reached or passed a multiple of it; each streamed
+    frame equals the same system run alone to the step it reports.  A FIRE
+    job streams while it runs; frames not taken are kept up to frame_queue,
+    newest last."""
+    quantum = 32
+    specs = [(abi.NVT, 1500, 0.80, 0.7, 0.5, 130, 0.0, 41), (abi.NPH, 1200, 0.78, 0.7, 0.5, 90, 1.5, 42),
+             (abi.NVE, 900, 0.75, 0.6, 0.3, 100, 0.0, 43), (abi.FIRE, 700, 0.85, 0.0, 0.5, 150, 0.0, 44)]
+    every = [25, 40, 20, 30]
+    frames = [make(s) for s in specs]
+    srv = abi.Server(min_class_log2=9, max_class_log2=12, md_slots_per_class=2, fire_slots_per_class=1,
+                     quantum=quantum, list_slots_per_class=2, list_skin=LIST_SKIN, frame_queue=3)
+    ids = []
+    for spec, (f, Lx, Ly, kw), fe in zip(specs, frames, every):
+        kind, n, rho, T, eab, nsteps, pext, seed = spec
+        job = abi.make_job(abi.make_params(Lx, Ly, **kw), kind, nsteps, p_ext=pext, force_tol=1e-6, fire=FP,
+                           frame_interval=fe)
+        ids.append(srv.submit(job, f))
+    got = {i: [] for i in range(len(specs))}
+    done, ticks = set(), 0
+    while len(done) < len(specs):
+        _, _, fin = srv.tick()
+        done.update(fin)
+        ticks += 1
+        assert ticks < 1000
+        for i in range(3):   # the MD jobs' frames are taken as they come
+            while (fr := srv.frame(ids[i])) is not None:
+                got[i].append(fr)
+    say(f"streamed: {[[s for s, _ in g] for g in got.values()]} in {ticks} ticks")
+    for i in range(3):
+        spec, (f, Lx, Ly, kw), fe = specs[i], frames[i], every[i]
+        kind, nsteps, pext = spec[0], spec[5], spec[6]
+        steps = [s for s, _ in got[i]]
+        assert steps == sorted(set(steps)) and steps[-1] <= nsteps
+        # every multiple of the interval is covered by the first frame at or
+        # after it, within one tick
+        for m in range(fe, nsteps + 1, fe):
+            first = min(s for s in steps if s >= m)
+            assert first - m < quantum, (i, m, steps)
+        s = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), f)
+        if kind in (abi.NVE, abi.NVT):
+            s.set_tuning(1.0, 1, LIST_SKIN)
+        prev = 0
+        for step, fr in got[i]:
+            s.step(kind, step - prev, p_ext=pext)
+            prev = step
+            ref = s.collect()
+            assert np.array_equal(fr["pos"], ref["pos"]) and np.array_equal(fr["vel"], ref["vel"]), (i, step)
+        s.close()
+    # FIRE: frames not taken until the end; at most three remain, oldest first
+    info = srv.info(ids[3])
+    assert info.state == abi.JOB_DONE and info.n_frames <= 3
+    fire_frames = []
+    while (fr := srv.frame(ids[3])) is not None:
+        fire_frames.append(fr)
+    steps = [s for s, _ in fire_frames]
+    assert steps == sorted(set(steps))
+    assert all(np.isfinite(fr["pos"]).all() for _, fr in fire_frames)
+    st = srv.stats()
+    streamed = sum(len(g) for g in got.values()) + len(fire_frames)
+    assert st.frames_streamed == streamed + st.frames_dropped
+    srv.close()

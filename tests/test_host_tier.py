@@ -1,6 +1,7 @@
 """CPU tests of the host side: C ABI exports, the config JSON against the
 reference's golden file, PFRAME v2 bytes checked independently with Python's
 struct and zlib, and the initial-frame generators."""
+import ctypes
 import hashlib
 import json
 import re
@@ -35,6 +36,36 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
     assert abi.load().mdx_version().startswith(b"mdx")
+
+
+ABI_STRUCTS = [
+    ("Params", "mdx_params"), ("Obs", "mdx_obs"), ("FireParams", "mdx_fire_params"),
+    ("FireResult", "mdx_fire_result"), ("Timing", "mdx_timing"), ("InitParams", "mdx_init_params"),
+    ("Tuning", "mdx_tuning"), ("TuneReport", "mdx_tune_report"), ("ServerParams", "mdx_server_params"),
+    ("Job", "mdx_job"), ("EnergyRecord", "mdx_energy_record"), ("JobInfo", "mdx_job_info"),
+    ("ServerCounters", "mdx_server_counters"),
+]
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """Every ctypes mirror in abi.py has the size and field offsets that a C
+    compiler gives the declaration in include/mdx.h."""
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "mdx.h"', "int main(void) {"]
+    want = []
+    for py, c in ABI_STRUCTS:
+        cls = getattr(abi, py)
+        lines.append(f'printf("%zu\\n", sizeof({c}));')
+        want.append(ctypes.sizeof(cls))
+        for name, *_ in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({c}, {name}));')
+            want.append(getattr(cls, name).offset)
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == want
 
 
 def test_particle_record_layout():
