@@ -875,8 +875,10 @@ constexpr uint32_t kListRows = ~(kListGuard | kListMixed);
 // the pair kernel's wave) gets the same length: the wave's longest list
 // rounded up to a multiple of 4, the rows past a lane's own list holding
 // list_pad.  A lane stages its entries in its own column of LDS and the wave
-// writes whole rows, so the stores are coalesced; lists longer than
-// kListStage take one more scan per kListStage rows.  Between rebuilds no
+// writes whole rows, so the stores are coalesced; the class bit is added as
+// the rows are written (one class load per entry kept rather than per
+// candidate); lists longer than kListStage take one more scan per kListStage
+// rows.  Between rebuilds no
 // particle moves more than skin / 2 from its build position, so a pair comes
 // closer than the FP64 guard distance only if it is closer than
 // guard + skin now; the wave's length carries kListGuard when one is.
@@ -892,6 +894,7 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
     const bool active = li < sp.n;
     const Grid g = sc.g;
     const int2 *img = static_cast<const int2 *>(P.img);
+    const char *imgb = static_cast<const char *>(P.img);
     // r^2 thresholds in fixed-point units squared
     const float rl2 = static_cast<float>(sp.r_list * sp.r_list) * g.q2f;
     const uint32_t cap = static_cast<uint32_t>(sp.list_cap);   // a multiple of 4
@@ -911,7 +914,9 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
     // (with a margin for the FP32 difference between this and the pair form)
     const float rgd = sqrtf(sp.pt.r2_guard) + 2.0f * sqrtf(static_cast<float>(sp.skin_half2)) + 1.0e-3f;
     const float rg2 = rgd * rgd * g.q2f;
-    bool near = false, mixed = false;
+    // the smallest r^2 of any candidate (as its IEEE bit pattern less one)
+    uint32_t near_rb = 0xFFFFFFFFu;
+    bool mixed = false;
     uint32_t rows = 0, cnt = 0;
     for (uint32_t base = 0;; base += kListStage) {
         cnt = 0;
@@ -930,34 +935,34 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
                 const uint32_t b0 = P.cs[rowbase + cx], b1 = P.cs[rowbase + cx + 1];
                 const uint32_t c0 = P.cs[rowbase + xr], c1 = P.cs[rowbase + xr + 1];
                 const uint32_t na = a1 - a0, nab = na + (b1 - b0), nt = nab + (c1 - c0);
-                // candidate t of the run is slot t + off(t): off = a0, b0 - na or
-                // c0 - nab by the cell it falls in; per cell also the particle's
-                // image moved by minus the cell's offset and the entry's cell
-                // code bits (selects, no branches)
-                const uint32_t off1 = a0, off2 = b0 - na, off3 = c0 - nab;
+                // candidate t of the run is slot t + off(t), off = a0, b0 - na
+                // or c0 - nab by the cell it falls in; its entry without the
+                // class bit is (t << 3) + ((off << 3) + code bits) (mod 2^32:
+                // slots are below 2^27, so the slot field neither wraps nor
+                // reaches the code bits), its image offset the entry's slot
+                // field.  Per cell also the particle's image moved by minus the
+                // cell's offset (selects, no branches).  Past the run: the
+                // particle itself one cell to the side, at distance w >= r_list,
+                // never accepted.
+                const uint32_t e1 = (a0 << 3) + list_entry(0u, -1, oy, false);
+                const uint32_t e2 = ((b0 - na) << 3) + list_entry(0u, 0, oy, false);
+                const uint32_t e3 = ((c0 - nab) << 3) + list_entry(0u, 1, oy, false);
+                const uint32_t epast = list_entry(s, 1, oy, false);
                 const int pxa = pi.x + g.wq, pxc = pi.x - g.wq;
-                const uint32_t cba = list_entry(0u, -1, oy, false), cbb = list_entry(0u, 0, oy, false),
-                               cbc = list_entry(0u, 1, oy, false);
                 // four candidates per pass, their loads in flight together
                 for (uint32_t t = 0; t < nt; t += 4) {
-                    uint32_t k[4], cb[4];
+                    uint32_t en[4];
                     int px[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const uint32_t tt = t + static_cast<uint32_t>(u);
                         const bool g1 = tt >= na, g2 = tt >= nab;
-                        // past the run: the particle itself, rejected below
-                        k[u] = tt < nt ? tt + (g2 ? off3 : (g1 ? off2 : off1)) : s;
+                        en[u] = tt < nt ? (tt << 3) + (g2 ? e3 : (g1 ? e2 : e1)) : epast;
                         px[u] = g2 ? pxc : (g1 ? pi.x : pxa);
-                        cb[u] = g2 ? cbc : (g1 ? cbb : cba);
                     }
                     int2 pj[4];
-                    int cj[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        pj[u] = img[k[u]];
-                        cj[u] = P.icls[k[u]];
-                    }
+                    for (int u = 0; u < 4; ++u) pj[u] = *reinterpret_cast<const int2 *>(imgb + (en[u] & kListOff));
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const float dx = static_cast<float>(px[u] - pj[u].x);
@@ -967,17 +972,15 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
                         // cell pass, are not partners) and r^2 < r_list^2, as
                         // IEEE bit patterns
                         const uint32_t rb = __float_as_uint(rr) - 1u;
-                        const bool in_run = t + static_cast<uint32_t>(u) < nt;
-                        const bool acc = in_run && rb < __float_as_uint(rl2) - 1u;
-                        near = near || (in_run && rb < __float_as_uint(rg2) - 1u);
-                        mixed = mixed || (acc && cj[u] != ci);
+                        const bool acc = rb < __float_as_uint(rl2) - 1u;
+                        near_rb = rb < near_rb ? rb : near_rb;
                         // unconditional store: a rejected candidate, or an entry
                         // outside this pass's rows, goes to the spare row
                         const uint32_t rel = cnt - base;
                         const uint32_t row = acc ? (rel < static_cast<uint32_t>(kListStage) ? rel
                                                                                            : static_cast<uint32_t>(kListStage))
                                                  : static_cast<uint32_t>(kListStage);
-                        stage[row][tid] = (k[u] << 3) | cb[u] | (cj[u] == ci ? kListSame : 0u);
+                        stage[row][tid] = en[u];
                         cnt += acc ? 1u : 0u;
                     }
                 }
@@ -995,13 +998,36 @@ __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
         }
         const uint32_t end = rows < base + kListStage ? rows : base + kListStage;
         if (active) {
+            // the rows of this pass (four at a time: rows, base and end are
+            // multiples of 4), each entry with its class bit from the
+            // partner's class; rows past the list hold the padding entry
             const uint32_t pad = list_pad(s);
-            for (uint32_t e = base; e < end; ++e)
-                P.nlist[static_cast<size_t>(e) * stride + s] = e < cnt ? stage[e - base][tid] : pad;
+            for (uint32_t e = base; e < end; e += 4) {
+                uint32_t v[4];
+                int cj[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t ee = e + static_cast<uint32_t>(u);
+                    v[u] = ee < cnt ? stage[ee - base][tid] : pad;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    cj[u] = e + static_cast<uint32_t>(u) < cnt ? P.icls[(v[u] & kListOff) >> 3] : ci;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t ee = e + static_cast<uint32_t>(u);
+                    if (ee < cnt) {
+                        const bool same = cj[u] == ci;
+                        v[u] |= same ? kListSame : 0u;
+                        mixed = mixed || !same;
+                    }
+                    P.nlist[static_cast<size_t>(ee) * stride + s] = v[u];
+                }
+            }
         }
         if (base + kListStage >= rows) break;
     }
-    const bool guard = __any(near), mix = __any(mixed);
+    const bool guard = __any(near_rb < __float_as_uint(rg2) - 1u), mix = __any(mixed);
     if (active) {
         P.ncount[s] = rows | (guard ? kListGuard : 0u) | (mix ? kListMixed : 0u);
         if (cnt > cap) sc.list_overflow = 1;   // this step's pair pass uses the cells instead
