@@ -246,6 +246,16 @@ typedef struct mdx_timing {
 int mdx_timing_enable(mdx_sim *sim, int32_t enable);
 int mdx_timing_read(mdx_sim *sim, mdx_timing *out);
 
+/* Measurement support, no counterpart in the reference: the average duration
+ * (ms) of the pair pass a step of the current tuning runs (the lean-step
+ * variant inside rebuild windows), from reps (1..4096) launches captured back
+ * to back in one graph and timed with HIP events on the handle's stream, so
+ * no per-launch overhead is included.  The launches run on the current state,
+ * which is saved before and restored after (unchanged on return).  Allocates
+ * a temporary copy of the velocities, accelerations and scalars; synchronises
+ * the handle's stream. */
+int mdx_time_pair_pass(mdx_sim *sim, int32_t reps, double *avg_ms);
+
 
 
 /* ---- tuning of the pair pass ---------------------------------------------

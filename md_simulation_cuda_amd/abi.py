@@ -213,6 +213,7 @@ _SIGNATURES = {
                                 C.c_int64, C.c_int64]),
     "mdx_timing_enable": (C.c_int, [_P, C.c_int32]),
     "mdx_timing_read": (C.c_int, [_P, C.POINTER(Timing)]),
+    "mdx_time_pair_pass": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
     "mdx_last_error": (C.c_char_p, []),
     "mdx_version": (C.c_char_p, []),
     "mdx_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
@@ -431,6 +432,13 @@ class Simulation:
         t = Timing()
         check(self._lib.mdx_timing_read(self._h, C.byref(t)))
         return t
+
+    def time_pair_pass(self, reps: int = 100) -> float:
+        """Average pair-pass duration in ms, launches back to back in one
+        graph (no per-launch overhead); the state is left unchanged."""
+        out = C.c_double()
+        check(self._lib.mdx_time_pair_pass(self._h, reps, C.byref(out)))
+        return out.value
 
 
 def cell_capacity(params: Params, growth: float = 1.0) -> int:

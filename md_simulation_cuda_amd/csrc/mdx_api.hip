@@ -273,6 +273,13 @@ int mdx_timing_read(mdx_sim *sim, mdx_timing *out) {
     return timing_read(sim->b, out);
 }
 
+int mdx_time_pair_pass(mdx_sim *sim, int32_t reps, double *avg_ms) {
+    CHECK_HANDLE(sim);
+    if (!avg_ms) return set_error(MDX_ERR_INVALID, "null argument");
+    *avg_ms = 0.0;
+    return sim->b.time_pair_pass(reps, avg_ms);
+}
+
 /* ---- batches ------------------------------------------------------------ */
 
 int mdx_batch_create(int32_t nslots, const int64_t *n_cap, const int32_t *cell_cap, int32_t precision,

@@ -5,6 +5,8 @@
 // piston) and src/md_energy_minimizer.cu:122-272 (FIRE) for one periodic
 // domain per slot: the MPI slab exchange of the reference (src/md_env.cu:
 // 757-1274) is replaced by a cell list rebuilt on the device every step.
+#include <hip/hip_ext.h>
+
 #include "mdx_batch.hpp"
 #include "mdx_kernels.hpp"
 
@@ -357,22 +359,35 @@ void set_idle(SysParams &sp, const Slot &s) {
 }
 }  // namespace
 
+// Launch of a pair kernel; with events e0/e1, through hipExtLaunchKernel,
+// whose start and stop events take the kernel's own start and end times (no
+// dispatch overhead between them).
+template <typename K>
+static void launch_k(K kern, int grid, int block, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                     const BatchPtrs &p, int parity) {
+    if (e0 && e1)
+        hipExtLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, e0, e1, 0, p, static_cast<int32_t>(parity));
+    else
+        kern<<<grid, block, 0, st>>>(p, parity);
+}
+
 // FP32 pair kernel with lpp lanes per particle (1, 2 or 4).
 template <int ACC, bool OBS, bool UNI>
-static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &p, int parity, int lean) {
+static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &p, int parity, int lean,
+                            hipEvent_t e0, hipEvent_t e1) {
     if constexpr (!OBS) {   // lean steps take no energy sums
         if (lean) {
             switch (lpp) {
-            case 2: k_pair<int2, ACC, OBS, UNI, 2, true><<<grid, kBlock * 2, 0, st>>>(p, parity); return;
-            case 4: k_pair<int2, ACC, OBS, UNI, 4, true><<<grid, kBlock * 4, 0, st>>>(p, parity); return;
-            default: k_pair<int2, ACC, OBS, UNI, 1, true><<<grid, kBlock, 0, st>>>(p, parity); return;
+            case 2: launch_k(k_pair<int2, ACC, OBS, UNI, 2, true>, grid, kBlock * 2, st, e0, e1, p, parity); return;
+            case 4: launch_k(k_pair<int2, ACC, OBS, UNI, 4, true>, grid, kBlock * 4, st, e0, e1, p, parity); return;
+            default: launch_k(k_pair<int2, ACC, OBS, UNI, 1, true>, grid, kBlock, st, e0, e1, p, parity); return;
             }
         }
     }
     switch (lpp) {
-    case 2: k_pair<int2, ACC, OBS, UNI, 2, false><<<grid, kBlock * 2, 0, st>>>(p, 0); break;
-    case 4: k_pair<int2, ACC, OBS, UNI, 4, false><<<grid, kBlock * 4, 0, st>>>(p, 0); break;
-    default: k_pair<int2, ACC, OBS, UNI, 1, false><<<grid, kBlock, 0, st>>>(p, 0); break;
+    case 2: launch_k(k_pair<int2, ACC, OBS, UNI, 2, false>, grid, kBlock * 2, st, e0, e1, p, 0); break;
+    case 4: launch_k(k_pair<int2, ACC, OBS, UNI, 4, false>, grid, kBlock * 4, st, e0, e1, p, 0); break;
+    default: launch_k(k_pair<int2, ACC, OBS, UNI, 1, false>, grid, kBlock, st, e0, e1, p, 0); break;
     }
 }
 
@@ -414,42 +429,106 @@ int Batch::launch_step_kernels(bool obs, int mode, int parity) {
         if (nlist && prec != MDX_PREC_FP64) k_list_build<<<TB, kBlock, 0, stream>>>(ptrs);
     }
     if (TB > 0) {
-        const bool timed = timing;
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (timed && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
-            (void)hipEventRecord(e0, stream);
-        const bool uni = uniform_sigma();
-        switch (prec) {
-        case MDX_PREC_COMP32:
-            if (obs) {
-                if (uni) launch_pair_f32<ACC_COMP32, true, true>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
-                else launch_pair_f32<ACC_COMP32, true, false>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
-            } else {
-                if (uni) launch_pair_f32<ACC_COMP32, false, true>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
-                else launch_pair_f32<ACC_COMP32, false, false>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
-            }
-            break;
-        case MDX_PREC_ACC64:
-            if (obs) {
-                if (uni) launch_pair_f32<ACC_FP64, true, true>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
-                else launch_pair_f32<ACC_FP64, true, false>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
-            } else {
-                if (uni) launch_pair_f32<ACC_FP64, false, true>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
-                else launch_pair_f32<ACC_FP64, false, false>(lpp, TB, stream, ptrs, parity, lean ? 1 : 0);
-            }
-            break;
-        default:   // the FP64 form keeps one lane per particle (the oracle's order)
-            if (obs) k_pair<double2, ACC_FP64, true, false, 1, false><<<TB, kBlock, 0, stream>>>(ptrs, 0);
-            else k_pair<double2, ACC_FP64, false, false, 1, false><<<TB, kBlock, 0, stream>>>(ptrs, 0);
-            break;
+        if (timing && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
+            if (e0) (void)hipEventDestroy(e0);
+            e0 = e1 = nullptr;
         }
-        if (e1) {
-            (void)hipEventRecord(e1, stream);
-            force_ev.emplace_back(e0, e1);
-        }
+        launch_pair(obs, parity, lean, e0, e1);
+        if (e0 && e1) force_ev.emplace_back(e0, e1);
     }
     MDX_HIP(hipGetLastError());
     return MDX_OK;
+}
+
+// The pair kernel of the batch's precision, lanes and sigma form; lean: a
+// rebuild window's lean-step variant with step parity `parity`; e0/e1: events
+// for the kernel's start and end (launch_k), or null.
+void Batch::launch_pair(bool obs, int parity, bool lean, hipEvent_t e0, hipEvent_t e1) {
+    const bool uni = uniform_sigma();
+    const int ln = lean ? 1 : 0;
+    switch (prec) {
+    case MDX_PREC_COMP32:
+        if (obs) {
+            if (uni) launch_pair_f32<ACC_COMP32, true, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_COMP32, true, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+        } else {
+            if (uni) launch_pair_f32<ACC_COMP32, false, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_COMP32, false, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+        }
+        break;
+    case MDX_PREC_ACC64:
+        if (obs) {
+            if (uni) launch_pair_f32<ACC_FP64, true, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_FP64, true, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+        } else {
+            if (uni) launch_pair_f32<ACC_FP64, false, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_FP64, false, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+        }
+        break;
+    default:   // the FP64 form keeps one lane per particle (the oracle's order)
+        if (obs) launch_k(k_pair<double2, ACC_FP64, true, false, 1, false>, TB, kBlock, stream, e0, e1, ptrs, 0);
+        else launch_k(k_pair<double2, ACC_FP64, false, false, 1, false>, TB, kBlock, stream, e0, e1, ptrs, 0);
+        break;
+    }
+}
+
+// Duration of the pair pass a step of the current tuning runs (the lean-step
+// variant when rebuild windows apply), without launch overhead: reps launches
+// of it captured back to back in one graph, timed with HIP events around the
+// graph's launch on the batch's stream.  The launches run on the current
+// state; the arrays and scalars they write (velocities, accelerations, the
+// reduction partials, the slots' scalars) are saved before and restored
+// after, so the state is unchanged.
+int Batch::time_pair_pass(int reps, double *avg_ms) {
+    if (reps < 1 || reps > 4096) return set_error(MDX_ERR_INVALID, "reps must be in [1, 4096]");
+    if (TB <= 0) return set_error(MDX_ERR_STATE, "no particles");
+    MDX_HIP(hipSetDevice(device));
+    MDX_TRY(flush());
+    MDX_HIP(hipStreamSynchronize(stream));
+    const bool lean = windows_apply();
+    const size_t nv = static_cast<size_t>(std::max<int64_t>(N, 1)) * sizeof(double2);
+    const size_t na = static_cast<size_t>(std::max<int64_t>(N, 1)) * (prec == MDX_PREC_COMP32 ? sizeof(float2)
+                                                                                               : sizeof(double2));
+    const size_t np = static_cast<size_t>(TB) * 3 * sizeof(double);
+    const size_t ns = slots.size() * sizeof(DevScalars);
+    void *save = nullptr;
+    MDX_HIP(hipMalloc(&save, nv + na + np + ns));
+    char *sv = static_cast<char *>(save);
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MDX_OK;
+    auto hip = [&](hipError_t e) {
+        if (e != hipSuccess && rc == MDX_OK) rc = set_error(MDX_ERR_HIP, hipGetErrorString(e));
+        return e == hipSuccess;
+    };
+    if (hip(hipMemcpyAsync(sv, v, nv, hipMemcpyDeviceToDevice, stream)) &&
+        hip(hipMemcpyAsync(sv + nv, a, na, hipMemcpyDeviceToDevice, stream)) &&
+        hip(hipMemcpyAsync(sv + nv + na, partials, np, hipMemcpyDeviceToDevice, stream)) &&
+        hip(hipMemcpyAsync(sv + nv + na + np, d_sc, ns, hipMemcpyDeviceToDevice, stream)) &&
+        hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal))) {
+        for (int i = 0; i < reps; ++i) launch_pair(false, 0, lean, nullptr, nullptr);
+        const bool ended = hip(hipStreamEndCapture(stream, &g));
+        if (ended && hip(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0)) && hip(hipEventCreate(&e0)) &&
+            hip(hipEventCreate(&e1)) && hip(hipEventRecord(e0, stream)) && hip(hipGraphLaunch(ge, stream)) &&
+            hip(hipEventRecord(e1, stream)) && hip(hipEventSynchronize(e1))) {
+            float ms = 0.0f;
+            if (hip(hipEventElapsedTime(&ms, e0, e1))) *avg_ms = static_cast<double>(ms) / reps;
+        }
+    }
+    // the saved state back whatever happened above
+    (void)hipMemcpyAsync(v, sv, nv, hipMemcpyDeviceToDevice, stream);
+    (void)hipMemcpyAsync(a, sv + nv, na, hipMemcpyDeviceToDevice, stream);
+    (void)hipMemcpyAsync(partials, sv + nv + na, np, hipMemcpyDeviceToDevice, stream);
+    (void)hipMemcpyAsync(d_sc, sv + nv + na + np, ns, hipMemcpyDeviceToDevice, stream);
+    hip(hipStreamSynchronize(stream));
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(save);
+    return rc;
 }
 
 // count passes of the step sequence; replayed from a captured hipGraph unless

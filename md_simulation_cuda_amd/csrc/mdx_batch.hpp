@@ -160,6 +160,8 @@ struct Batch {
     int upload_sp(int r);
     int upload_all_sp();
     int launch_step_kernels(bool obs, int mode, int parity);
+    void launch_pair(bool obs, int parity, bool lean, hipEvent_t e0, hipEvent_t e1);
+    int time_pair_pass(int reps, double *avg_ms);   // pair kernel duration, launches back to back
     int run_passes(int count, bool obs);
     int replay(bool obs, int mode, int count, int parity = 0);
     int capture_steps(hipGraphExec_t &ge, bool obs, int mode, int parity, int steps);
