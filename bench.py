@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-vs-FP64 error report")
+    ap.add_argument("--no-back-to-back", action="store_true",
+                    help="skip timing the pair kernel back to back on an unchanged state (its 100 extra "
+                         "launches would enter a profiler's average for the kernel)")
     ap.add_argument("--workload", default="nvt1m",
                     choices=["nvt1m", "ensemble", "serve", "fire_nph", "nve4k", "long", "dropin", "liquidgas"],
                     help="nvt1m: N=1M Nose-Hoover (default line); ensemble: 256 NVT replicas of uneven N "
@@ -108,7 +111,8 @@ def in_cutoff_pairs_per_atom(rho):
     return rho * math.pi * 2.5 ** 2
 
 
-def run_gpu(frame, Lx, Ly, kw, prec, steps, warmup, tune_steps, fixed_tuning=None, timed_steps=200):
+def run_gpu(frame, Lx, Ly, kw, prec, steps, warmup, tune_steps, fixed_tuning=None, timed_steps=200,
+            back_to_back=True):
     """Timed run (each step replayed from a captured hipGraph), then a shorter
     eager run with HIP events around every pair-kernel launch."""
     from md_simulation_cuda_amd import abi
@@ -143,7 +147,7 @@ def run_gpu(frame, Lx, Ly, kw, prec, steps, warmup, tune_steps, fixed_tuning=Non
     run_gpu.warmup_steps = warm
     # the pair kernel's own duration: launches back to back in one graph,
     # HIP events around it on the library's stream (state left unchanged)
-    run_gpu.pair_graph_ms = sim.time_pair_pass(100)
+    run_gpu.pair_graph_ms = sim.time_pair_pass(100) if back_to_back else None
     sim.timing_enable(True)
     sim.step(abi.NVT, min(steps, timed_steps))
     tm = sim.timing_read()
@@ -919,7 +923,8 @@ def main():
     if args.tuning:
         parts = args.tuning.split(",") + ["0"]
         fixed = (float(parts[0]), int(parts[1]), float(parts[2]), int(parts[3]))
-    wall, tm, obs, tune, final = run_gpu(frame, Lx, Ly, kw, prec, args.steps, args.warmup, args.tune_steps, fixed)
+    wall, tm, obs, tune, final = run_gpu(frame, Lx, Ly, kw, prec, args.steps, args.warmup, args.tune_steps, fixed,
+                                         back_to_back=not args.no_back_to_back)
     value = n * args.steps / wall
     ms_step = wall * 1e3 / args.steps
     # the roofline's launch duration: every pair launch of the eager run timed
@@ -1000,7 +1005,7 @@ def main():
     }
     if not args.no_plain and prec == abi.PREC_COMP32:
         wall2, tm2, _, tune2, _ = run_gpu(frame, Lx, Ly, kw, abi.PREC_ACC64, args.steps, args.warmup,
-                                          args.tune_steps, fixed)
+                                          args.tune_steps, fixed, back_to_back=not args.no_back_to_back)
         out["plain_fp64_accumulation"] = {
             "value": n * args.steps / wall2,
             "ms_per_step": wall2 * 1e3 / args.steps,
