@@ -1039,13 +1039,14 @@ int Batch::step(int nsteps) {
         s.K_valid = s.kind == SK_NVT || s.kind == SK_NPH;
         s.W_valid = s.kind == SK_NPH;
     }
-    if (nph) {   // the reference aborts on an invalid proposed area
-        for (size_t r = 0; r < slots.size(); ++r) {
-            if (!slots[r].loaded || slots[r].kind != SK_NPH) continue;
-            DevScalars h;
-            MDX_TRY(read_scalars(static_cast<int>(r), h));
-            if (h.box_error) return set_error(MDX_ERR_BOX, "step_single_NPH: invalid proposed area");
-        }
+    if (nph) {   // the reference aborts on an invalid proposed area: every slot's flag in one read
+        MDX_TRY(flush());
+        std::vector<DevScalars> h(slots.size());
+        MDX_HIP(hipMemcpyAsync(h.data(), d_sc, h.size() * sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+        MDX_HIP(hipStreamSynchronize(stream));
+        for (size_t r = 0; r < slots.size(); ++r)
+            if (slots[r].loaded && slots[r].kind == SK_NPH && h[r].box_error)
+                return set_error(MDX_ERR_BOX, "step_single_NPH: invalid proposed area");
     }
     return MDX_OK;
 }

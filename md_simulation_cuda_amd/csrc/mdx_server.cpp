@@ -12,8 +12,9 @@
 //      fits them, while the bytes of the admitted slots stay within the HBM
 //      budget (a job that does not fit now does not block later ones);
 //   2. advances every running job by the same number of steps q (at most the
-//      quantum and at most any MD job's remaining steps); the FIRE pool's
-//      launches are queued first so the pools overlap on the device;
+//      quantum and at most any MD job's remaining steps); every pool's
+//      launches are queued before the host waits on any of them, so the pools
+//      overlap on the device;
 //   3. records K, U, P of every running MD job (one kinetic and one energy
 //      pass per pool), copies the current frame of each job that streams
 //      frames (frame_interval f > 0) and whose step count passed a multiple of
@@ -281,8 +282,11 @@ struct mdx_server {
             }
         if (any) {
             ++ticks;
-            // FIRE pool first (it does not wait on the host), then the MD pools
-            for (int pi : {kPoolFire, kPoolMD, kPoolList}) {
+            // FIRE pool first, then the neighbour-list pool, then the cell-list
+            // pool, whose NPH jobs end their step call with a read of the box
+            // flags that waits for that pool's work: the pools' launches are
+            // all queued before the host waits, so they overlap on the device
+            for (int pi : {kPoolFire, kPoolList, kPoolMD}) {
                 Pool &pl = pools[pi];
                 if (std::none_of(pl.job.begin(), pl.job.end(), [](int64_t x) { return x >= 0; })) continue;
                 rc = pl.b.step(static_cast<int>(q));
