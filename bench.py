@@ -329,7 +329,13 @@ def run_ensemble(args):
         b.load(r, abi.make_params(Lx, Ly, precision=prec, **kw), f)
         b.set_kind(r, abi.NVT)
     tune = None
-    if args.tune_steps > 0:
+    if args.tuning:
+        parts = args.tuning.split(",") + ["0"]
+        cs, lanes, skin, window = float(parts[0]), int(parts[1]), float(parts[2]), int(parts[3])
+        b.set_tuning(cs, lanes, skin, window)
+        tune = {"cell_scale": cs, "lanes_per_particle": lanes, "skin": skin, "window": window,
+                "candidates_ms_per_step": None}
+    elif args.tune_steps > 0:
         rep = b.autotune(args.tune_steps)
         tune = {"cell_scale": rep.best.cell_scale, "lanes_per_particle": rep.best.lanes_per_particle,
                 "skin": rep.best.skin, "window": rep.best.window,
@@ -374,8 +380,11 @@ def run_ensemble(args):
             s.close()
         return {"value": total_n * k / wall_pr, "steps": k, "wall_s": wall_pr, "window": window}
 
-    pr_same = per_replica(tune["window"] if tune is not None else 0)
-    pr_nowin = per_replica(0) if tune is not None and tune["window"] > 1 else pr_same
+    if args.per_replica_steps > 0:
+        pr_same = per_replica(tune["window"] if tune is not None else 0)
+        pr_nowin = per_replica(0) if tune is not None and tune["window"] > 1 else pr_same
+    else:   # --per-replica-steps 0: the batched leg only
+        pr_same = pr_nowin = None
     # the FP64 oracle on the host cores: NVT steps of a sample of the replicas
     cpu = None
     if not args.no_cpu:
@@ -410,8 +419,8 @@ def run_ensemble(args):
                    "replicas": len(reps), "total_atoms": total_n},
         "pair_avg_launch_ms": tm.force_ms / max(1, tm.force_launches),
         "per_replica_launches": {"same_tuning": pr_same, "no_windows": pr_nowin},
-        "speedup_vs_per_replica": value / pr_same["value"],
-        "speedup_vs_per_replica_no_windows": value / pr_nowin["value"],
+        "speedup_vs_per_replica": value / pr_same["value"] if pr_same else None,
+        "speedup_vs_per_replica_no_windows": value / pr_nowin["value"] if pr_nowin else None,
         "tuning": tune,
         "cpu_baseline": cpu,
     }
