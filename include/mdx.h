@@ -246,6 +246,18 @@ typedef struct mdx_timing {
 int mdx_timing_enable(mdx_sim *sim, int32_t enable);
 int mdx_timing_read(mdx_sim *sim, mdx_timing *out);
 
+/* ---- streams ------------------------------------------------------------ */
+/* The reference's methods synchronise after every launch; here a handle
+ * issues its work on one HIP stream and returns early.  mdx_get_stream gives
+ * that stream (a hipStream_t) so a caller can order its own work with the
+ * library's (an event recorded on it, hipStreamWaitEvent).  mdx_set_stream
+ * makes the handle issue everything after the call on the caller's stream
+ * (a hipStream_t of the handle's device, not the null stream; the caller keeps
+ * it alive until the handle is destroyed or set back); null returns to the
+ * handle's own stream.  Work already issued completes before it returns. */
+int mdx_get_stream(mdx_sim *sim, void **stream);
+int mdx_set_stream(mdx_sim *sim, void *stream);
+
 /* Measurement support, no counterpart in the reference: the average duration
  * (ms) of the pair pass a step of the current tuning runs (the lean-step
  * variant inside rebuild windows), from reps (1..4096) launches captured back

@@ -214,6 +214,8 @@ _SIGNATURES = {
     "mdx_timing_enable": (C.c_int, [_P, C.c_int32]),
     "mdx_timing_read": (C.c_int, [_P, C.POINTER(Timing)]),
     "mdx_time_pair_pass": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
+    "mdx_get_stream": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
+    "mdx_set_stream": (C.c_int, [_P, C.c_void_p]),
     "mdx_last_error": (C.c_char_p, []),
     "mdx_version": (C.c_char_p, []),
     "mdx_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
@@ -432,6 +434,16 @@ class Simulation:
         t = Timing()
         check(self._lib.mdx_timing_read(self._h, C.byref(t)))
         return t
+
+    def get_stream(self) -> int:
+        """The handle's HIP stream (hipStream_t as an integer)."""
+        out = C.c_void_p()
+        check(self._lib.mdx_get_stream(self._h, C.byref(out)))
+        return out.value or 0
+
+    def set_stream(self, stream: int) -> None:
+        """Issue the handle's work on the caller's HIP stream (0: its own)."""
+        check(self._lib.mdx_set_stream(self._h, C.c_void_p(stream or None)))
 
     def time_pair_pass(self, reps: int = 100) -> float:
         """Average pair-pass duration in ms, launches back to back in one

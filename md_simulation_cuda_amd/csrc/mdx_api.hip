@@ -273,6 +273,18 @@ int mdx_timing_read(mdx_sim *sim, mdx_timing *out) {
     return timing_read(sim->b, out);
 }
 
+int mdx_get_stream(mdx_sim *sim, void **stream) {
+    CHECK_HANDLE(sim);
+    if (!stream) return set_error(MDX_ERR_INVALID, "null argument");
+    *stream = static_cast<void *>(sim->b.stream);
+    return MDX_OK;
+}
+
+int mdx_set_stream(mdx_sim *sim, void *stream) {
+    CHECK_HANDLE(sim);
+    return sim->b.set_stream(static_cast<hipStream_t>(stream));
+}
+
 int mdx_time_pair_pass(mdx_sim *sim, int32_t reps, double *avg_ms) {
     CHECK_HANDLE(sim);
     if (!avg_ms) return set_error(MDX_ERR_INVALID, "null argument");

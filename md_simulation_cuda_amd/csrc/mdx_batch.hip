@@ -124,7 +124,25 @@ Batch::~Batch() {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
-    if (stream) (void)hipStreamDestroy(stream);
+    if (own_stream && own_stream != stream) (void)hipStreamSynchronize(own_stream);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+}
+
+// Issues every later launch and copy on s, a stream of the batch's device
+// owned by the caller (null: back to the batch's own stream).  The work
+// already issued completes first; captured graphs do not depend on the
+// stream they are launched on and stay valid.
+int Batch::set_stream(hipStream_t s) {
+    MDX_HIP(hipSetDevice(device));
+    if (s) {
+        int dev = -1;
+        MDX_HIP(hipStreamGetDevice(s, &dev));
+        if (dev != device) return set_error(MDX_ERR_INVALID, "stream of another device");
+    }
+    MDX_TRY(flush());
+    MDX_HIP(hipStreamSynchronize(stream));
+    stream = s ? s : own_stream;
+    return MDX_OK;
 }
 
 void Batch::invalidate_graphs() {
@@ -156,7 +174,8 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     if (device_ordinal >= ndev) return set_error(MDX_ERR_INVALID, "device ordinal out of range");
     if (device_ordinal >= 0) MDX_HIP(hipSetDevice(device_ordinal));
     MDX_HIP(hipGetDevice(&device));
-    MDX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    MDX_HIP(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
+    stream = own_stream;
     prec = precision;
     slots.assign(nslots, Slot{});
     h_sp.assign(nslots, SysParams{});

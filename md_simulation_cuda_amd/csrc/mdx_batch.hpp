@@ -60,7 +60,8 @@ constexpr int kReorderDue = 1 << 30;
 
 struct Batch {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // the stream every launch and copy is issued on
+    hipStream_t own_stream = nullptr;   // the stream the batch created (destroyed with it)
     int prec = MDX_PREC_COMP32;
     std::vector<Slot> slots;
     int64_t N = 0;     // total particle capacity
@@ -162,6 +163,7 @@ struct Batch {
     int launch_step_kernels(bool obs, int mode, int parity);
     void launch_pair(bool obs, int parity, bool lean, hipEvent_t e0, hipEvent_t e1);
     int time_pair_pass(int reps, double *avg_ms);   // pair kernel duration, launches back to back
+    int set_stream(hipStream_t s);                  // issue on s (null: the batch's own stream)
     int run_passes(int count, bool obs);
     int replay(bool obs, int mode, int count, int parity = 0);
     int capture_steps(hipGraphExec_t &ge, bool obs, int mode, int parity, int steps);

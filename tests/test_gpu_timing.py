@@ -41,3 +41,43 @@ def test_time_pair_pass_leaves_state_unchanged(skin, window):
     for k in ("pos", "vel", "acc"):
         assert np.array_equal(ga[k], gb[k]), k
     assert (oa.K, oa.U, oa.P, oa.xi) == (ob.K, ob.U, ob.P, ob.xi)
+
+
+def _hip():
+    import ctypes
+    for name in ("libamdhip64.so", "libamdhip64.so.7", "libamdhip64.so.6"):
+        try:
+            return ctypes.CDLL(name)
+        except OSError:
+            continue
+    pytest.skip("libamdhip64 not found")
+
+
+def test_caller_stream_gives_the_same_run():
+    import ctypes
+    hip = _hip()
+    st = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(st)) == 0
+    f, Lx, Ly = lattice_frame(12000, 0.75, seed=11, jitter=0.05, T=0.7, aspect=2.0)
+    kw = dict(eps_ab=0.5, dt=0.005, T_target=0.7, Q=10.0)
+    say("caller stream")
+    a = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+    b = abi.Simulation(abi.make_params(Lx, Ly, **kw), f)
+    for s in (a, b):
+        s.set_tuning(1.0, 1, 0.5, 12)
+    own = b.get_stream()
+    assert own != 0
+    b.set_stream(st.value)
+    assert b.get_stream() == st.value
+    for s in (a, b):
+        s.step(abi.NVT, 30)
+    b.set_stream(0)   # back to its own stream
+    assert b.get_stream() == own
+    for s in (a, b):
+        s.step(abi.NVT, 20)
+    ga, gb = a.collect(), b.collect()
+    a.close()
+    b.close()
+    assert hip.hipStreamDestroy(st) == 0
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(ga[k], gb[k]), k
