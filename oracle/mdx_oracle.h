@@ -19,11 +19,12 @@
  *   FIRE                          src/md_energy_minimizer.cu:24-71,122-272,442-475
  *   periodic wrap                 include/md_cuda_common.hpp:20-29
  *
- * What pins each function is listed with the tests (tests/test_oracle.py).
- * FIRE: the fixed point of the reference's two-particle case and the
- * max_steps / immediate-convergence semantics are pinned; the dt / alpha
- * schedule constants are otherwise parity unpinned (no reference vector or
- * closed form exercises them).
+ * What pins each function is listed with the tests (tests/test_oracle.py,
+ * tests/test_oracle_pins.py).  FIRE: the fixed point of the reference's
+ * two-particle case, the max_steps / immediate-convergence semantics, and the
+ * whole dt / alpha / streak schedule, step for step against an independent
+ * numpy FIRE written from Bitzek et al. (PRL 97, 170201, 2006) in the
+ * reference's step order, on runs that visit every branch of the schedule.
  *
  * Deliberate, documented differences from the reference:
  *   - one periodic domain: every pair is counted once per ordered pair (the
