@@ -569,6 +569,16 @@ def run_fire_nph(args):
     sim.set_precision(abi.PREC_COMP32)
     sim.set_state(f)
     sim.step(abi.NPH, 50, p_ext=1.0)
+    # the NPH phase's tuning: the cell list every step against neighbour lists
+    # rebuilt on demand (the barostat's scaling is part of the rebuild test)
+    nph_tune = None
+    if args.tune_steps > 0:
+        rep = sim.autotune(abi.NPH, args.tune_steps)
+        nph_tune = {"cell_scale": rep.best.cell_scale, "lanes_per_particle": rep.best.lanes_per_particle,
+                    "skin": rep.best.skin,
+                    "candidates_ms_per_step": [[c, l, k, w, round(m, 5)] for c, l, k, w, m in rep.table()
+                                               if w == 0]}
+        sim.step(abi.NPH, 50, p_ext=1.0)
     sim.sync()
     steps = args.steps if args.steps != 2000 else 5000
     t0 = time.perf_counter()
@@ -625,6 +635,7 @@ def run_fire_nph(args):
                  "max_force": res.max_force, "pe_per_particle": res.potential_energy / n, "wall_s": t_fire,
                  "iterations_per_s": res.steps / t_fire},
         "nph_final": {"P": ob.P, "Lx": ob.Lx, "Ly": ob.Ly, "K_per_atom": ob.K / n, "U_per_atom": ob.U / n},
+        "nph_tuning": nph_tune,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -275,12 +275,17 @@ int mdx_time_pair_pass(mdx_sim *sim, int32_t reps, double *avg_ms);
  * cell_scale: cell side >= cell_scale * cutoff * sigma_max (>= 1).
  * lanes_per_particle: 1, 2 or 4 lanes share one particle's neighbour cells
  * (64, 32 or 16 particles per wavefront); the FP64 form always uses 1.
- * skin: with skin > 0, NVE and NVT slots in the FP32 forms keep a list of the
- * partners within cutoff + skin and rebuild it (and the cell list, with cells
- * of at least cutoff + skin) at the step where some particle has moved more
- * than skin / 2 since the last build; the pairs evaluated are exactly those
- * within the cutoff, as without the list.  skin = 0: cell list every step.
- * window: with window W > 1 and every stepping slot in list mode (boxes of at
+ * skin: with skin > 0, NVE, NVT and NPH slots in the FP32 forms keep a list
+ * of the partners within cutoff + skin and rebuild it (and the cell list, with
+ * cells of at least cutoff + skin) at the step where some particle has moved
+ * more than skin / 2 since the last build; the pairs evaluated are exactly
+ * those within the cutoff, as without the list.  NPH: the barostat scales
+ * every position by the same factor s between builds, so a particle counts as
+ * moved by its distance from its scaled build position, against
+ * (s (cutoff + skin) - cutoff) / 2, and a step that changes the cell grid's
+ * dimensions rebuilds.  skin = 0: cell list every step.
+ * window: with window W > 1 and every stepping slot an NVE or NVT slot in list
+ * mode (boxes of at
  * least 5 cells per side), steps run in windows of W: the first (head) step
  * rebuilds a slot's lists when some particle, at 1.5 x its current speed,
  * could move beyond skin / 2 before the window ends; the other W - 1 steps run
@@ -298,7 +303,7 @@ int mdx_time_pair_pass(mdx_sim *sim, int32_t reps, double *avg_ms);
  * the same tuning). */
 typedef struct mdx_tuning {
     double cell_scale;
-    double skin;                 /* > 0: neighbour-list mode (FP32 forms, NVE/NVT slots) */
+    double skin;                 /* > 0: neighbour-list mode (FP32 forms, NVE/NVT/NPH slots) */
     int32_t lanes_per_particle;
     int32_t window;              /* list mode: steps per rebuild window (0: rebuild on demand) */
 } mdx_tuning;
@@ -380,13 +385,13 @@ int mdx_batch_autotune(mdx_batch *b, int32_t steps, mdx_tune_report *out);
  * admitted slots within hbm_budget_bytes, arrival order without head-of-line
  * blocking), advances every running job by up to `quantum` steps in the same
  * launches, records K/U/P of every running MD job, and retires finished jobs
- * (final frame copied to the host, slot freed).  NVE and NVT jobs run in a
- * batch of md_precision with neighbour lists (skin list_skin, rebuilt on
- * demand), NPH and piston jobs in a batch of md_precision with the cell list
- * every step, FIRE jobs in an ACC64 batch; the batches' launches overlap on the
+ * (final frame copied to the host, slot freed).  NVE, NVT and NPH jobs run in
+ * a batch of md_precision with neighbour lists (skin list_skin, rebuilt on
+ * demand), piston jobs in a batch of md_precision with the cell list every
+ * step, FIRE jobs in an ACC64 batch; the batches' launches overlap on the
  * device.  A job's result is the same as the same system run alone through
- * mdx_create + mdx_step (NVE / NVT with mdx_set_tuning {1.0, list_skin, 1, 0}
- * in the FP32 forms) or mdx_fire with fp64_switch = 0. */
+ * mdx_create + mdx_step (NVE / NVT / NPH with mdx_set_tuning {1.0, list_skin,
+ * 1, 0} in the FP32 forms) or mdx_fire with fp64_switch = 0. */
 #define MDX_FIRE 4            /* mdx_job.kind for a FIRE minimisation */
 #define MDX_JOB_QUEUED 0
 #define MDX_JOB_RUNNING 1
@@ -398,14 +403,14 @@ typedef struct mdx_server_params {
     double max_box_growth;        /* box growth the slots' cells allow (default 1.4)     */
     int32_t min_class_log2;       /* smallest slot: 2^min_class_log2 particles (10)      */
     int32_t max_class_log2;       /* largest slot (18)                                   */
-    int32_t md_slots_per_class;   /* MD slots per size class (6)                         */
+    int32_t md_slots_per_class;   /* cell-list MD slots (piston jobs) per size class (2) */
     int32_t fire_slots_per_class; /* FIRE slots per size class (2)                       */
     int32_t quantum;              /* most steps per tick (100)                           */
     int32_t record_energies;      /* 1: K/U/P of every running MD job every tick         */
     int32_t md_precision;         /* mdx_precision of the MD batches (COMP32)            */
     int32_t device;               /* -1: current                                         */
-    int32_t list_slots_per_class; /* NVE / NVT slots per size class, in a batch with
-                                     neighbour lists (6; 0: they share the MD slots)     */
+    int32_t list_slots_per_class; /* NVE / NVT / NPH slots per size class, in a batch
+                                     with neighbour lists (12; 0: they use the MD slots) */
     int32_t frame_queue;          /* streamed frames a job holds until taken (8); a
                                      further frame drops the oldest                      */
     double list_skin;             /* neighbour-list skin of those slots (0.5)           */

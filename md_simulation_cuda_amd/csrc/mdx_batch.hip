@@ -553,7 +553,7 @@ int Batch::time_pair_pass(int reps, double *avg_ms) {
 // count passes of the step sequence; replayed from a captured hipGraph unless
 // kernel timing is on.
 bool Batch::list_kind(int kind) const {
-    return nlist && skin > 0.0 && prec != MDX_PREC_FP64 && (kind == SK_NVE || kind == SK_NVT);
+    return nlist && skin > 0.0 && prec != MDX_PREC_FP64 && (kind == SK_NVE || kind == SK_NVT || kind == SK_NPH);
 }
 
 bool Batch::uniform_sigma() const {
@@ -612,7 +612,8 @@ bool Batch::windows_apply() {
     for (size_t r = 0; r < slots.size(); ++r) {
         Slot &sl = slots[r];
         if (!sl.loaded || h_sp[r].kind == SK_NONE) continue;
-        if (!h_sp[r].list_mode) return false;
+        // lean steps are NVE / NVT steps (an NPH slot in list mode rebuilds on demand)
+        if (!h_sp[r].list_mode || (h_sp[r].kind != SK_NVE && h_sp[r].kind != SK_NVT)) return false;
         // the lean-step fallback needs 5 distinct cells per side; list-mode
         // kinds (NVE, NVT) keep the box, which is read back once after an NPH
         // or piston run may have changed it
@@ -771,6 +772,7 @@ int Batch::load(int r, const mdx_params &p, const mdx_particle64 *frame, int64_t
     MDX_TRY(reset_ids(r));
     DevScalars h{};
     h.g = g;
+    h.lin_cum = 1.0;
     h.gamma = p.Lx / p.Ly;
     h.Ly_floor = std::max(1e-12, p.barostat_height_min_ratio * p.Ly);
     h.Ly_ceil = std::max(h.Ly_floor, p.barostat_height_max_ratio * p.Ly);

@@ -147,6 +147,7 @@ __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
                 const double lin = sqrt(area_new / area_old);
                 sc.nph_lin = lin;
                 sc.nph_vsc = 1.0 / lin;
+                sc.lin_cum *= lin;
                 sc.gamma = gam;
                 sc.clamped = clamped;
                 Lx = Lx_new;
@@ -188,7 +189,10 @@ __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
     // outside list mode every step rebuilds the cell list
     const bool on = sp.list_mode && !sc.list_overflow && g.periodic_y && g.w >= sp.r_list && g.h >= sp.r_list;
     sc.list_on = on ? 1 : 0;
-    sc.rebuild = (!on || !sc.list_built || mode == STEP_REBUILD) ? 1 : 0;
+    // a barostat step that changes the grid's dimensions renumbers the cells
+    // the lists refer to
+    const bool regrid = g.ncx != sc.g_old.ncx || g.ncy != sc.g_old.ncy;
+    sc.rebuild = (!on || !sc.list_built || mode == STEP_REBUILD || regrid) ? 1 : 0;
     sc.fallback[0] = sc.fallback[1] = 0;   // (lean steps: k_lean of the step before clears its own)
 }
 
@@ -353,6 +357,20 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t ahead) {
         if (sc.list_on && sc.list_built) {
             const double2 p0 = P.x0[i];
             double dx = p.x - p0.x, dy = p.y - p0.y;
+            double lim2 = sp.skin_half2;
+            if (kind == SK_NPH) {
+                // the barostat has scaled every position by s = lin_cum since
+                // the build, so two particles that were r >= r_list apart
+                // are at least s r_list - |d_i| - |d_j| apart, d the
+                // displacement from the scaled build position: the list holds
+                // every pair inside r_c while each |d| <= (s r_list - r_c) / 2
+                // (lowered by 1e-9 for rounding)
+                const double sc_ = sc.lin_cum;
+                dx = p.x - sc_ * p0.x;
+                dy = p.y - sc_ * p0.y;
+                const double a = 0.5 * (sc_ * sp.r_list - sp.rc_max) - 1.0e-9;
+                lim2 = a > 0.0 ? a * a : -1.0;
+            }
             if (dx > 0.5 * g.Lx) dx -= g.Lx;
             else if (dx < -0.5 * g.Lx) dx += g.Lx;
             if (dy > 0.5 * g.Ly) dy -= g.Ly;
@@ -362,7 +380,7 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t ahead) {
                 const double rr = sqrt(r2) + 1.5 * ahead * sp.dt * sqrt(speed2);
                 r2 = rr * rr;
             }
-            if (r2 > sp.skin_half2) sc.rebuild = 1;   // same value from every writer
+            if (r2 > lim2) sc.rebuild = 1;   // same value from every writer
         }
         P.key[i] = c;
         mover_id = i;
@@ -671,6 +689,7 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
         if (sp.list_mode) {
             P.x0[id] = p;
             P.key_list[id] = P.key[id];
+            if (li == 0) sc.lin_cum = 1.0;   // positions are unscaled since this build
         }
     }
 }

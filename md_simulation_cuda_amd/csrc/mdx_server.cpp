@@ -2,12 +2,13 @@
 // slots.  No counterpart in the reference, whose drivers (run/*.cpp) run one
 // system per process from start to end.
 //
-// Three batches ("pools") hold fixed slots in power-of-two size classes: NVE
-// and NVT jobs in a batch of the server's MD precision with neighbour lists
-// (rebuilt on demand, so a job's rebuild points depend only on its own
-// particles), NPH and piston jobs in a batch of that precision with the cell
-// list every step, FIRE jobs in an FP64-accelerating batch (FIRE needs
-// accelerations beyond FP32 range at overlapping starts).  Each tick:
+// Three batches ("pools") hold fixed slots in power-of-two size classes: NVE,
+// NVT and NPH jobs in a batch of the server's MD precision with neighbour
+// lists (rebuilt on demand, so a job's rebuild points depend only on its own
+// particles; for NPH the rebuild test follows the barostat's scaling), piston
+// jobs in a batch of that precision with the cell list every step, FIRE jobs
+// in an FP64-accelerating batch (FIRE needs accelerations beyond FP32 range
+// at overlapping starts).  Each tick:
 //   1. admits queued jobs in arrival order into the smallest free slot that
 //      fits them, while the bytes of the admitted slots stay within the HBM
 //      budget (a job that does not fit now does not block later ones);
@@ -147,7 +148,8 @@ struct mdx_server {
     // the pool a job of this kind runs in
     int pool_of(int32_t kind) const {
         if (kind == MDX_FIRE) return kPoolFire;
-        if ((kind == MDX_NVE || kind == MDX_NVT) && !pools[kPoolList].cap.empty()) return kPoolList;
+        if ((kind == MDX_NVE || kind == MDX_NVT || kind == MDX_NPH) && !pools[kPoolList].cap.empty())
+            return kPoolList;
         return kPoolMD;
     }
 
@@ -282,11 +284,11 @@ struct mdx_server {
             }
         if (any) {
             ++ticks;
-            // FIRE pool first, then the neighbour-list pool, then the cell-list
+            // FIRE pool first, then the cell-list pool, then the neighbour-list
             // pool, whose NPH jobs end their step call with a read of the box
             // flags that waits for that pool's work: the pools' launches are
             // all queued before the host waits, so they overlap on the device
-            for (int pi : {kPoolFire, kPoolList, kPoolMD}) {
+            for (int pi : {kPoolFire, kPoolMD, kPoolList}) {
                 Pool &pl = pools[pi];
                 if (std::none_of(pl.job.begin(), pl.job.end(), [](int64_t x) { return x >= 0; })) continue;
                 rc = pl.b.step(static_cast<int>(q));
@@ -375,7 +377,7 @@ int mdx_server_default_params(mdx_server_params *out) {
     out->hbm_budget_bytes = 288e9;
     out->min_class_log2 = 10;
     out->max_class_log2 = 18;
-    out->md_slots_per_class = 6;
+    out->md_slots_per_class = 2;
     out->fire_slots_per_class = 2;
     out->quantum = 100;
     out->record_energies = 1;
@@ -383,7 +385,7 @@ int mdx_server_default_params(mdx_server_params *out) {
     out->min_density = 0.3;
     out->max_box_growth = 1.4;
     out->device = -1;
-    out->list_slots_per_class = 6;
+    out->list_slots_per_class = 12;
     out->list_skin = 0.5;
     out->frame_queue = 8;
     return MDX_OK;

@@ -117,6 +117,9 @@ struct DevScalars {
     int32_t rebuild;        // this step rebuilds the cell list (always outside list mode)
     int32_t list_built;     // the cell list was built from key_list (a list-mode build); nlist and x0 match it
     int32_t list_overflow;  // a particle had more than list_cap partners: list mode off
+    // NPH in list mode: the factor the barostat has scaled positions by since
+    // the last list build (the product of the steps' nph_lin)
+    double lin_cum;
     // by step parity: a lean step with a particle beyond skin / 2 (set by
     // k_lean, cleared by the step before); its pair pass takes the 5x5 cells of
     // the last cell list instead of the list

@@ -232,7 +232,7 @@ def test_served_stream_up_to_262144_equals_alone():
         info = srv.info(ids[i])
         got = srv.result(ids[i])
         s = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), f)
-        if kind in (abi.NVE, abi.NVT):
+        if kind in (abi.NVE, abi.NVT, abi.NPH):
             s.set_tuning(1.0, 1, LIST_SKIN)
         if kind == abi.FIRE:
             res, _, _ = s.fire(FP, 1e-6, nsteps)

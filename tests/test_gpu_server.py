@@ -44,14 +44,14 @@ def make(spec):
     return f, Lx, Ly, kw
 
 
-LIST_SKIN = 0.5   # the server's default for its NVE / NVT slots
+LIST_SKIN = 0.5   # the server's default for its NVE / NVT / NPH slots
 
 
 def alone(spec, f, Lx, Ly, kw):
     kind, n, rho, T, eab, nsteps, pext, seed = spec
     s = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), f)
-    if kind in (abi.NVE, abi.NVT):
-        s.set_tuning(1.0, 1, LIST_SKIN)   # as the server's NVE / NVT slots
+    if kind in (abi.NVE, abi.NVT, abi.NPH):
+        s.set_tuning(1.0, 1, LIST_SKIN)   # as the server's NVE / NVT / NPH slots
     if kind == abi.FIRE:
         res, _, _ = s.fire(FP, 1e-6, nsteps)
         out = s.collect(), res
@@ -187,7 +187,7 @@ def test_streamed_frames_equal_alone_runs_at_their_steps():
             first = min(s for s in steps if s >= m)
             assert first - m < quantum, (i, m, steps)
         s = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), f)
-        if kind in (abi.NVE, abi.NVT):
+        if kind in (abi.NVE, abi.NVT, abi.NPH):
             s.set_tuning(1.0, 1, LIST_SKIN)
         prev = 0
         for step, fr in got[i]:
