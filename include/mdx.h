@@ -275,11 +275,12 @@ int mdx_time_pair_pass(mdx_sim *sim, int32_t reps, double *avg_ms);
  * cell_scale: cell side >= cell_scale * cutoff * sigma_max (>= 1).
  * lanes_per_particle: 1, 2 or 4 lanes share one particle's neighbour cells
  * (64, 32 or 16 particles per wavefront); the FP64 form always uses 1.
- * skin: with skin > 0, NVE, NVT and NPH slots in the FP32 forms keep a list
- * of the partners within cutoff + skin and rebuild it (and the cell list, with
- * cells of at least cutoff + skin) at the step where some particle has moved
- * more than skin / 2 since the last build; the pairs evaluated are exactly
- * those within the cutoff, as without the list.  NPH: the barostat scales
+ * skin: with skin > 0, NVE, NVT and NPH slots and FIRE runs in the FP32 forms
+ * keep a list of the partners within cutoff + skin and rebuild it (and the
+ * cell list, with cells of at least cutoff + skin) at the step (iteration)
+ * where some particle has moved more than skin / 2 since the last build; the
+ * pairs evaluated are exactly those within the cutoff, as without the list.
+ * FIRE leaves list mode where it switches to FP64 pairs.  NPH: the barostat scales
  * every position by the same factor s between builds, so a particle counts as
  * moved by its distance from its scaled build position, against
  * (s (cutoff + skin) - cutoff) / 2, and a step that changes the cell grid's
@@ -303,7 +304,7 @@ int mdx_time_pair_pass(mdx_sim *sim, int32_t reps, double *avg_ms);
  * the same tuning). */
 typedef struct mdx_tuning {
     double cell_scale;
-    double skin;                 /* > 0: neighbour-list mode (FP32 forms, NVE/NVT/NPH slots) */
+    double skin;                 /* > 0: neighbour-list mode (FP32 forms: NVE/NVT/NPH, FIRE) */
     int32_t lanes_per_particle;
     int32_t window;              /* list mode: steps per rebuild window (0: rebuild on demand) */
 } mdx_tuning;

@@ -553,7 +553,8 @@ int Batch::time_pair_pass(int reps, double *avg_ms) {
 // count passes of the step sequence; replayed from a captured hipGraph unless
 // kernel timing is on.
 bool Batch::list_kind(int kind) const {
-    return nlist && skin > 0.0 && prec != MDX_PREC_FP64 && (kind == SK_NVE || kind == SK_NVT || kind == SK_NPH);
+    return nlist && skin > 0.0 && prec != MDX_PREC_FP64 &&
+           (kind == SK_NVE || kind == SK_NVT || kind == SK_NPH || kind == SK_FIRE);
 }
 
 bool Batch::uniform_sigma() const {
@@ -1802,6 +1803,12 @@ int Batch::fire(int r, const mdx_fire_params &fp, double ftol, int max_steps, md
         sp.pmode = PM_FIRE;
         sp.periodic_y = 1;
         sp.wrap_keys = 0;
+        // neighbour lists (FP32 pair forms, skin > 0): rebuilt when an
+        // iteration has moved some particle beyond skin / 2 since the build
+        sp.list_mode = list_kind(SK_FIRE) ? 1 : 0;
+        sp.list_cap = list_cap;
+        sp.skin_half2 = 0.25 * skin * skin;
+        sp.r_list = s.rc_max + skin;
         MDX_TRY(upload_all_sp());
         if (reorder_every > 0 && s.cells_valid && s.since_reorder >= reorder_every) MDX_TRY(reorder({r}));
         MDX_TRY(run_passes(end - j, false));
