@@ -187,16 +187,20 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
         if (n_cap[r] < 0 || ccap[r] < 0) return set_error(MDX_ERR_INVALID, "negative capacity");
         Slot &s = slots[r];
         s.n_cap = n_cap[r];
-        s.ccap = ccap[r];
+        // the slot's cells and its padding cell fill whole scan chunks, so
+        // that the cell-start scan can skip the slots that do not rebuild
+        const int64_t cc = (static_cast<int64_t>(ccap[r]) + kScanChunk) / kScanChunk * kScanChunk - 1;
+        if (cc > INT32_MAX / 2) return set_error(MDX_ERR_CAPACITY, "batch above 2^29 particles or 2^30 cells");
+        s.ccap = static_cast<int32_t>(cc);
         SysParams &sp = h_sp[r];
         sp.off = static_cast<int32_t>(off);
         sp.n_cap = static_cast<int32_t>(n_cap[r]);
         sp.coff = static_cast<int32_t>(coff);
-        sp.ccap = ccap[r];
+        sp.ccap = s.ccap;
         sp.boff = boff;
         sp.nblk = ceil_div(n_cap[r], kBlock);
         sp.cboff = cboff;
-        sp.ncblk = ceil_div(ccap[r], kBlock);
+        sp.ncblk = ceil_div(s.ccap, kBlock);   // (s.ccap + 1) / kBlock: cell block g holds cells kBlock g on
         sp.lboff = lboff;
         sp.nlblk = ceil_div(n_cap[r], kLeanSpan);
         sp.periodic_y = 1;
@@ -205,7 +209,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
         for (int b = 0; b < sp.ncblk; ++b) cb2s.push_back(r);
         for (int b = 0; b < sp.nlblk; ++b) lb2s.push_back(r);
         off += n_cap[r];
-        coff += ccap[r] + 1;   // + the slot's padding cell (mdx_sim.hpp)
+        coff += s.ccap + 1;   // + the slot's padding cell (mdx_sim.hpp)
         boff += sp.nblk;
         cboff += sp.ncblk;
         lboff += sp.nlblk;
@@ -368,6 +372,16 @@ int Batch::sync() {
 }
 
 namespace {
+// The neighbour-list parameters of a slot under the batch's tuning, set
+// wherever a pass may run the slot in list mode (steps, FIRE, energy passes):
+// a list built with unset ones (r_list 0) takes every candidate, overflows and
+// turns list mode off for the slot.
+void list_fields(SysParams &sp, const Slot &s, double skin, int list_cap) {
+    sp.list_cap = list_cap;
+    sp.skin_half2 = 0.25 * skin * skin;
+    sp.r_list = s.rc_max + skin;
+}
+
 void set_idle(SysParams &sp, const Slot &s) {
     sp.kind = SK_NONE;
     sp.pmode = PM_NONE;
@@ -391,10 +405,10 @@ static void launch_k(K kern, int grid, int block, hipStream_t st, hipEvent_t e0,
 }
 
 // FP32 pair kernel with lpp lanes per particle (1, 2 or 4).
-template <int ACC, bool OBS, bool UNI>
+template <int ACC, int OBS, bool UNI>
 static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &p, int parity, int lean,
                             hipEvent_t e0, hipEvent_t e1) {
-    if constexpr (!OBS) {   // lean steps take no energy sums
+    if constexpr (OBS == SUMS_NONE) {   // lean steps take no sums
         if (lean) {
             switch (lpp) {
             case 2: launch_k(k_pair<int2, ACC, OBS, UNI, 2, true>, grid, kBlock * 2, st, e0, e1, p, parity); return;
@@ -413,7 +427,7 @@ static void launch_pair_f32(int lpp, int grid, hipStream_t st, const BatchPtrs &
 // Launch one step of every slot (their kind/pmode in d_sp).  A lean step
 // (STEP_LEAN, step parity `parity`) is k_lean and the pair pass; every other
 // step starts with k_box and runs the cell-list kernels.
-int Batch::launch_step_kernels(bool obs, int mode, int parity) {
+int Batch::launch_step_kernels(int obs, int mode, int parity) {
     const int S = static_cast<int>(slots.size());
     const bool lean = mode == STEP_LEAN;
     if (lean && obs) return set_error(MDX_ERR_STATE, "a lean step takes no energy sums");
@@ -434,8 +448,8 @@ int Batch::launch_step_kernels(bool obs, int mode, int parity) {
         if (TCB + TB > 0) k_stay_arrive<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
         const int nchunk = ceil_div(C, kScanChunk);
         if (nchunk > 0) {
-            k_scan_a<<<nchunk, kBlock, 0, stream>>>(count, arrivals, C, bsum);
-            k_scan_b<<<nchunk, kBlock, 0, stream>>>(count, arrivals, C, bsum, cs);
+            k_scan_a<<<nchunk, kBlock, 0, stream>>>(ptrs, C, bsum);
+            k_scan_b<<<nchunk, kBlock, 0, stream>>>(ptrs, C, bsum);
         }
         if (TCB + TB > 0) k_fill<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
         if (TCB > 0) k_cell_sort<<<TCB, kBlock, 0, stream>>>(ptrs);
@@ -460,34 +474,43 @@ int Batch::launch_step_kernels(bool obs, int mode, int parity) {
     return MDX_OK;
 }
 
-// The pair kernel of the batch's precision, lanes and sigma form; lean: a
-// rebuild window's lean-step variant with step parity `parity`; e0/e1: events
-// for the kernel's start and end (launch_k), or null.
-void Batch::launch_pair(bool obs, int parity, bool lean, hipEvent_t e0, hipEvent_t e1) {
+// The pair kernel of the batch's precision, lanes and sigma form, with the
+// sums obs (PairSums; the FP64 form takes its energy sums for the virial);
+// lean: a rebuild window's lean-step variant with step parity `parity`;
+// e0/e1: events for the kernel's start and end (launch_k), or null.
+void Batch::launch_pair(int obs, int parity, bool lean, hipEvent_t e0, hipEvent_t e1) {
     const bool uni = uniform_sigma();
     const int ln = lean ? 1 : 0;
     switch (prec) {
     case MDX_PREC_COMP32:
-        if (obs) {
-            if (uni) launch_pair_f32<ACC_COMP32, true, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
-            else launch_pair_f32<ACC_COMP32, true, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+        if (obs == SUMS_ENERGY) {
+            if (uni) launch_pair_f32<ACC_COMP32, SUMS_ENERGY, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_COMP32, SUMS_ENERGY, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+        } else if (obs == SUMS_VIRIAL) {
+            if (uni) launch_pair_f32<ACC_COMP32, SUMS_VIRIAL, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_COMP32, SUMS_VIRIAL, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
         } else {
-            if (uni) launch_pair_f32<ACC_COMP32, false, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
-            else launch_pair_f32<ACC_COMP32, false, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            if (uni) launch_pair_f32<ACC_COMP32, SUMS_NONE, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_COMP32, SUMS_NONE, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
         }
         break;
     case MDX_PREC_ACC64:
-        if (obs) {
-            if (uni) launch_pair_f32<ACC_FP64, true, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
-            else launch_pair_f32<ACC_FP64, true, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+        if (obs == SUMS_ENERGY) {
+            if (uni) launch_pair_f32<ACC_FP64, SUMS_ENERGY, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_FP64, SUMS_ENERGY, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+        } else if (obs == SUMS_VIRIAL) {
+            if (uni) launch_pair_f32<ACC_FP64, SUMS_VIRIAL, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_FP64, SUMS_VIRIAL, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
         } else {
-            if (uni) launch_pair_f32<ACC_FP64, false, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
-            else launch_pair_f32<ACC_FP64, false, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            if (uni) launch_pair_f32<ACC_FP64, SUMS_NONE, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_FP64, SUMS_NONE, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
         }
         break;
-    default:   // the FP64 form keeps one lane per particle (the oracle's order)
-        if (obs) launch_k(k_pair<double2, ACC_FP64, true, false, 1, false>, TB, kBlock, stream, e0, e1, ptrs, 0);
-        else launch_k(k_pair<double2, ACC_FP64, false, false, 1, false>, TB, kBlock, stream, e0, e1, ptrs, 0);
+    default:   // the FP64 form keeps one lane per particle (the oracle's order) and its FP64 energy sums
+        if (obs != SUMS_NONE)
+            launch_k(k_pair<double2, ACC_FP64, SUMS_ENERGY, false, 1, false>, TB, kBlock, stream, e0, e1, ptrs, 0);
+        else
+            launch_k(k_pair<double2, ACC_FP64, SUMS_NONE, false, 1, false>, TB, kBlock, stream, e0, e1, ptrs, 0);
         break;
     }
 }
@@ -527,7 +550,7 @@ int Batch::time_pair_pass(int reps, double *avg_ms) {
         hip(hipMemcpyAsync(sv + nv + na, partials, np, hipMemcpyDeviceToDevice, stream)) &&
         hip(hipMemcpyAsync(sv + nv + na + np, d_sc, ns, hipMemcpyDeviceToDevice, stream)) &&
         hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal))) {
-        for (int i = 0; i < reps; ++i) launch_pair(false, 0, lean, nullptr, nullptr);
+        for (int i = 0; i < reps; ++i) launch_pair(SUMS_NONE, 0, lean, nullptr, nullptr);
         const bool ended = hip(hipStreamEndCapture(stream, &g));
         if (ended && hip(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0)) && hip(hipEventCreate(&e0)) &&
             hip(hipEventCreate(&e1)) && hip(hipEventRecord(e0, stream)) && hip(hipGraphLaunch(ge, stream)) &&
@@ -566,7 +589,7 @@ bool Batch::uniform_sigma() const {
 // count steps of one mode, replayed from a captured hipGraph unless kernel
 // timing is on.
 // Captures `steps` steps of one mode and parity into ge (when it is empty).
-int Batch::capture_steps(hipGraphExec_t &ge, bool obs, int mode, int parity, int steps) {
+int Batch::capture_steps(hipGraphExec_t &ge, int obs, int mode, int parity, int steps) {
     if (ge) return MDX_OK;
     hipGraph_t g = nullptr;
     MDX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -584,18 +607,19 @@ int Batch::capture_steps(hipGraphExec_t &ge, bool obs, int mode, int parity, int
     return MDX_OK;
 }
 
-int Batch::replay(bool obs, int mode, int count_, int parity) {
+int Batch::replay(int obs, int mode, int count_, int parity) {
     if (count_ <= 0) return MDX_OK;
     if (mode != STEP_LEAN) parity = 0;
     if (use_graph && !timing) {
+        const int oi = obs;
         int k = 0;
         if (count_ >= kGraphSteps && mode != STEP_LEAN) {   // fewer launches: kGraphSteps steps per graph
-            hipGraphExec_t &gm = graph_multi[obs ? 1 : 0][mode];
+            hipGraphExec_t &gm = graph_multi[oi][mode];
             MDX_TRY(capture_steps(gm, obs, mode, 0, kGraphSteps));
             for (; k + kGraphSteps <= count_; k += kGraphSteps) MDX_HIP(hipGraphLaunch(gm, stream));
         }
         if (k < count_) {
-            hipGraphExec_t &ge = graph[obs ? 1 : 0][mode][parity];
+            hipGraphExec_t &ge = graph[oi][mode][parity];
             MDX_TRY(capture_steps(ge, obs, mode, parity, 1));
             for (; k < count_; ++k) MDX_HIP(hipGraphLaunch(ge, stream));
         }
@@ -634,7 +658,7 @@ bool Batch::windows_apply() {
 
 // count passes of the step sequence; a run ends with the combination of the
 // last pair pass's reduction (earlier ones are combined by the next k_box).
-int Batch::run_passes(int count_, bool obs) {
+int Batch::run_passes(int count_, int obs) {
     if (count_ <= 0) return MDX_OK;
     if (graph_uni != uniform_sigma()) {   // captured graphs hold the other pair-kernel variant
         invalidate_graphs();
@@ -674,7 +698,7 @@ int Batch::run_passes(int count_, bool obs) {
 int Batch::run_windows(int count_) {
     int left = count_;
     while (win_pos > 0 && left > 0) {   // the open window's remaining lean steps
-        MDX_TRY(replay(false, STEP_LEAN, 1, win_pos & 1));
+        MDX_TRY(replay(SUMS_NONE, STEP_LEAN, 1, win_pos & 1));
         win_pos = (win_pos + 1) % window;
         --left;
     }
@@ -686,8 +710,8 @@ int Batch::run_windows(int count_) {
             if (!graph_window) {
                 hipGraph_t g = nullptr;
                 MDX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                int rc = launch_step_kernels(false, STEP_HEAD, 0);
-                for (int j = 1; j < w && rc == MDX_OK; ++j) rc = launch_step_kernels(false, STEP_LEAN, j & 1);
+                int rc = launch_step_kernels(SUMS_NONE, STEP_HEAD, 0);
+                for (int j = 1; j < w && rc == MDX_OK; ++j) rc = launch_step_kernels(SUMS_NONE, STEP_LEAN, j & 1);
                 hipError_t e = hipStreamEndCapture(stream, &g);
                 if (rc != MDX_OK) {
                     if (g) (void)hipGraphDestroy(g);
@@ -700,8 +724,8 @@ int Batch::run_windows(int count_) {
             }
             MDX_HIP(hipGraphLaunch(graph_window, stream));
         } else {
-            MDX_TRY(replay(false, STEP_HEAD, 1));   // step parity 0
-            for (int j = 1; j < w; ++j) MDX_TRY(replay(false, STEP_LEAN, 1, j & 1));
+            MDX_TRY(replay(SUMS_NONE, STEP_HEAD, 1));   // step parity 0
+            for (int j = 1; j < w; ++j) MDX_TRY(replay(SUMS_NONE, STEP_LEAN, 1, j & 1));
         }
         left -= w;
         win_pos = w % window;
@@ -893,10 +917,10 @@ static int one_pass(Batch &b, const std::vector<int> &which, int periodic_y, int
         sp.pmode = pmode;
         sp.wrap_keys = 1;
         sp.list_mode = (pmode == PM_OBS && b.list_kind(sl.kind)) ? 1 : 0;
+        list_fields(sp, sl, b.skin, b.list_cap);
     }
     MDX_TRY(b.upload_all_sp());
-    const bool obs = pmode == PM_OBS;
-    MDX_TRY(b.run_passes(1, obs));
+    MDX_TRY(b.run_passes(1, pmode == PM_OBS ? SUMS_ENERGY : SUMS_NONE));
     b.win_pos = 0;   // the next step starts a window (a head step) from this pass's cell list
     for (int r : which) {
         b.slots[r].cells_valid = true;
@@ -976,7 +1000,8 @@ int Batch::step(int nsteps) {
     if (nsteps == 0) return MDX_OK;
     MDX_HIP(hipSetDevice(device));
     MDX_TRY(pre_step());
-    bool obs = false, any = false, nph = false;
+    int obs = SUMS_NONE;   // NPH slots take the pair pass's virial (the barostat's pressure)
+    bool any = false, nph = false;
     for (size_t r = 0; r < slots.size(); ++r) {
         Slot &s = slots[r];
         SysParams &sp = h_sp[r];
@@ -993,15 +1018,13 @@ int Batch::step(int nsteps) {
         switch (s.kind) {
         case SK_NVE: sp.pmode = PM_NVE; break;
         case SK_NVT: sp.pmode = PM_NVT; break;
-        case SK_NPH: sp.pmode = PM_NPH; obs = true; nph = true; break;
+        case SK_NPH: sp.pmode = PM_NPH; obs = SUMS_VIRIAL; nph = true; break;
         case SK_PISTON: sp.pmode = PM_PISTON; break;
         case SK_FIRE: sp.pmode = PM_FIRE; break;
         default: return set_error(MDX_ERR_INVALID, "unknown ensemble");
         }
         sp.list_mode = list_kind(s.kind) ? 1 : 0;
-        sp.list_cap = list_cap;
-        sp.skin_half2 = 0.25 * skin * skin;
-        sp.r_list = s.rc_max + skin;
+        list_fields(sp, s, skin, list_cap);
         if (s.kind == SK_NVT && (s.flags & MDX_STEP_MIDDLE_WRAP)) {
             DevScalars h;
             MDX_TRY(read_scalars(static_cast<int>(r), h));
@@ -1806,12 +1829,10 @@ int Batch::fire(int r, const mdx_fire_params &fp, double ftol, int max_steps, md
         // neighbour lists (FP32 pair forms, skin > 0): rebuilt when an
         // iteration has moved some particle beyond skin / 2 since the build
         sp.list_mode = list_kind(SK_FIRE) ? 1 : 0;
-        sp.list_cap = list_cap;
-        sp.skin_half2 = 0.25 * skin * skin;
-        sp.r_list = s.rc_max + skin;
+        list_fields(sp, s, skin, list_cap);
         MDX_TRY(upload_all_sp());
         if (reorder_every > 0 && s.cells_valid && s.since_reorder >= reorder_every) MDX_TRY(reorder({r}));
-        MDX_TRY(run_passes(end - j, false));
+        MDX_TRY(run_passes(end - j, SUMS_NONE));
         s.since_reorder += end - j;
         j = end;
         MDX_TRY(read_scalars(r, h));

@@ -85,10 +85,11 @@ struct Batch {
     DevScalars *d_sc = nullptr;
     int32_t *d_blk2sys = nullptr, *d_cblk2sys = nullptr, *d_lb2sys = nullptr, *d_want = nullptr;
     BatchPtrs ptrs{};
-    // captured step graphs, by [obs]
-    hipGraphExec_t graph[2][4][2] = {};   // [obs][StepMode][step parity (lean steps)]
+    // captured step graphs, by the pair pass's sums (PairSums: none, energy
+    // sums, virial)
+    hipGraphExec_t graph[3][4][2] = {};   // [PairSums][StepMode][step parity (lean steps)]
     hipGraphExec_t graph_window = nullptr;  // one whole rebuild window: the head step and window - 1 lean steps
-    hipGraphExec_t graph_multi[2][4] = {};  // [obs][StepMode]: kGraphSteps steps of one mode (not lean)
+    hipGraphExec_t graph_multi[3][4] = {};  // [PairSums][StepMode]: kGraphSteps steps of one mode (not lean)
     bool use_graph = true;
     bool graph_uni = true;   // pair-kernel variant the captured graphs hold
     int lpp = 1;             // lanes per particle of the FP32 pair kernel (1, 2, 4)
@@ -160,13 +161,14 @@ struct Batch {
     // internals
     int upload_sp(int r);
     int upload_all_sp();
-    int launch_step_kernels(bool obs, int mode, int parity);
-    void launch_pair(bool obs, int parity, bool lean, hipEvent_t e0, hipEvent_t e1);
+    // obs: the pair pass's sums (PairSums)
+    int launch_step_kernels(int obs, int mode, int parity);
+    void launch_pair(int obs, int parity, bool lean, hipEvent_t e0, hipEvent_t e1);
     int time_pair_pass(int reps, double *avg_ms);   // pair kernel duration, launches back to back
     int set_stream(hipStream_t s);                  // issue on s (null: the batch's own stream)
-    int run_passes(int count, bool obs);
-    int replay(bool obs, int mode, int count, int parity = 0);
-    int capture_steps(hipGraphExec_t &ge, bool obs, int mode, int parity, int steps);
+    int run_passes(int count, int obs);
+    int replay(int obs, int mode, int count, int parity = 0);
+    int capture_steps(hipGraphExec_t &ge, int obs, int mode, int parity, int steps);
     int n_profile(int r, int nb, int32_t *out);
     int pressure_profile(int r, int nb, double *out);   // [Pxx | Pyy | Pxy] per slab along y
     int n_profile_noalloc(int r, int nb, int32_t *out);   // counters in the device's fixed scratch array

@@ -89,9 +89,10 @@ __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
     DevScalars &sc = P.sc[r];
     // the padding cell anchors the slot's cell starts at its first particle
     if (threadIdx.x == 0) P.count[sp.coff + sp.ccap] = static_cast<uint32_t>(sp.n > 0 ? sp.n_cap - sp.n : sp.n_cap);
-    if (sp.n <= 0) {   // unloaded: the cell-list kernels clear every cell of the slot
+    if (sp.n <= 0) {   // unloaded: the cell-list kernels of one step clear every cell of the slot
         if (threadIdx.x == 0) {
-            sc.rebuild = 1;
+            sc.rebuild = sc.cleared ? 0 : 1;   // after that the slot takes no part in them
+            sc.cleared = 1;
             sc.list_valid = 0;
             sc.list_built = 0;
             sc.list_on = 0;
@@ -396,15 +397,17 @@ __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t ahead) {
 // cell list: scan, scatter, per-cell sort + image
 // --------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(kBlock) k_scan_a(const uint32_t *__restrict__ count,
-                                                   const uint32_t *__restrict__ arrivals, int ncell,
-                                                   uint32_t *__restrict__ bsum) {
+// Per-chunk sums of count + arrivals.  A chunk lies in one slot (each slot's
+// cells, its padding cell included, are a whole number of chunks); the chunks
+// of a slot that does not rebuild are skipped, its cell starts stay as they are.
+__global__ void __launch_bounds__(kBlock) k_scan_a(BatchPtrs P, int ncell, uint32_t *__restrict__ bsum) {
+    if (!P.sc[P.cblk2sys[blockIdx.x * (kScanChunk / kBlock)]].rebuild) return;
     __shared__ uint32_t sh[kWaves];
     const int base = blockIdx.x * kScanChunk + threadIdx.x * 4;
     uint32_t s = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-        if (base + j < ncell) s += count[base + j] + arrivals[base + j];
+        if (base + j < ncell) s += P.count[base + j] + P.arrivals[base + j];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
@@ -415,14 +418,21 @@ __global__ void __launch_bounds__(kBlock) k_scan_a(const uint32_t *__restrict__ 
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_b(const uint32_t *__restrict__ count,
-                                                   const uint32_t *__restrict__ arrivals, int ncell,
-                                                   const uint32_t *__restrict__ bsum, uint32_t *__restrict__ cs) {
+// Cell starts of the slots that rebuild: an exclusive scan of count +
+// arrivals within the slot, from the slot's first particle.  Each slot's cells
+// (its padding cell last, whose count is n_cap - n) sum to n_cap, so this
+// equals the scan over the whole batch.
+__global__ void __launch_bounds__(kBlock) k_scan_b(BatchPtrs P, int ncell, const uint32_t *__restrict__ bsum) {
+    const int r = P.cblk2sys[blockIdx.x * (kScanChunk / kBlock)];
+    const SysParams &sp = P.sp[r];
+    if (!P.sc[r].rebuild) return;
     __shared__ uint32_t shw[kWaves];
     __shared__ uint32_t shp[kWaves];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // the slot's chunks before this one
+    const int b0 = sp.coff / kScanChunk;
     uint32_t pre = 0;
-    for (int b = threadIdx.x; b < static_cast<int>(blockIdx.x); b += kBlock) pre += bsum[b];
+    for (int b = b0 + static_cast<int>(threadIdx.x); b < static_cast<int>(blockIdx.x); b += kBlock) pre += bsum[b];
     for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o, 64);
     if (lane == 0) shp[wid] = pre;
     const int base = blockIdx.x * kScanChunk + threadIdx.x * 4;
@@ -430,7 +440,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_b(const uint32_t *__restrict__ 
     uint32_t tsum = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        c[j] = (base + j < ncell) ? count[base + j] + arrivals[base + j] : 0u;
+        c[j] = (base + j < ncell) ? P.count[base + j] + P.arrivals[base + j] : 0u;
         tsum += c[j];
     }
     uint32_t inc = tsum;
@@ -440,17 +450,17 @@ __global__ void __launch_bounds__(kBlock) k_scan_b(const uint32_t *__restrict__ 
     }
     if (lane == 63) shw[wid] = inc;
     __syncthreads();
-    uint32_t chunk_pre = 0;
+    uint32_t chunk_pre = static_cast<uint32_t>(sp.off);
     for (int w = 0; w < kWaves; ++w) chunk_pre += shp[w];
     uint32_t wave_off = 0;
     for (int w = 0; w < wid; ++w) wave_off += shw[w];
     uint32_t run = chunk_pre + wave_off + inc - tsum;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        if (base + j < ncell) cs[base + j] = run;
+        if (base + j < ncell) P.cs[base + j] = run;
         run += c[j];
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) cs[ncell] = run;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) P.cs[ncell] = run;
 }
 
 // Drift reduction of a slot (Nose-Hoover / NPH kinetic energy after the
@@ -1088,6 +1098,17 @@ __global__ void __launch_bounds__(kBlock) k_n_profile(BatchPtrs P, int32_t r, in
 
 enum Acc { ACC_COMP32 = 0, ACC_FP64 = 1 };
 
+// The sums a pair pass forms besides the forces (its OBS template argument),
+// for the slots that take them (energy passes, NPH steps):
+//   SUMS_ENERGY  U, the virial W and dU/dlambda, each pair added in FP64 (energy
+//                passes, and the FP64 form's NPH steps);
+//   SUMS_VIRIAL  W alone, for the NPH steps of the FP32 forms: the pair's r.F
+//                is its force factor times r^2, summed like its force (FP32
+//                runs entering an FP64 sum in the compensated form, FP64 per
+//                pair in the FP64-accumulating form); FP64-evaluated pairs add
+//                theirs in FP64.
+enum PairSums { SUMS_NONE = 0, SUMS_ENERGY = 1, SUMS_VIRIAL = 2 };
+
 template <int ACC>
 struct ForceSum;
 template <>
@@ -1115,7 +1136,7 @@ struct ObsSum {
 // FP64 evaluation of one pair from its displacement; the operation order is
 // the oracle's (and the reference's, src/md_cuda_common.cu:237-254) so the
 // FP64 form reproduces the oracle bit for bit.
-template <bool OBS>
+template <int OBS>
 __device__ inline bool pair_fp64(double dx, double dy, int pidx, const PairTable &pt, double &fx, double &fy,
                                  ObsSum &ob, bool want_obs) {
     const double r2 = dx * dx + dy * dy;
@@ -1127,12 +1148,14 @@ __device__ inline bool pair_fp64(double dx, double dy, int pidx, const PairTable
     const double tmp = pt.eps24[pidx] * (2.0 * sr12 - sr6) * r2i;
     fx = tmp * dx;
     fy = tmp * dy;
-    if constexpr (OBS) {
+    if constexpr (OBS == SUMS_ENERGY) {
         if (want_obs) {
             ob.u += 0.5 * (pt.eps4[pidx] * (sr12 - sr6) - pt.vc[pidx]);
             ob.w += 0.5 * (dx * fx + dy * fy);
             ob.lam += 0.5 * (fx * dx - fy * dy);
         }
+    } else if constexpr (OBS == SUMS_VIRIAL) {
+        if (want_obs) ob.w += 0.5 * (dx * fx + dy * fy);
     }
     return true;
 }
@@ -1289,7 +1312,7 @@ __device__ inline CoreConst core_select(bool same, const CoreConst &a, const Cor
 // the hardware FP64 reciprocal refined by two Newton steps (relative error at
 // the FP64 rounding level).  Only core pairs come here, well inside the
 // cutoff.
-template <bool OBS>
+template <int OBS>
 __device__ inline void pair_core64(int dxi, int dyi, double qinv, const CoreConst &k, double &gx, double &gy,
                                    ObsSum &ob, bool want_obs) {
     const double dx = static_cast<double>(dxi) * qinv, dy = static_cast<double>(dyi) * qinv;
@@ -1305,12 +1328,14 @@ __device__ inline void pair_core64(int dxi, int dyi, double qinv, const CoreCons
     const double fx = tmp * dx, fy = tmp * dy;
     gx += fx;
     gy += fy;
-    if constexpr (OBS) {
+    if constexpr (OBS == SUMS_ENERGY) {
         if (want_obs) {
             ob.u += 0.5 * (k.eps4 * (sr12 - sr6) - k.vc);
             ob.w += 0.5 * (dx * fx + dy * fy);
             ob.lam += 0.5 * (fx * dx - fy * dy);
         }
+    } else if constexpr (OBS == SUMS_VIRIAL) {
+        if (want_obs) ob.w += 0.5 * (dx * fx + dy * fy);
     }
 }
 
@@ -1338,7 +1363,7 @@ __device__ inline void disp_fixed(int pix, int piy, int2 pj, const Grid &g, int 
 // slot k and the offset code (ox + 2) + 5 (oy + 2)) while it has room, else
 // (and with LPP > 1) they are evaluated at once.  ACC64: pairs closer than
 // r2_guard, whose FP32 (sigma/r)^12 could overflow, are evaluated in FP64.
-template <int ACC, bool OBS, bool UNI, int LPP>
+template <int ACC, int OBS, bool UNI, int LPP>
 __device__ __forceinline__ void pair_cell_f32(const BatchPtrs &P, uint32_t k0, uint32_t k1, int pix, int piy,
                                               int ci, int ox, int oy, const Grid &g, const PairTable &pt,
                                               const PairF32 &pc, ForceSum<ACC> &F, ObsSum &ob, bool want_obs,
@@ -1346,6 +1371,7 @@ __device__ __forceinline__ void pair_cell_f32(const BatchPtrs &P, uint32_t k0, u
     const int2 *__restrict__ img = static_cast<const int2 *>(P.img);
     const uint8_t *__restrict__ clsb = P.icls;
     float fxc = 0.0f, fyc = 0.0f;
+    float wc = 0.0f;   // SUMS_VIRIAL, compensated form: the cell's sum of tmp r^2
     // with LPP lanes per particle, lane q of the group takes candidates
     // k0 + q, k0 + q + LPP, ...
     for (uint32_t k = k0 + (LPP > 1 ? threadIdx.x % LPP : 0u); k < k1; k += LPP) {
@@ -1392,10 +1418,14 @@ __device__ __forceinline__ void pair_cell_f32(const BatchPtrs &P, uint32_t k0, u
                 if constexpr (ACC == ACC_COMP32) {
                     fxc += fx;
                     fyc += fy;
+                    if constexpr (OBS == SUMS_VIRIAL) wc = __builtin_fmaf(tmp, rr, wc);
                 } else {
                     F.add(fx, fy);
+                    if constexpr (OBS == SUMS_VIRIAL) {
+                        if (want_obs) ob.w += 0.5 * g.qinv * static_cast<double>(tmp * rr);
+                    }
                 }
-                if constexpr (OBS) {
+                if constexpr (OBS == SUMS_ENERGY) {
                     if (want_obs) {
                         const float e4 = same ? pc.e4s : pc.e4o;
                         const float u = e4 * (sr12 - sr6);
@@ -1409,12 +1439,17 @@ __device__ __forceinline__ void pair_cell_f32(const BatchPtrs &P, uint32_t k0, u
             }
         }
     }
-    if constexpr (ACC == ACC_COMP32) F.add(fxc, fyc);
+    if constexpr (ACC == ACC_COMP32) {
+        F.add(fxc, fyc);
+        if constexpr (OBS == SUMS_VIRIAL) {
+            if (want_obs) ob.w += 0.5 * g.qinv * static_cast<double>(wc);
+        }
+    }
 }
 
 // The queued core pairs of a cell pass (pair_cell_f32), in queue order; pi is
 // the particle's own fixed-point image.
-template <bool OBS>
+template <int OBS>
 __device__ inline void cell_core_queue(const BatchPtrs &P, int2 pi, int ci, const Grid &g, const PairTable &pt,
                                        const uint2 *cq, int qn, double &gx, double &gy, ObsSum &ob, bool want_obs) {
     const int2 *img = static_cast<const int2 *>(P.img);
@@ -1442,10 +1477,10 @@ __device__ inline void cell_core_queue(const BatchPtrs &P, int2 pi, int ci, cons
 //      to whether this entry is one.
 // PURE: the wave's lists pair no two type classes (kListMixed clear), so
 // every entry takes the lane's own-class constants.
-template <int ACC, bool OBS, bool UNI, int CHK, bool PURE = false>
+template <int ACC, int OBS, bool UNI, int CHK, bool PURE = false>
 __device__ __forceinline__ void list_pair_f32(uint32_t v, int2 q, int psx, int psy, int ci, const PairTable &pt,
-                                              const PairF32 &pc, float &fxc, float &fyc, ForceSum<ACC> &F,
-                                              ObsSum &ob, bool want_obs, bool &flag) {
+                                              const PairF32 &pc, float &fxc, float &fyc, float &wc,
+                                              ForceSum<ACC> &F, ObsSum &ob, bool want_obs, bool &flag) {
     const float dx = static_cast<float>(psx - q.x);
     const float dy = static_cast<float>(psy - q.y);
     const float rr = __builtin_fmaf(dx, dx, dy * dy);
@@ -1471,10 +1506,14 @@ __device__ __forceinline__ void list_pair_f32(uint32_t v, int2 q, int psx, int p
     if constexpr (ACC == ACC_COMP32) {   // the run's FP32 sum, one rounding per term
         fxc = __builtin_fmaf(tmp, dx, fxc);
         fyc = __builtin_fmaf(tmp, dy, fyc);
+        if constexpr (OBS == SUMS_VIRIAL) wc = __builtin_fmaf(tmp, rr, wc);
     } else {
         F.add(tmp * dx, tmp * dy);
+        if constexpr (OBS == SUMS_VIRIAL) {
+            if (want_obs) ob.w += 0.5 * static_cast<double>(pc.qinv) * static_cast<double>(tmp * rr);
+        }
     }
-    if constexpr (OBS) {
+    if constexpr (OBS == SUMS_ENERGY) {
         if (want_obs && in) {
             const float fx = tmp * dx, fy = tmp * dy;
             const float uu = __builtin_fmaf(same ? pc.uas : pc.uao, r6i, -(same ? pc.ubs : pc.ubo)) * r6i;
@@ -1491,7 +1530,7 @@ __device__ __forceinline__ void list_pair_f32(uint32_t v, int2 q, int psx, int p
 // The FP64 evaluation of the guarded pairs among U entries (0 < r^2 <
 // r2_guard), in entry order; ps[u] is the particle's image moved by minus
 // entry u's cell offset.
-template <int U, bool OBS>
+template <int U, int OBS>
 __device__ __forceinline__ void list_pairs_guarded(const uint32_t (&v)[U], const int2 (&q)[U], const int2 (&ps)[U],
                                                    int ci, const PairTable &pt, const PairF32 &pc, double &gx,
                                                    double &gy, ObsSum &ob, bool want_obs) {
@@ -1521,11 +1560,11 @@ __device__ __forceinline__ void list_pairs_guarded(const uint32_t (&v)[U], const
 // list_pair_f32; with CHK == 2 (at most 32 rows) each row shifts its core flag
 // into m from the right (m = 2 m + flag), so row e0 + j ends at bit
 // e1 - e0 - 1 - j.  PURE as list_pair_f32.
-template <int ACC, bool OBS, bool UNI, int CHK, bool PURE>
+template <int ACC, int OBS, bool UNI, int CHK, bool PURE>
 __device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, uint32_t e0, uint32_t e1,
                                               const char *shb, int ci, const PairTable &pt, const PairF32 &pc,
                                               ForceSum<ACC> &F, ObsSum &ob, bool want_obs, float &fxc, float &fyc,
-                                              double &gx, double &gy, uint32_t &m) {
+                                              float &wc, double &gx, double &gy, uint32_t &m) {
     constexpr int U = 4;
     const char *imgb = static_cast<const char *>(P.img);
     const char *lb = reinterpret_cast<const char *>(P.nlist);
@@ -1566,12 +1605,12 @@ __device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, ui
         for (int u = 0; u < U; ++u) {
             if constexpr (CHK == 2) {
                 bool cf = false;
-                list_pair_f32<ACC, OBS, UNI, CHK, PURE>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
-                                                        want_obs, cf);
+                list_pair_f32<ACC, OBS, UNI, CHK, PURE>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, wc, F,
+                                                        ob, want_obs, cf);
                 m = m + m + (cf ? 1u : 0u);   // one add with carry-in
             } else {
-                list_pair_f32<ACC, OBS, UNI, CHK, PURE>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
-                                                        want_obs, slow);
+                list_pair_f32<ACC, OBS, UNI, CHK, PURE>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, wc, F,
+                                                        ob, want_obs, slow);
             }
         }
         if constexpr (CHK == 1) {
@@ -1581,6 +1620,10 @@ __device__ __forceinline__ void list_rows_f32(const BatchPtrs &P, uint32_t s, ui
             if (e & U) {   // every eight rows
                 F.add(fxc, fyc);
                 fxc = fyc = 0.0f;
+                if constexpr (OBS == SUMS_VIRIAL) {
+                    ob.w += 0.5 * static_cast<double>(pc.qinv) * static_cast<double>(wc);
+                    wc = 0.0f;
+                }
             }
         }
 #pragma unroll
@@ -1603,7 +1646,7 @@ __device__ inline bool core_take(uint32_t &m, uint32_t base, uint32_t cnt, uint3
 // the exact fixed-point displacement (the shifted own image from the table
 // shb, the partner's image reloaded).  A wave runs as many passes as its lane
 // with the most core rows needs (usually none or one).
-template <bool OBS>
+template <int OBS>
 __device__ inline void list_core_rows(const BatchPtrs &P, uint32_t s, int ci, const Grid &g, const PairTable &pt,
                                       const char *shb, uint32_t m1, uint32_t n1, uint32_t m2, uint32_t n2,
                                       double &gx, double &gy, ObsSum &ob, bool want_obs) {
@@ -1633,12 +1676,13 @@ __device__ inline void list_core_rows(const BatchPtrs &P, uint32_t s, int ci, co
 // list stand for list_pad.  COMP32: core pairs are evaluated in FP64 (after
 // the rows with one lane per particle, at once with several).  shift: the
 // workgroup's [9][kBlock] LDS table (LPP == 1).
-template <int ACC, bool OBS, bool UNI, int LPP>
+template <int ACC, int OBS, bool UNI, int LPP>
 __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, int2 pi, int ci, const Grid &g,
                                               const PairTable &pt, const PairF32 &pc, ForceSum<ACC> &F,
                                               ObsSum &ob, bool want_obs, int2 (*shift)[kBlock], uint32_t *stash) {
     constexpr int U = 4;
     float fxc = 0.0f, fyc = 0.0f;
+    float wc = 0.0f;   // SUMS_VIRIAL, compensated form: the run's sum of tmp r^2
     double gx = 0.0, gy = 0.0;
     if constexpr (LPP == 1) {
         // the nine shifted images of each particle, one table row per cell
@@ -1658,15 +1702,15 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, in
             // in LDS while the rows past 32 (long lists only) run
             const uint32_t n1 = n < 32u ? n : 32u;
             if (pure)
-                list_rows_f32<ACC, OBS, UNI, 2, true>(P, s, 0u, n1, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy,
-                                                      m1);
+                list_rows_f32<ACC, OBS, UNI, 2, true>(P, s, 0u, n1, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, wc,
+                                                      gx, gy, m1);
             else
-                list_rows_f32<ACC, OBS, UNI, 2, false>(P, s, 0u, n1, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx,
-                                                       gy, m1);
+                list_rows_f32<ACC, OBS, UNI, 2, false>(P, s, 0u, n1, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, wc,
+                                                       gx, gy, m1);
             if (n > 32u) {
                 stash[threadIdx.x] = m1;
-                list_rows_f32<ACC, OBS, UNI, 2, false>(P, s, 32u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx,
-                                                       gy, m2);
+                list_rows_f32<ACC, OBS, UNI, 2, false>(P, s, 32u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, wc,
+                                                       gx, gy, m2);
                 uint32_t t = threadIdx.x;
                 __asm__ __volatile__("" : "+v"(t));   // a fresh LDS read, not m1 kept in a register
                 m1 = stash[t];
@@ -1674,14 +1718,14 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, in
             list_core_rows<OBS>(P, s, ci, g, pt, shb, m1, n1, m2, n > 32u ? n - 32u : 0u, gx, gy, ob, want_obs);
         } else {
             if (nr & kListGuard)
-                list_rows_f32<ACC, OBS, UNI, 1, false>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy,
-                                                       m1);
+                list_rows_f32<ACC, OBS, UNI, 1, false>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, wc,
+                                                       gx, gy, m1);
             else if (pure)
-                list_rows_f32<ACC, OBS, UNI, 0, true>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy,
-                                                      m1);
+                list_rows_f32<ACC, OBS, UNI, 0, true>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, wc,
+                                                      gx, gy, m1);
             else
-                list_rows_f32<ACC, OBS, UNI, 0, false>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, gx, gy,
-                                                       m1);
+                list_rows_f32<ACC, OBS, UNI, 0, false>(P, s, 0u, n, shb, ci, pt, pc, F, ob, want_obs, fxc, fyc, wc,
+                                                       gx, gy, m1);
         }
     } else {
         (void)shift;
@@ -1709,19 +1753,23 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, in
             for (int u = 0; u < U; ++u) {
                 if constexpr (ACC == ACC_COMP32) {
                     bool core = false;
-                    list_pair_f32<ACC, OBS, UNI, 2>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
-                                                    want_obs, core);
+                    list_pair_f32<ACC, OBS, UNI, 2>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, wc, F,
+                                                    ob, want_obs, core);
                     if (core)
                         pair_core64<OBS>(ps[u].x - q[u].x, ps[u].y - q[u].y, g.qinv,
                                          core_const(pt, (v[u] & kListSame) ? ci : 2), gx, gy, ob, want_obs);
                     if (++run == 8u) {
                         F.add(fxc, fyc);
                         fxc = fyc = 0.0f;
+                        if constexpr (OBS == SUMS_VIRIAL) {
+                            ob.w += 0.5 * g.qinv * static_cast<double>(wc);
+                            wc = 0.0f;
+                        }
                         run = 0;
                     }
                 } else {
-                    list_pair_f32<ACC, OBS, UNI, 1>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, F, ob,
-                                                    want_obs, slow);
+                    list_pair_f32<ACC, OBS, UNI, 1>(v[u], q[u], ps[u].x, ps[u].y, ci, pt, pc, fxc, fyc, wc, F,
+                                                    ob, want_obs, slow);
                 }
             }
             if constexpr (ACC != ACC_COMP32) {
@@ -1729,7 +1777,10 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, in
             }
         }
     }
-    if constexpr (ACC == ACC_COMP32) F.add(fxc, fyc);
+    if constexpr (ACC == ACC_COMP32) {
+        F.add(fxc, fyc);
+        if constexpr (OBS == SUMS_VIRIAL) ob.w += 0.5 * g.qinv * static_cast<double>(wc);
+    }
     F.addd(gx, gy);
 }
 
@@ -1737,7 +1788,7 @@ __device__ __forceinline__ void pair_list_f32(const BatchPtrs &P, uint32_t s, in
 // [k0, k1) of one neighbour cell shifted by (shx, shy) = +-L across the
 // periodic boundary; the oracle's operation order, so FP64 forces are
 // bit-identical to the oracle's.
-template <bool OBS>
+template <int OBS>
 __device__ __forceinline__ void pair_cell_f64(const double2 *__restrict__ img, const uint8_t *__restrict__ cls,
                                               uint32_t k0, uint32_t k1, uint32_t self, const double2 &pi, int ci,
                                               double shx, double shy, const Grid &g, const PairTable &pt,
@@ -1790,8 +1841,9 @@ __device__ inline void combine_lanes(ForceSum<ACC> &F, ObsSum &ob) {
 // Forces of every particle of every active slot (LPP lanes per cell-ordered
 // slot, 3x3 neighbour cells in the oracle's order: rows -1, 0, +1, cells -1,
 // 0, +1, ascending id), then the slot's mode: second kick, FIRE kick, walls,
-// and its reduction.  OBS = true compiles the energy/virial/lambda sums (used
-// when some slot needs them); UNI = true when every slot of the batch has one
+// and its reduction.  OBS (PairSums) compiles the energy/virial/lambda sums
+// or the virial alone, for the slots that take them (energy passes, NPH
+// steps); UNI = true when every slot of the batch has one
 // sigma for all type pairs.  A workgroup (kBlock * LPP threads) covers the
 // same kBlock particles whatever LPP is; LPP > 1 only for the FP32 pair forms.
 // LEAN: a rebuild window's lean step with step parity `lean_parity`, whose
@@ -1800,7 +1852,7 @@ __device__ inline void combine_lanes(ForceSum<ACC> &F, ObsSum &ob) {
 // cell list.  Other passes have parity 0.  (A compile-time LEAN keeps the
 // other passes' code free of the lean branch: as a runtime argument it cost
 // the cell pass about 13%.)
-template <typename IT, int ACC, bool OBS, bool UNI, int LPP, bool LEAN>
+template <typename IT, int ACC, int OBS, bool UNI, int LPP, bool LEAN>
 __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean_parity) {
     const int32_t parity = LEAN ? lean_parity : 0;
     using AT = std::conditional_t<ACC == ACC_COMP32, float2, double2>;
@@ -1814,7 +1866,7 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
     int pmode = sp.pmode;
     if (pmode == PM_FIRE && !sc.f_do_pair) pmode = PM_NONE;
     if (pmode == PM_NONE || sp.n <= 0) return;
-    const bool want_obs = OBS && (pmode == PM_OBS || pmode == PM_NPH);
+    const bool want_obs = OBS != SUMS_NONE && (pmode == PM_OBS || pmode == PM_NPH);
     const Grid g = sc.g;
     const PairTable &pt = sp.pt;
     const int li = (gb - sp.boff) * kBlock + static_cast<int>(threadIdx.x) / LPP;

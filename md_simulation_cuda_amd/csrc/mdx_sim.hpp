@@ -127,6 +127,8 @@ struct DevScalars {
     // a lean step with a particle beyond the reach of that 5x5 pass
     // ((2 w - rc_max) / 2), or a grid too small for it: its forces are not exact
     int32_t violated;
+    // an unloaded slot whose cells a step has cleared (zero when loaded)
+    int32_t cleared;
 };
 
 // Pointers of a batch, passed by value to every kernel.

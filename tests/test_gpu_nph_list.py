@@ -99,3 +99,35 @@ def test_nph_list_slot_in_a_batch_equals_alone():
         s.close()
         for k in ("pos", "vel", "acc"):
             assert np.array_equal(a[k], g[k]), k
+
+
+
+def _run(f, p, kind, steps, skin, cell_scale=1.0, energy_first=False, pe=0.0):
+    s = abi.Simulation(p, f)
+    s.set_tuning(cell_scale, 1, skin, 0)
+    if energy_first:
+        s.observables()
+    s.step(kind, steps, p_ext=pe)
+    a = s.collect()
+    s.close()
+    return a
+
+
+@pytest.mark.parametrize("kind,first", [(abi.NVT, True), (abi.NPH, False)])
+def test_list_mode_survives_an_energy_pass_before_the_first_step(kind, first):
+    """An energy pass before a slot's first step (observables() on a fresh
+    handle; an NPH run's initial virial pass) runs in list mode with the
+    slot's list parameters set, so list mode stays on for the run.  Compared
+    with the cell list on the same grid (cells of r_c + skin = 3.0: cell_scale
+    1.2 and no skin), whose pair sums group the terms by cell, the list run
+    differs in rounding only; a slot that had fallen back to the cell list
+    would equal it bit for bit."""
+    f, Lx, Ly = lattice_frame(4096, 0.75, seed=12, jitter=0.05, T=0.7)
+    p = abi.make_params(Lx, Ly, eps_ab=0.5, dt=0.005, T_target=0.7, Q=10.0, barostat_mass=50.0)
+    pe = 1.0 if kind == abi.NPH else 0.0
+    lists = _run(f, p, kind, 30, 0.5, energy_first=first, pe=pe)
+    cells = _run(f, p, kind, 30, 0.0, cell_scale=1.2, energy_first=first, pe=pe)
+    d = np.abs(lists["pos"] - cells["pos"]).max()
+    say(f"  lists vs cell list on the same grid: max |dx| {d:.2e}")
+    assert not np.array_equal(lists["acc"], cells["acc"])
+    assert d < 1e-6
