@@ -916,8 +916,8 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(multi-GPU slab decomposition is a design note only)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU slab decomposition is a design note only)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "ensemble":
         run_ensemble(args)

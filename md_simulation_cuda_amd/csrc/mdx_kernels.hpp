@@ -699,7 +699,7 @@ __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
         if (sp.list_mode) {
             P.x0[id] = p;
             P.key_list[id] = P.key[id];
-            if (li == 0) sc.lin_cum = 1.0;   // positions are unscaled since this build
+            if (li == 0) sc.lin_cum = 1.0;   // no barostat scaling since this list build
         }
     }
 }
