@@ -389,10 +389,13 @@ int mdx_batch_autotune(mdx_batch *b, int32_t steps, mdx_tune_report *out);
  * (final frame copied to the host, slot freed).  NVE, NVT and NPH jobs run in
  * a batch of md_precision with neighbour lists (skin list_skin, rebuilt on
  * demand), piston jobs in a batch of md_precision with the cell list every
- * step, FIRE jobs in an ACC64 batch; the batches' launches overlap on the
- * device.  A job's result is the same as the same system run alone through
- * mdx_create + mdx_step (NVE / NVT / NPH with mdx_set_tuning {1.0, list_skin,
- * 1, 0} in the FP32 forms) or mdx_fire with fp64_switch = 0. */
+ * step, FIRE jobs in an ACC64 batch (with neighbour lists of skin fire_skin,
+ * rebuilt on demand, when fire_skin > 0); the batches' launches overlap on the
+ * device.  A job's
+ * result is the same as the same system run alone through mdx_create +
+ * mdx_step (NVE / NVT / NPH with mdx_set_tuning {1.0, list_skin, 1, 0} in the
+ * FP32 forms) or mdx_fire with fp64_switch = 0 (with mdx_set_tuning {1.0,
+ * fire_skin, 1, 0}). */
 #define MDX_FIRE 4            /* mdx_job.kind for a FIRE minimisation */
 #define MDX_JOB_QUEUED 0
 #define MDX_JOB_RUNNING 1
@@ -415,6 +418,10 @@ typedef struct mdx_server_params {
     int32_t frame_queue;          /* streamed frames a job holds until taken (8); a
                                      further frame drops the oldest                      */
     double list_skin;             /* neighbour-list skin of those slots (0.5)           */
+    double fire_skin;             /* > 0: neighbour lists for the FIRE slots, rebuilt when
+                                     an iteration needs it (0: cell list every
+                                     iteration, the default: fast-moving FIRE jobs rebuild
+                                     nearly every iteration)                             */
 } mdx_server_params;
 
 typedef struct mdx_job {

@@ -119,7 +119,7 @@ class ServerParams(C.Structure):
         ("md_slots_per_class", C.c_int32), ("fire_slots_per_class", C.c_int32),
         ("quantum", C.c_int32), ("record_energies", C.c_int32), ("md_precision", C.c_int32),
         ("device", C.c_int32), ("list_slots_per_class", C.c_int32), ("frame_queue", C.c_int32),
-        ("list_skin", C.c_double),
+        ("list_skin", C.c_double), ("fire_skin", C.c_double),
     ]
 
 

@@ -1721,12 +1721,12 @@ int Batch::fire_poll(int r, int *done, mdx_fire_result *res) {
     res->converged = converged ? 1 : 0;
     res->max_force = h.f_maxF;
     if (finished) {
+        MDX_TRY(obs_pass(r, 1));   // :270, while the slot is a FIRE slot (its lists, as in Batch::fire)
+        MDX_TRY(read_scalars(r, h));
+        res->potential_energy = h.U;
         slots[r].kind = SK_NONE;
         set_idle(h_sp[r], slots[r]);
         MDX_TRY(upload_sp(r));
-        MDX_TRY(obs_pass(r, 1));   // :270
-        MDX_TRY(read_scalars(r, h));
-        res->potential_energy = h.U;
     }
     return MDX_OK;
 }
@@ -1791,7 +1791,9 @@ int Batch::fire(int r, const mdx_fire_params &fp, double ftol, int max_steps, md
     // FP64 accelerations (FP32 pairs with FP64 accumulation, or FP64 pairs).
     if (prec == MDX_PREC_COMP32) MDX_TRY(set_precision(MDX_PREC_ACC64));
     MDX_TRY(fire_begin(r, fp, ftol, max_steps));
-    s.kind = SK_NONE;   // this loop steps the slot itself, the others idle
+    // this loop steps the slot itself (the others idle); its kind stays
+    // SK_FIRE, so that its energy passes use the neighbour lists as its steps
+    // do and leave their rebuild points where a run without them has them
     DevScalars h;
     SysParams &sp = h_sp[r];
     int ns = 0, last_pushed = -1;

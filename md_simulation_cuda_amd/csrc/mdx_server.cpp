@@ -8,7 +8,8 @@
 // particles; for NPH the rebuild test follows the barostat's scaling), piston
 // jobs in a batch of that precision with the cell list every step, FIRE jobs
 // in an FP64-accelerating batch (FIRE needs accelerations beyond FP32 range
-// at overlapping starts).  Each tick:
+// at overlapping starts), with neighbour lists rebuilt on demand when
+// fire_skin > 0.  Each tick:
 //   1. admits queued jobs in arrival order into the smallest free slot that
 //      fits them, while the bytes of the admitted slots stay within the HBM
 //      budget (a job that does not fit now does not block later ones);
@@ -94,7 +95,8 @@ struct Pool {
 struct mdx_server {
     mdx_server_params p{};
     Pool pools[kPools];
-    bool lists = false;   // the NVE / NVT pool has neighbour lists
+    bool lists = false;        // the NVE / NVT / NPH pool has neighbour lists
+    bool fire_lists = false;   // the FIRE pool has neighbour lists
     std::map<int64_t, Job> jobs;
     std::deque<int64_t> queue;
     int64_t next_id = 1;
@@ -140,6 +142,11 @@ struct mdx_server {
                 rc = pl.b.set_tuning(mdx_tuning{1.0, p.list_skin, 1, 0});
                 if (rc != MDX_OK) return rc;
                 lists = true;
+            }
+            if (q == kPoolFire && p.fire_skin > 0.0) {   // rebuilt when an iteration needs it
+                rc = pl.b.set_tuning(mdx_tuning{1.0, p.fire_skin, 1, 0});
+                if (rc != MDX_OK) return rc;
+                fire_lists = true;
             }
         }
         return MDX_OK;
@@ -199,7 +206,8 @@ struct mdx_server {
                 ++it;
                 continue;
             }
-            const double bytes = slot_bytes(pl.cap[s], pl.ccap[s], pl.prec, pool == kPoolList && lists);
+            const double bytes = slot_bytes(pl.cap[s], pl.ccap[s], pl.prec,
+                                            (pool == kPoolList && lists) || (pool == kPoolFire && fire_lists));
             if (reserved + bytes > p.hbm_budget_bytes) {
                 ++it;
                 continue;
@@ -388,6 +396,7 @@ int mdx_server_default_params(mdx_server_params *out) {
     out->list_slots_per_class = 12;
     out->list_skin = 0.5;
     out->frame_queue = 8;
+    out->fire_skin = 0.0;
     return MDX_OK;
 }
 

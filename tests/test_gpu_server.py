@@ -47,11 +47,13 @@ def make(spec):
 LIST_SKIN = 0.5   # the server's default for its NVE / NVT / NPH slots
 
 
-def alone(spec, f, Lx, Ly, kw):
+def alone(spec, f, Lx, Ly, kw, fire_skin=0.0):
     kind, n, rho, T, eab, nsteps, pext, seed = spec
     s = abi.Simulation(abi.make_params(Lx, Ly, precision=abi.PREC_COMP32, **kw), f)
     if kind in (abi.NVE, abi.NVT, abi.NPH):
         s.set_tuning(1.0, 1, LIST_SKIN)   # as the server's NVE / NVT / NPH slots
+    else:
+        s.set_tuning(1.0, 1, fire_skin)   # as its FIRE slots
     if kind == abi.FIRE:
         res, _, _ = s.fire(FP, 1e-6, nsteps)
         out = s.collect(), res
@@ -62,9 +64,9 @@ def alone(spec, f, Lx, Ly, kw):
     return out
 
 
-def run_server(specs, frames, budget=None, stagger=True):
+def run_server(specs, frames, budget=None, stagger=True, fire_skin=0.0):
     kw = dict(min_class_log2=9, max_class_log2=12, md_slots_per_class=2, fire_slots_per_class=1, quantum=32,
-              list_slots_per_class=2, list_skin=LIST_SKIN)
+              list_slots_per_class=2, list_skin=LIST_SKIN, fire_skin=fire_skin)
     if budget is not None:
         kw["hbm_budget_bytes"] = budget
     srv = abi.Server(**kw)
@@ -115,6 +117,30 @@ def test_server_results_equal_alone_runs():
             assert len(rec) >= 1 and rec["step"][-1] == spec[5]
             assert np.all(np.diff(rec["step"]) > 0)
             assert info.last.U == ro.U and info.last.K == ro.K, i
+    srv.close()
+
+
+def test_server_fire_slots_with_lists_equal_alone_runs():
+    """FIRE slots with neighbour lists (fire_skin > 0, rebuilt when an
+    iteration needs it) beside an NVT job: each FIRE job equals mdx_fire with
+    the same tuning, including one long enough for the alone run's energy
+    sample at iteration 1000, which uses the lists and so leaves the rebuild
+    points where the server's run has them."""
+    specs = [(abi.FIRE, 700, 0.85, 0.0, 0.5, 150, 0.0, 24), (abi.NVT, 1500, 0.80, 0.7, 0.5, 120, 0.0, 21),
+             (abi.FIRE, 900, 0.80, 0.0, 0.7, 1300, 0.0, 31)]
+    frames = [make(s) for s in specs]
+    say("server with FIRE lists (skin 0.3)")
+    srv, ids = run_server(specs, frames, fire_skin=0.3)
+    for i, spec in enumerate(specs):
+        f, Lx, Ly, kw = frames[i]
+        info = srv.info(ids[i])
+        got = srv.result(ids[i])
+        ref, ro = alone(spec, f, Lx, Ly, kw, fire_skin=0.3)
+        say(f"job {i} kind={spec[0]} n={spec[1]} steps={spec[5]}")
+        assert np.array_equal(got["pos"], ref["pos"]) and np.array_equal(got["vel"], ref["vel"]), i
+        if spec[0] == abi.FIRE:
+            assert info.fire.steps == ro.steps and info.fire.max_force == ro.max_force
+            assert info.last.U == ro.potential_energy
     srv.close()
 
 
