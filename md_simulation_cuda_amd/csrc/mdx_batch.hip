@@ -115,7 +115,8 @@ Batch::~Batch() {
     if (stream) (void)hipStreamSynchronize(stream);
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
-                     d_sp, d_sc, d_blk2sys, d_cblk2sys, d_lb2sys, d_want, sorted_old, cs_old, movers, mover_off,
+                     d_sp, d_sc, d_blk2sys, d_cblk2sys, d_lb2sys, d_want, d_actblk, sorted_old, cs_old, movers,
+                     mover_off,
                      n_movers, arrivals, key_list, x0, img0, nlist, ncount, pid, sid, reorder_tmp};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
@@ -163,6 +164,29 @@ void Batch::invalidate_graphs() {
                 (void)hipGraphExecDestroy(g);
                 g = nullptr;
             }
+}
+
+// The per-particle workgroups of loaded slots (TA of them), for the step
+// kernels' launches: free slots and the unfilled end of a slot take no
+// workgroups.  Captured graphs hold the launch size, so a change of TA
+// invalidates them; the list itself is read at run time.
+int Batch::update_active() {
+    h_actblk.clear();
+    for (size_t r = 0; r < slots.size(); ++r) {
+        if (!slots[r].loaded) continue;
+        const int nb = ceil_div(slots[r].n, kBlock);
+        for (int b = 0; b < nb; ++b) h_actblk.push_back(h_sp[r].boff + b);
+    }
+    const int32_t ta = static_cast<int32_t>(h_actblk.size());
+    if (ta > 0) {
+        // ordered after the work in flight on the stream, which may still read the old list
+        MDX_HIP(hipMemcpyAsync(d_actblk, h_actblk.data(), h_actblk.size() * 4, hipMemcpyHostToDevice, stream));
+        MDX_HIP(hipStreamSynchronize(stream));
+    }
+    if (ta != TA) invalidate_graphs();
+    TA = ta;
+    ptrs.act_all = TA == TB ? 1 : 0;   // then the list is 0, 1, ..., TB - 1
+    return MDX_OK;
 }
 
 int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int precision, int device_ordinal) {
@@ -254,6 +278,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     MDX_HIP(hipMalloc(&d_blk2sys, std::max<size_t>(b2s.size(), 1) * 4));
     MDX_HIP(hipMalloc(&d_cblk2sys, std::max<size_t>(cb2s.size(), 1) * 4));
     MDX_HIP(hipMalloc(&d_lb2sys, std::max<size_t>(lb2s.size(), 1) * 4));
+    MDX_HIP(hipMalloc(&d_actblk, std::max<size_t>(b2s.size(), 1) * 4));
     MDX_HIP(hipMalloc(&d_want, nslots * 4));
     MDX_HIP(hipMalloc(&pid, n4));
     MDX_HIP(hipMalloc(&sid, n4));
@@ -288,6 +313,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     ptrs.sp = d_sp;
     ptrs.sc = d_sc;
     ptrs.blk2sys = d_blk2sys;
+    ptrs.actblk = d_actblk;
     ptrs.cblk2sys = d_cblk2sys;
     ptrs.lb2sys = d_lb2sys;
     ptrs.pid = pid;
@@ -438,30 +464,30 @@ int Batch::launch_step_kernels(int obs, int mode, int parity) {
     if (TLB > 0 && lean) {   // windows run in the FP32 forms only
         if (prec == MDX_PREC_COMP32) k_lean<float2><<<TLB, kLeanThreads, 0, stream>>>(ptrs, parity);
         else k_lean<double2><<<TLB, kLeanThreads, 0, stream>>>(ptrs, parity);
-    } else if (TB > 0 && !lean) {
+    } else if (TA > 0 && !lean) {
         if (prec == MDX_PREC_COMP32)
-            k_drift<float2><<<TB, kBlock, 0, stream>>>(ptrs, ahead);
+            k_drift<float2><<<TA, kBlock, 0, stream>>>(ptrs, ahead);
         else
-            k_drift<double2><<<TB, kBlock, 0, stream>>>(ptrs, ahead);
+            k_drift<double2><<<TA, kBlock, 0, stream>>>(ptrs, ahead);
     }
     if (!lean) {   // cell list (and, in list mode, neighbour lists) of the slots that rebuild
-        if (TCB + TB > 0) k_stay_arrive<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
+        if (TCB + TA > 0) k_stay_arrive<<<TCB + TA, kBlock, 0, stream>>>(ptrs, TCB);
         const int nchunk = ceil_div(C, kScanChunk);
         if (nchunk > 0) {
             k_scan_a<<<nchunk, kBlock, 0, stream>>>(ptrs, C, bsum);
             k_scan_b<<<nchunk, kBlock, 0, stream>>>(ptrs, C, bsum);
         }
-        if (TCB + TB > 0) k_fill<<<TCB + TB, kBlock, 0, stream>>>(ptrs, TCB);
+        if (TCB + TA > 0) k_fill<<<TCB + TA, kBlock, 0, stream>>>(ptrs, TCB);
         if (TCB > 0) k_cell_sort<<<TCB, kBlock, 0, stream>>>(ptrs);
     }
-    if (TB > 0 && !lean) {
+    if (TA > 0 && !lean) {
         if (prec == MDX_PREC_FP64)
-            k_slot_image<double2><<<TB, kBlock, 0, stream>>>(ptrs);
+            k_slot_image<double2><<<TA, kBlock, 0, stream>>>(ptrs);
         else
-            k_slot_image<int2><<<TB, kBlock, 0, stream>>>(ptrs);
-        if (nlist && prec != MDX_PREC_FP64) k_list_build<<<TB, kBlock, 0, stream>>>(ptrs);
+            k_slot_image<int2><<<TA, kBlock, 0, stream>>>(ptrs);
+        if (nlist && prec != MDX_PREC_FP64) k_list_build<<<TA, kBlock, 0, stream>>>(ptrs);
     }
-    if (TB > 0) {
+    if (TA > 0) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timing && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
             if (e0) (void)hipEventDestroy(e0);
@@ -484,33 +510,33 @@ void Batch::launch_pair(int obs, int parity, bool lean, hipEvent_t e0, hipEvent_
     switch (prec) {
     case MDX_PREC_COMP32:
         if (obs == SUMS_ENERGY) {
-            if (uni) launch_pair_f32<ACC_COMP32, SUMS_ENERGY, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
-            else launch_pair_f32<ACC_COMP32, SUMS_ENERGY, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            if (uni) launch_pair_f32<ACC_COMP32, SUMS_ENERGY, true>(lpp, TA, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_COMP32, SUMS_ENERGY, false>(lpp, TA, stream, ptrs, parity, ln, e0, e1);
         } else if (obs == SUMS_VIRIAL) {
-            if (uni) launch_pair_f32<ACC_COMP32, SUMS_VIRIAL, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
-            else launch_pair_f32<ACC_COMP32, SUMS_VIRIAL, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            if (uni) launch_pair_f32<ACC_COMP32, SUMS_VIRIAL, true>(lpp, TA, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_COMP32, SUMS_VIRIAL, false>(lpp, TA, stream, ptrs, parity, ln, e0, e1);
         } else {
-            if (uni) launch_pair_f32<ACC_COMP32, SUMS_NONE, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
-            else launch_pair_f32<ACC_COMP32, SUMS_NONE, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            if (uni) launch_pair_f32<ACC_COMP32, SUMS_NONE, true>(lpp, TA, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_COMP32, SUMS_NONE, false>(lpp, TA, stream, ptrs, parity, ln, e0, e1);
         }
         break;
     case MDX_PREC_ACC64:
         if (obs == SUMS_ENERGY) {
-            if (uni) launch_pair_f32<ACC_FP64, SUMS_ENERGY, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
-            else launch_pair_f32<ACC_FP64, SUMS_ENERGY, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            if (uni) launch_pair_f32<ACC_FP64, SUMS_ENERGY, true>(lpp, TA, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_FP64, SUMS_ENERGY, false>(lpp, TA, stream, ptrs, parity, ln, e0, e1);
         } else if (obs == SUMS_VIRIAL) {
-            if (uni) launch_pair_f32<ACC_FP64, SUMS_VIRIAL, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
-            else launch_pair_f32<ACC_FP64, SUMS_VIRIAL, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            if (uni) launch_pair_f32<ACC_FP64, SUMS_VIRIAL, true>(lpp, TA, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_FP64, SUMS_VIRIAL, false>(lpp, TA, stream, ptrs, parity, ln, e0, e1);
         } else {
-            if (uni) launch_pair_f32<ACC_FP64, SUMS_NONE, true>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
-            else launch_pair_f32<ACC_FP64, SUMS_NONE, false>(lpp, TB, stream, ptrs, parity, ln, e0, e1);
+            if (uni) launch_pair_f32<ACC_FP64, SUMS_NONE, true>(lpp, TA, stream, ptrs, parity, ln, e0, e1);
+            else launch_pair_f32<ACC_FP64, SUMS_NONE, false>(lpp, TA, stream, ptrs, parity, ln, e0, e1);
         }
         break;
     default:   // the FP64 form keeps one lane per particle (the oracle's order) and its FP64 energy sums
         if (obs != SUMS_NONE)
-            launch_k(k_pair<double2, ACC_FP64, SUMS_ENERGY, false, 1, false>, TB, kBlock, stream, e0, e1, ptrs, 0);
+            launch_k(k_pair<double2, ACC_FP64, SUMS_ENERGY, false, 1, false>, TA, kBlock, stream, e0, e1, ptrs, 0);
         else
-            launch_k(k_pair<double2, ACC_FP64, SUMS_NONE, false, 1, false>, TB, kBlock, stream, e0, e1, ptrs, 0);
+            launch_k(k_pair<double2, ACC_FP64, SUMS_NONE, false, 1, false>, TA, kBlock, stream, e0, e1, ptrs, 0);
         break;
     }
 }
@@ -524,7 +550,7 @@ void Batch::launch_pair(int obs, int parity, bool lean, hipEvent_t e0, hipEvent_
 // after, so the state is unchanged.
 int Batch::time_pair_pass(int reps, double *avg_ms) {
     if (reps < 1 || reps > 4096) return set_error(MDX_ERR_INVALID, "reps must be in [1, 4096]");
-    if (TB <= 0) return set_error(MDX_ERR_STATE, "no particles");
+    if (TA <= 0) return set_error(MDX_ERR_STATE, "no particles");
     MDX_HIP(hipSetDevice(device));
     MDX_TRY(flush());
     MDX_HIP(hipStreamSynchronize(stream));
@@ -804,6 +830,7 @@ int Batch::load(int r, const mdx_params &p, const mdx_particle64 *frame, int64_t
     MDX_HIP(hipMemcpyAsync(d_sc + r, &h, sizeof(h), hipMemcpyHostToDevice, stream));
     MDX_TRY(upload_sp(r));
     MDX_HIP(hipStreamSynchronize(stream));
+    MDX_TRY(update_active());
     return compute_forces({r}, 1, false);   // src/md_env.cu:66
 }
 
@@ -815,7 +842,8 @@ int Batch::unload(int r) {
     s.n = 0;
     h_sp[r].n = 0;
     set_idle(h_sp[r], s);
-    return upload_sp(r);
+    MDX_TRY(upload_sp(r));
+    return update_active();
 }
 
 int Batch::set_state(int r, const mdx_particle64 *frame, int64_t n) {

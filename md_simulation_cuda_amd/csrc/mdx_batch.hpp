@@ -69,6 +69,7 @@ struct Batch {
     int32_t TB = 0;    // per-particle workgroups
     int32_t TCB = 0;   // per-cell workgroups
     int32_t TLB = 0;   // lean-step workgroups (kLeanSpan slots each)
+    int32_t TA = 0;    // per-particle workgroups of loaded slots (BatchPtrs::actblk)
     std::vector<SysParams> h_sp;
     std::vector<int32_t> h_want;
     // device memory
@@ -84,6 +85,8 @@ struct Batch {
     SysParams *d_sp = nullptr;
     DevScalars *d_sc = nullptr;
     int32_t *d_blk2sys = nullptr, *d_cblk2sys = nullptr, *d_lb2sys = nullptr, *d_want = nullptr;
+    int32_t *d_actblk = nullptr;
+    std::vector<int32_t> h_actblk;
     BatchPtrs ptrs{};
     // captured step graphs, by the pair pass's sums (PairSums: none, energy
     // sums, virial)
@@ -181,6 +184,7 @@ struct Batch {
     int walls(const std::vector<int> &which);
     int obs_pass(int r, int periodic_y);
     void invalidate_graphs();
+    int update_active();   // BatchPtrs::actblk and TA after a load or unload
     bool uniform_sigma() const;   // every loaded slot has one sigma for all type pairs
     // a slot of this ensemble kind steps in list mode: a positive skin, list
     // arrays, an FP32 form and NVE or NVT (skin 0 is the cell list every step,

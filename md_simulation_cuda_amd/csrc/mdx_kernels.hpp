@@ -61,7 +61,7 @@ __device__ inline void finalize_pair(int pmode, const double (&out)[3], const Sy
 __device__ inline void finalize_pair_pending(const BatchPtrs &P, const SysParams &sp, DevScalars &sc) {
     const int pmode = sp.pmode;
     double out[3];
-    slot_reduce<3>(pmode == PM_FIRE ? 1u : 0u, P.partials, P.TB, sp.boff, sp.nblk, out);
+    slot_reduce<3>(pmode == PM_FIRE ? 1u : 0u, P.partials, P.TB, sp.boff, act_blocks(sp), out);
     if (threadIdx.x == 0) {
         finalize_pair(pmode, out, sp, sc.g, sc);
         sc.pair_pending = 0;
@@ -254,7 +254,7 @@ __device__ inline uint32_t canon_index(const BatchPtrs &P, const SysParams &sp, 
 // anyway makes its lean step fall back).
 template <typename AT>
 __global__ void __launch_bounds__(kBlock) k_drift(BatchPtrs P, int32_t ahead) {
-    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int gb = act_block(P, xcd_block(blockIdx.x, gridDim.x));
     const int r = P.blk2sys[gb];
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_b(BatchPtrs P, int ncell, const
 __device__ inline void finalize_drift(const BatchPtrs &P, const SysParams &sp, DevScalars &sc) {
     if (sp.n <= 0 || sp.nblk <= 0 || (sp.kind != SK_NVT && sp.kind != SK_NPH)) return;
     double out[1];
-    slot_reduce<1>(0u, P.partials, P.TB, sp.boff, sp.nblk, out);
+    slot_reduce<1>(0u, P.partials, P.TB, sp.boff, act_blocks(sp), out);
     if (threadIdx.x == 0) {
         sc.K_half = out[0];
         if (sp.kind == SK_NVT) {
@@ -490,7 +490,7 @@ constexpr int kCellUnroll = 8;
 // new cell (atomics on distinct cells).  The two parts touch different arrays.
 __global__ void __launch_bounds__(kBlock) k_stay_arrive(BatchPtrs P, int ncb) {
     if (static_cast<int>(blockIdx.x) >= ncb) {
-        const uint32_t b = blockIdx.x - ncb;
+        const uint32_t b = static_cast<uint32_t>(act_block(P, blockIdx.x - ncb));   // k_drift's workgroup
         if (!P.sc[P.blk2sys[b]].rebuild) return;
         const uint32_t nm = P.n_movers[b];
         if (threadIdx.x < nm) {
@@ -535,7 +535,7 @@ __global__ void __launch_bounds__(kBlock) k_stay_arrive(BatchPtrs P, int ncb) {
 // of their new cell, at their arrival index.
 __global__ void __launch_bounds__(kBlock) k_fill(BatchPtrs P, int ncb) {
     if (static_cast<int>(blockIdx.x) >= ncb) {
-        const uint32_t b = blockIdx.x - ncb;
+        const uint32_t b = static_cast<uint32_t>(act_block(P, blockIdx.x - ncb));   // k_drift's workgroup
         if (!P.sc[P.blk2sys[b]].rebuild) return;
         const uint32_t nm = P.n_movers[b];
         if (threadIdx.x < nm) {
@@ -655,7 +655,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_sort(BatchPtrs P) {
 // and its id for the next step's update.  Writes are coalesced.
 template <typename IT>
 __global__ void __launch_bounds__(kBlock) k_slot_image(BatchPtrs P) {
-    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int gb = act_block(P, xcd_block(blockIdx.x, gridDim.x));
     const int r = P.blk2sys[gb];
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
@@ -772,7 +772,7 @@ __global__ void __launch_bounds__(kLeanThreads) k_lean(BatchPtrs P, int32_t pari
     double xi1 = 0.0;
     if (kind == SK_NVT && sp.nblk > 0) {   // uniform per workgroup
         double out[3];
-        slot_reduce<3>(0u, P.partials, P.TB, sp.boff, sp.nblk, out);
+        slot_reduce<3>(0u, P.partials, P.TB, sp.boff, act_blocks(sp), out);
         const double xi = sc.xi_mid[parity ^ 1];
         xi1 = nh_half_update(xi, out[0], sp);
         if (first) {
@@ -912,7 +912,7 @@ constexpr uint32_t kListRows = ~(kListGuard | kListMixed);
 // closer than the FP64 guard distance only if it is closer than
 // guard + skin now; the wave's length carries kListGuard when one is.
 __global__ void __launch_bounds__(kBlock) k_list_build(BatchPtrs P) {
-    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int gb = act_block(P, xcd_block(blockIdx.x, gridDim.x));
     const int r = P.blk2sys[gb];
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
@@ -1859,7 +1859,7 @@ __global__ void __launch_bounds__(kBlock * LPP) k_pair(BatchPtrs P, int32_t lean
     constexpr bool kFp64 = std::is_same_v<IT, double2>;
     static_assert(!kFp64 || LPP == 1, "the FP64 form keeps the oracle's sequential order");
     static_assert(LPP == 1 || LPP == 2 || LPP == 4, "lanes per particle");
-    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int gb = act_block(P, xcd_block(blockIdx.x, gridDim.x));
     const int r = P.blk2sys[gb];
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
@@ -2075,12 +2075,12 @@ __global__ void __launch_bounds__(kBlock) k_finalize(BatchPtrs P, int32_t stage,
     } else if (stage == FIN_KINETIC) {
         if (!want[r]) return;
         double out[1];
-        slot_reduce<1>(0u, P.partials, P.TB, sp.boff, sp.nblk, out);
+        slot_reduce<1>(0u, P.partials, P.TB, sp.boff, act_blocks(sp), out);
         if (threadIdx.x == 0) sc.K = out[0];
     } else if (stage == FIN_WALLS) {
         if (!want[r]) return;
         double out[2];
-        slot_reduce<2>(0u, P.partials, P.TB, sp.boff, sp.nblk, out);
+        slot_reduce<2>(0u, P.partials, P.TB, sp.boff, act_blocks(sp), out);
         if (threadIdx.x == 0) {
             sc.F_p = out[0];
             sc.wallE = out[1];

@@ -131,6 +131,11 @@ struct DevScalars {
     int32_t cleared;
 };
 
+// The per-particle workgroups of a slot that hold its particles, the ones
+// the step kernels run (BatchPtrs::actblk) and whose reduction partials the
+// slot's sums combine.
+__host__ __device__ inline int act_blocks(const SysParams &sp) { return sp.n > 0 ? (sp.n + kBlock - 1) / kBlock : 0; }
+
 // Pointers of a batch, passed by value to every kernel.
 struct BatchPtrs {
     double2 *x, *v;
@@ -163,6 +168,13 @@ struct BatchPtrs {
     const SysParams *sp;
     DevScalars *sc;
     const int32_t *blk2sys;   // per-particle workgroup -> slot
+    // the per-particle workgroups that hold particles of loaded slots
+    // (ceil(n / kBlock) from each slot's first), in order: the step kernels
+    // launch one workgroup per entry and take their workgroup id from it
+    // (act_block); act_all: every workgroup is in it, in order, so the id is
+    // the launch index itself and the list is not read
+    const int32_t *actblk;
+    int32_t act_all;
     const int32_t *cblk2sys;  // per-cell workgroup -> slot
     const int32_t *lb2sys;    // lean-step workgroup -> slot
     int32_t TB;               // total per-particle workgroups
@@ -170,6 +182,10 @@ struct BatchPtrs {
     int32_t C;                // total cells
     int32_t N;                // total particle capacity (stride of nlist)
 };
+
+// Workgroup id of launch index i of a step kernel launched over the active
+// workgroups (BatchPtrs::actblk).
+__device__ inline int act_block(const BatchPtrs &P, int i) { return P.act_all ? i : P.actblk[i]; }
 
 // Canonical grid of a box (the oracle evaluates the same expressions).
 __host__ __device__ inline int grid_dim(double L, double side, bool periodic) {
