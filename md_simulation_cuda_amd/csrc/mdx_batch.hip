@@ -115,8 +115,8 @@ Batch::~Batch() {
     if (stream) (void)hipStreamSynchronize(stream);
     invalidate_graphs();
     void *ptrs_[] = {x, v, a, img, cls, icls, key, sorted, scell, count, cs, bsum, partials,
-                     d_sp, d_sc, d_blk2sys, d_cblk2sys, d_lb2sys, d_want, d_actblk, sorted_old, cs_old, movers,
-                     mover_off,
+                     d_sp, d_sc, d_blk2sys, d_cblk2sys, d_lb2sys, d_want, d_actblk, d_actcblk, sorted_old, cs_old,
+                     movers, mover_off,
                      n_movers, arrivals, key_list, x0, img0, nlist, ncount, pid, sid, reorder_tmp};
     for (void *p : ptrs_)
         if (p) (void)hipFree(p);
@@ -172,20 +172,26 @@ void Batch::invalidate_graphs() {
 // invalidates them; the list itself is read at run time.
 int Batch::update_active() {
     h_actblk.clear();
+    h_actcblk.clear();
     for (size_t r = 0; r < slots.size(); ++r) {
         if (!slots[r].loaded) continue;
         const int nb = ceil_div(slots[r].n, kBlock);
         for (int b = 0; b < nb; ++b) h_actblk.push_back(h_sp[r].boff + b);
+        const int ncb = ceil_div(h_sp[r].cbound, kBlock);
+        for (int b = 0; b < ncb; ++b) h_actcblk.push_back(h_sp[r].cboff + b);
     }
-    const int32_t ta = static_cast<int32_t>(h_actblk.size());
-    if (ta > 0) {
-        // ordered after the work in flight on the stream, which may still read the old list
+    const int32_t ta = static_cast<int32_t>(h_actblk.size()), tca = static_cast<int32_t>(h_actcblk.size());
+    // ordered after the work in flight on the stream, which may still read the old lists
+    if (ta > 0)
         MDX_HIP(hipMemcpyAsync(d_actblk, h_actblk.data(), h_actblk.size() * 4, hipMemcpyHostToDevice, stream));
-        MDX_HIP(hipStreamSynchronize(stream));
-    }
-    if (ta != TA) invalidate_graphs();
+    if (tca > 0)
+        MDX_HIP(hipMemcpyAsync(d_actcblk, h_actcblk.data(), h_actcblk.size() * 4, hipMemcpyHostToDevice, stream));
+    MDX_HIP(hipStreamSynchronize(stream));
+    if (ta != TA || tca != TCA) invalidate_graphs();
     TA = ta;
-    ptrs.act_all = TA == TB ? 1 : 0;   // then the list is 0, 1, ..., TB - 1
+    TCA = tca;
+    ptrs.act_all = TA == TB ? 1 : 0;     // then the list is 0, 1, ..., TB - 1
+    ptrs.act_call = TCA == TCB ? 1 : 0;
     return MDX_OK;
 }
 
@@ -279,6 +285,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     MDX_HIP(hipMalloc(&d_cblk2sys, std::max<size_t>(cb2s.size(), 1) * 4));
     MDX_HIP(hipMalloc(&d_lb2sys, std::max<size_t>(lb2s.size(), 1) * 4));
     MDX_HIP(hipMalloc(&d_actblk, std::max<size_t>(b2s.size(), 1) * 4));
+    MDX_HIP(hipMalloc(&d_actcblk, std::max<size_t>(cb2s.size(), 1) * 4));
     MDX_HIP(hipMalloc(&d_want, nslots * 4));
     MDX_HIP(hipMalloc(&pid, n4));
     MDX_HIP(hipMalloc(&sid, n4));
@@ -314,6 +321,7 @@ int Batch::init(int nslots, const int64_t *n_cap, const int32_t *ccap, int preci
     ptrs.sc = d_sc;
     ptrs.blk2sys = d_blk2sys;
     ptrs.actblk = d_actblk;
+    ptrs.actcblk = d_actcblk;
     ptrs.cblk2sys = d_cblk2sys;
     ptrs.lb2sys = d_lb2sys;
     ptrs.pid = pid;
@@ -471,14 +479,14 @@ int Batch::launch_step_kernels(int obs, int mode, int parity) {
             k_drift<double2><<<TA, kBlock, 0, stream>>>(ptrs, ahead);
     }
     if (!lean) {   // cell list (and, in list mode, neighbour lists) of the slots that rebuild
-        if (TCB + TA > 0) k_stay_arrive<<<TCB + TA, kBlock, 0, stream>>>(ptrs, TCB);
+        if (TCA + TA > 0) k_stay_arrive<<<TCA + TA, kBlock, 0, stream>>>(ptrs, TCA);
         const int nchunk = ceil_div(C, kScanChunk);
         if (nchunk > 0) {
             k_scan_a<<<nchunk, kBlock, 0, stream>>>(ptrs, C, bsum);
             k_scan_b<<<nchunk, kBlock, 0, stream>>>(ptrs, C, bsum);
         }
-        if (TCB + TA > 0) k_fill<<<TCB + TA, kBlock, 0, stream>>>(ptrs, TCB);
-        if (TCB > 0) k_cell_sort<<<TCB, kBlock, 0, stream>>>(ptrs);
+        if (TCA + TA > 0) k_fill<<<TCA + TA, kBlock, 0, stream>>>(ptrs, TCA);
+        if (TCA > 0) k_cell_sort<<<TCA, kBlock, 0, stream>>>(ptrs);
     }
     if (TA > 0 && !lean) {
         if (prec == MDX_PREC_FP64)
@@ -778,6 +786,15 @@ int Batch::load(int r, const mdx_params &p, const mdx_particle64 *frame, int64_t
     const Grid g = make_grid(p.Lx, p.Ly, side, rc_max, 1);
     if (static_cast<int64_t>(g.ncx) * g.ncy > s.ccap)
         return set_error(MDX_ERR_CAPACITY, "box needs more cells than the slot holds");
+    // the cells the grid may reach while this job is loaded: the largest box
+    // the barostat clamps allow (height up to barostat_height_max_ratio of
+    // the loaded one; width through a changed aspect ratio up to max / min of
+    // the ratios), at the smallest cell side a tuning sets (rc_max)
+    const double fy = std::max(1.0, p.barostat_height_max_ratio);
+    const double fx = fy / std::min(1.0, std::max(p.barostat_height_min_ratio, 1.0e-3));
+    const Grid gmax = make_grid(p.Lx * fx, p.Ly * fy, rc_max, rc_max, 1);
+    const int64_t cb = std::min<int64_t>(s.ccap, static_cast<int64_t>(gmax.ncx) * gmax.ncy);
+    sp.cbound = static_cast<int32_t>(std::max<int64_t>(cb, static_cast<int64_t>(g.ncx) * g.ncy));
     s.p = p;
     s.rc_max = rc_max;
     s.n = n;

@@ -70,6 +70,7 @@ struct Batch {
     int32_t TCB = 0;   // per-cell workgroups
     int32_t TLB = 0;   // lean-step workgroups (kLeanSpan slots each)
     int32_t TA = 0;    // per-particle workgroups of loaded slots (BatchPtrs::actblk)
+    int32_t TCA = 0;   // per-cell workgroups of loaded slots (BatchPtrs::actcblk)
     std::vector<SysParams> h_sp;
     std::vector<int32_t> h_want;
     // device memory
@@ -85,8 +86,8 @@ struct Batch {
     SysParams *d_sp = nullptr;
     DevScalars *d_sc = nullptr;
     int32_t *d_blk2sys = nullptr, *d_cblk2sys = nullptr, *d_lb2sys = nullptr, *d_want = nullptr;
-    int32_t *d_actblk = nullptr;
-    std::vector<int32_t> h_actblk;
+    int32_t *d_actblk = nullptr, *d_actcblk = nullptr;
+    std::vector<int32_t> h_actblk, h_actcblk;
     BatchPtrs ptrs{};
     // captured step graphs, by the pair pass's sums (PairSums: none, energy
     // sums, virial)

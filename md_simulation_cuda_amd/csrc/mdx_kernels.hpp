@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(kBlock) k_box(BatchPtrs P, int32_t mode) {
     sc.g_old = sc.g;
     Grid g = make_grid(Lx, Ly, sp.cell_side, sp.rc_max, sp.periodic_y);
     double side = sp.cell_side;
-    while (g.ncx * g.ncy > sp.ccap) {   // never for boxes within the allocation bound
+    for (int it = 0; it < 64 && g.ncx * g.ncy > sp.cbound; ++it) {   // never for boxes within the load-time bound
         side *= 1.25;
         g = make_grid(Lx, Ly, side, sp.rc_max, sp.periodic_y);
     }
@@ -499,7 +499,7 @@ __global__ void __launch_bounds__(kBlock) k_stay_arrive(BatchPtrs P, int ncb) {
         }
         return;
     }
-    const int gb = xcd_block(blockIdx.x, ncb);
+    const int gb = act_cblock(P, xcd_block(blockIdx.x, ncb));
     const int r = P.cblk2sys[gb];
     const SysParams &sp = P.sp[r];
     DevScalars &sc = P.sc[r];
@@ -546,7 +546,7 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchPtrs P, int ncb) {
         }
         return;
     }
-    const int gb = xcd_block(blockIdx.x, ncb);
+    const int gb = act_cblock(P, xcd_block(blockIdx.x, ncb));
     const int r = P.cblk2sys[gb];
     const SysParams &sp = P.sp[r];
     const DevScalars &sc = P.sc[r];
@@ -608,7 +608,7 @@ __device__ inline void sort_cell(uint32_t *a, uint32_t n, const uint32_t *pid, c
 // the arrivals in any order); every cell records its range for the next
 // step's update.
 __global__ void __launch_bounds__(kBlock) k_cell_sort(BatchPtrs P) {
-    const int gb = xcd_block(blockIdx.x, gridDim.x);
+    const int gb = act_cblock(P, xcd_block(blockIdx.x, gridDim.x));
     const int r = P.cblk2sys[gb];
     const SysParams &sp = P.sp[r];
     const Grid g = P.sc[r].g;

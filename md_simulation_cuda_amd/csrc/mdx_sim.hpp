@@ -60,6 +60,10 @@ struct SysParams {
     int32_t ccap;        // cell capacity
     int32_t boff, nblk;  // per-particle workgroups
     int32_t cboff, ncblk;
+    // the cells the slot's grid may use while its job is loaded (at most
+    // ccap): the cell-side step kernels cover these, and k_box keeps the grid
+    // within them
+    int32_t cbound;
     int32_t lboff, nlblk;  // lean-step workgroups (kLeanSpan cell-ordered slots each)
     int32_t kind;        // StepKind of the drift kernel for this slot
     int32_t pmode;       // PairMode of the pair kernel for this slot
@@ -175,6 +179,10 @@ struct BatchPtrs {
     // the launch index itself and the list is not read
     const int32_t *actblk;
     int32_t act_all;
+    // the same for the per-cell workgroups: those of loaded slots up to
+    // ceil(cbound / kBlock) from each slot's first
+    const int32_t *actcblk;
+    int32_t act_call;
     const int32_t *cblk2sys;  // per-cell workgroup -> slot
     const int32_t *lb2sys;    // lean-step workgroup -> slot
     int32_t TB;               // total per-particle workgroups
@@ -186,6 +194,7 @@ struct BatchPtrs {
 // Workgroup id of launch index i of a step kernel launched over the active
 // workgroups (BatchPtrs::actblk).
 __device__ inline int act_block(const BatchPtrs &P, int i) { return P.act_all ? i : P.actblk[i]; }
+__device__ inline int act_cblock(const BatchPtrs &P, int i) { return P.act_call ? i : P.actcblk[i]; }
 
 // Canonical grid of a box (the oracle evaluates the same expressions).
 __host__ __device__ inline int grid_dim(double L, double side, bool periodic) {
