@@ -414,7 +414,7 @@ typedef struct mdx_server_params {
     int32_t md_precision;         /* mdx_precision of the MD batches (COMP32)            */
     int32_t device;               /* -1: current                                         */
     int32_t list_slots_per_class; /* NVE / NVT / NPH slots per size class, in a batch
-                                     with neighbour lists (12; 0: they use the MD slots) */
+                                     with neighbour lists (16; 0: they use the MD slots) */
     int32_t frame_queue;          /* streamed frames a job holds until taken (8); a
                                      further frame drops the oldest                      */
     double list_skin;             /* neighbour-list skin of those slots (0.5)           */

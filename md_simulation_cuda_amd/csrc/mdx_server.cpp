@@ -393,7 +393,7 @@ int mdx_server_default_params(mdx_server_params *out) {
     out->min_density = 0.3;
     out->max_box_growth = 1.4;
     out->device = -1;
-    out->list_slots_per_class = 12;
+    out->list_slots_per_class = 16;
     out->list_skin = 0.5;
     out->frame_queue = 8;
     out->fire_skin = 0.0;
